@@ -1,0 +1,375 @@
+"""MI355X-native EM-PIC time step: Python binding of libpic_amd.so.
+
+The product is the C ABI in include/pic_amd.h, implemented by hand-written
+HIP kernels for gfx950.  This module is a thin ctypes layer over it (numpy in,
+numpy out) plus the seeded beam-plasma workloads used by tests and bench.py.
+There is no CPU fallback: every compute call goes through the HIP library,
+and loading fails loudly when the library is missing.
+
+Import with importlib (the package name starts with a digit):
+    pic = importlib.import_module("0buf_amd")
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpic_amd.so")
+
+# enum pic_field (include/pic_amd.h)
+EX, EY, EZ, HX, HY, HZ, JX, JY, JZ, QX, QY, QZ = range(12)
+FIELD_NAMES = ["Ex", "Ey", "Ez", "Hx", "Hy", "Hz", "Jx", "Jy", "Jz", "Qx", "Qy", "Qz"]
+ION, PLASMA_ELECTRON, BEAM_ELECTRON = 0, 1, 2
+FP64, FP32 = 0, 1
+
+HOST_TO_DEVICE, HOST_TO_HOST, DEVICE_TO_HOST, DEVICE_TO_DEVICE = (
+    -131313, -131314, -131315, -131316)
+
+STATUS = {0: "PIC_SUCCESS", 1: "PIC_ERR_INVALID", 2: "PIC_ERR_HIP",
+          3: "PIC_ERR_NO_DEVICE", 4: "PIC_ERR_TOO_MANY_PARTICLES",
+          5: "PIC_ERR_QUEUE_OVERFLOW", 6: "PIC_ERR_NOMEM", 7: "PIC_ERR_STATE",
+          8: "PIC_ERR_CELL_RANGE"}
+
+
+class PicParams(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double),
+                ("ni", C.c_double), ("n_per_cell", C.c_int),
+                ("ion_q_m", C.c_double), ("tau", C.c_double),
+                ("precision", C.c_int), ("departure_cap", C.c_int),
+                ("bucket_capacity", C.c_int), ("device", C.c_int)]
+
+
+class PicStepStatus(C.Structure):
+    _fields_ = [("departures", C.c_longlong), ("dropped", C.c_longlong),
+                ("duplicated", C.c_longlong), ("max_cell_departures", C.c_int),
+                ("bucket_regrows", C.c_int), ("bucket_capacity", C.c_int),
+                ("flags", C.c_int)]
+
+
+class PicRng(C.Structure):
+    _fields_ = [("ix", C.c_int), ("iy", C.c_int), ("iz", C.c_int),
+                ("store1", C.c_double)]
+
+
+class PicSpeciesSpec(C.Structure):
+    _fields_ = [("sort", C.c_int), ("ppc", C.c_int), ("density", C.c_double),
+                ("charge", C.c_double), ("q_m", C.c_double),
+                ("drift", C.c_double * 3), ("vth", C.c_double * 3),
+                ("profile", C.c_int), ("sigma_cells", C.c_double),
+                ("perturb", C.c_double * 8)]
+
+
+EXPORTS = [
+    # archAPI (archAPI.h:92-125)
+    "MemoryCopy", "MemoryAllocate", "MemoryFree", "GetDeviceMemory", "MemorySet",
+    "ThreadSynchronize", "DeviceSynchronize", "getLastError", "getErrorString",
+    "SetDevice",
+    # context and data
+    "pic_params_default", "pic_create", "pic_destroy", "pic_error_string",
+    "pic_last_hip_error", "pic_load_particles", "pic_load_field", "pic_get_field",
+    # step and stages
+    "pic_step", "pic_run", "pic_yee_h_half", "pic_push_deposit", "pic_migrate",
+    "pic_fold_j", "pic_yee_h2", "pic_yee_e", "pic_e_periodic", "pic_energy",
+    "pic_sync",
+    # readback
+    "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
+    "pic_get_particles", "pic_moments",
+    # seeded inputs
+    "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
+    "pic_generate_species",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libpic_amd.so and declare its signatures.  Raises if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it with `python 0buf_amd/build.py` "
+            "(there is no CPU fallback for the PIC step)")
+    lib = C.CDLL(path)
+    dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
+    vp = C.c_void_p
+    sig = {
+        "MemoryCopy": (C.c_int, [vp, vp, C.c_size_t, C.c_int]),
+        "MemoryAllocate": (C.c_int, [C.POINTER(vp), C.c_size_t]),
+        "MemoryFree": (C.c_int, [vp]),
+        "GetDeviceMemory": (C.c_int, [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+        "MemorySet": (C.c_int, [vp, C.c_int, C.c_size_t]),
+        "ThreadSynchronize": (C.c_int, []),
+        "DeviceSynchronize": (C.c_int, []),
+        "getLastError": (C.c_int, []),
+        "getErrorString": (C.c_char_p, [C.c_int]),
+        "SetDevice": (C.c_int, [C.c_int]),
+        "pic_params_default": (None, [C.POINTER(PicParams)]),
+        "pic_create": (C.c_int, [C.POINTER(PicParams), C.POINTER(vp)]),
+        "pic_destroy": (C.c_int, [vp]),
+        "pic_error_string": (C.c_char_p, [C.c_int]),
+        "pic_last_hip_error": (C.c_char_p, []),
+        "pic_load_particles": (C.c_int, [vp, C.c_int, dp, dp, dp, dp, dp, dp, C.c_long,
+                                         C.c_double, C.c_double, lp]),
+        "pic_load_field": (C.c_int, [vp, C.c_int, dp]),
+        "pic_get_field": (C.c_int, [vp, C.c_int, dp]),
+        "pic_step": (C.c_int, [vp, C.c_int, dp]),
+        "pic_run": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+        "pic_yee_h_half": (C.c_int, [vp]),
+        "pic_push_deposit": (C.c_int, [vp]),
+        "pic_migrate": (C.c_int, [vp]),
+        "pic_fold_j": (C.c_int, [vp]),
+        "pic_yee_h2": (C.c_int, [vp]),
+        "pic_yee_e": (C.c_int, [vp]),
+        "pic_e_periodic": (C.c_int, [vp]),
+        "pic_energy": (C.c_int, [vp, dp]),
+        "pic_sync": (C.c_int, [vp]),
+        "pic_get_status": (C.c_int, [vp, C.POINTER(PicStepStatus)]),
+        "pic_particle_count": (C.c_long, [vp]),
+        "pic_get_cell_counts": (C.c_int, [vp, ip]),
+        "pic_get_particles": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip,
+                                        C.c_long, lp]),
+        "pic_moments": (C.c_int, [vp, dp]),
+        "pic_rng_seed": (None, [C.POINTER(PicRng), C.c_int, C.c_int, C.c_int]),
+        "pic_rng_uniform": (C.c_double, [C.POINTER(PicRng)]),
+        "pic_rng_normal": (C.c_double, [C.POINTER(PicRng), C.c_double, C.c_double]),
+        "pic_species_count": (C.c_long, [C.POINTER(PicParams), C.POINTER(PicSpeciesSpec)]),
+        "pic_generate_species": (C.c_int, [C.POINTER(PicParams), C.POINTER(PicSpeciesSpec),
+                                           C.POINTER(PicRng), C.c_int, C.c_ulonglong,
+                                           C.c_int, dp, dp, dp, dp, dp, dp, C.c_long, lp,
+                                           dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double)) if a is not None else None
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int)) if a is not None else None
+
+
+class PicError(RuntimeError):
+    def __init__(self, code, where):
+        lib = load_library()
+        msg = lib.pic_error_string(code).decode()
+        hip = lib.pic_last_hip_error().decode()
+        super().__init__(f"{where}: {STATUS.get(code, code)} ({msg}) {hip}")
+        self.code = code
+
+
+def _check(code, where):
+    if code != 0:
+        raise PicError(code, where)
+
+
+def default_params(**kw):
+    lib = load_library()
+    p = PicParams()
+    lib.pic_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Pic:
+    """One simulation context on one GPU (owns all its device memory)."""
+
+    def __init__(self, params):
+        self.lib = load_library()
+        self.params = params
+        self._ctx = C.c_void_p()
+        _check(self.lib.pic_create(C.byref(params), C.byref(self._ctx)), "pic_create")
+        p = params
+        self.G = (p.nx + 2) * (p.ny + 2) * (p.nz + 2)
+        self.shape = (p.nx + 2, p.ny + 2, p.nz + 2)
+        self.ncell = p.nx * p.ny * p.nz
+
+    def close(self):
+        if self._ctx:
+            self.lib.pic_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- data in/out -------------------------------------------------------
+    def load_particles(self, sort, x, y, z, px, py, pz, m, q_m):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, z, px, py, pz)]
+        n = len(arrs[0])
+        dropped = C.c_long(0)
+        _check(self.lib.pic_load_particles(self._ctx, sort, *[_dptr(a) for a in arrs], n,
+                                           float(m), float(q_m), C.byref(dropped)),
+               "pic_load_particles")
+        return dropped.value
+
+    def load_field(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64).reshape(-1)
+        assert a.size == self.G
+        _check(self.lib.pic_load_field(self._ctx, which, _dptr(a)), "pic_load_field")
+
+    def get_field(self, which):
+        a = np.empty(self.G, dtype=np.float64)
+        _check(self.lib.pic_get_field(self._ctx, which, _dptr(a)), "pic_get_field")
+        return a
+
+    def fields(self):
+        return {n: self.get_field(i) for i, n in enumerate(FIELD_NAMES)}
+
+    # -- stepping ----------------------------------------------------------
+    def step(self, nt=1):
+        e = C.c_double(0.0)
+        _check(self.lib.pic_step(self._ctx, nt, C.byref(e)), "pic_step")
+        return e.value
+
+    def run(self, nt0, nsteps):
+        out = np.zeros(nsteps, dtype=np.float64)
+        _check(self.lib.pic_run(self._ctx, nt0, nsteps, _dptr(out)), "pic_run")
+        return out
+
+    def stage(self, name):
+        fn = getattr(self.lib, "pic_" + name)
+        _check(fn(self._ctx), "pic_" + name)
+
+    def energy(self):
+        e = C.c_double(0.0)
+        _check(self.lib.pic_energy(self._ctx, C.byref(e)), "pic_energy")
+        return e.value
+
+    def sync(self):
+        _check(self.lib.pic_sync(self._ctx), "pic_sync")
+
+    # -- readback ----------------------------------------------------------
+    def status(self):
+        st = PicStepStatus()
+        _check(self.lib.pic_get_status(self._ctx, C.byref(st)), "pic_get_status")
+        return {f: getattr(st, f) for f, _ in PicStepStatus._fields_}
+
+    def particle_count(self):
+        return int(self.lib.pic_particle_count(self._ctx))
+
+    def cell_counts(self):
+        a = np.empty(self.ncell, dtype=np.int32)
+        _check(self.lib.pic_get_cell_counts(self._ctx, _iptr(a)), "pic_get_cell_counts")
+        return a
+
+    def particles(self):
+        n = self.particle_count()
+        cap = max(n, 1)
+        out = {k: np.empty(cap, dtype=np.float64) for k in
+               ("x", "y", "z", "px", "py", "pz", "m", "q_m")}
+        sort = np.empty(cap, dtype=np.int32)
+        cell = np.empty(cap, dtype=np.int32)
+        nout = C.c_long(0)
+        _check(self.lib.pic_get_particles(
+            self._ctx, *[_dptr(out[k]) for k in ("x", "y", "z", "px", "py", "pz", "m", "q_m")],
+            _iptr(sort), _iptr(cell), cap, C.byref(nout)), "pic_get_particles")
+        n = nout.value
+        res = {k: v[:n] for k, v in out.items()}
+        res["sort"] = sort[:n]
+        res["cell"] = cell[:n]
+        return res
+
+    def moments(self):
+        a = np.empty(27, dtype=np.float64)
+        _check(self.lib.pic_moments(self._ctx, _dptr(a)), "pic_moments")
+        return a.reshape(3, 9)
+
+
+# ---------------------------------------------------------------------------
+# Seeded beam-plasma workloads (units c = 1, w_pe = 1; cell size 0.1)
+# ---------------------------------------------------------------------------
+
+def species_spec(sort, ppc, density, charge, q_m, drift=(0, 0, 0), vth=(0, 0, 0),
+                 profile=0, sigma_cells=0.0, perturb=None):
+    s = PicSpeciesSpec()
+    s.sort, s.ppc, s.density, s.charge, s.q_m = sort, ppc, density, charge, q_m
+    for i in range(3):
+        s.drift[i] = drift[i]
+        s.vth[i] = vth[i]
+    s.profile, s.sigma_cells = profile, sigma_cells
+    for i, a in enumerate(perturb or []):
+        s.perturb[i] = a
+    return s
+
+
+def beam_plasma_config(index):
+    """The five beam-plasma workloads: grid, steps, species specs."""
+    h = 0.1
+    if index == 0:
+        n, ppc, steps = (32, 8, 8), (16, 16, 16), 100
+    elif index == 1:
+        n, ppc, steps = (256, 64, 64), (16, 32, 16), 200
+    elif index == 2:
+        n, ppc, steps = (512, 128, 128), (24, 24, 24), 500
+    elif index == 3:
+        n, ppc, steps = (64, 16, 16), (8, 16, 8), 200
+    elif index == 4:
+        n, ppc, steps = (256, 64, 64), (16, 16, 16), 400
+    else:
+        raise ValueError(index)
+    nb = 0.05 if index == 4 else 0.01
+    ions = species_spec(ION, ppc[0], 1.0 + nb, +1.0, 1.0 / 1836.0)
+    elec = species_spec(PLASMA_ELECTRON, ppc[1], 1.0, -1.0, -1.0, vth=(0.02, 0.02, 0.02))
+    beam = species_spec(BEAM_ELECTRON, ppc[2], nb, -1.0, -1.0, drift=(0.3, 0.0, 0.0),
+                        vth=(0.005, 0.005, 0.005),
+                        profile=1 if index == 4 else 0, sigma_cells=8.0)
+    return {"n": n, "l": tuple(k * h for k in n), "tau": 0.05, "steps": steps,
+            "species": [ions, elec, beam], "bx0": 0.5 if index == 2 else 0.0}
+
+
+def make_params(n, l, tau, precision=FP64, **kw):
+    return default_params(nx=n[0], ny=n[1], nz=n[2], lx=l[0], ly=l[1], lz=l[2], tau=tau,
+                          precision=precision, **kw)
+
+
+def generate_species(params, spec, rng=None, stream_mode=0, seed=0, nthreads=1):
+    """Returns (x, y, z, px, py, pz, m) for one species (host generator)."""
+    lib = load_library()
+    n = int(lib.pic_species_count(C.byref(params), C.byref(spec)))
+    arrs = [np.empty(max(n, 1), dtype=np.float64) for _ in range(6)]
+    nout, m = C.c_long(0), C.c_double(0)
+    _check(lib.pic_generate_species(C.byref(params), C.byref(spec),
+                                    C.byref(rng) if rng is not None else None, stream_mode,
+                                    seed, nthreads, *[_dptr(a) for a in arrs], n,
+                                    C.byref(nout), C.byref(m)), "pic_generate_species")
+    return [a[:n] for a in arrs] + [m.value]
+
+
+def generate_workload(index_or_cfg, stream_mode=None, seed=12345, nthreads=8, scale_n=None):
+    """Seeded particles of a workload: list of (sort, x,y,z,px,py,pz, m, q_m).
+
+    stream_mode 0 (one Wichmann-Hill stream from seeds 1,255,25555 in cell
+    order, species in enum order) for small grids; 1 (a stream per cell) for
+    large ones.  scale_n overrides the grid (same cell size)."""
+    cfg = beam_plasma_config(index_or_cfg) if isinstance(index_or_cfg, int) else index_or_cfg
+    n = scale_n or cfg["n"]
+    l = tuple(k * 0.1 for k in n)
+    params = make_params(n, l, cfg["tau"])
+    if stream_mode is None:
+        stream_mode = 0 if n[0] * n[1] * n[2] <= 65536 else 1
+    rng = PicRng()
+    load_library().pic_rng_seed(C.byref(rng), 1, 255, 25555)
+    out = []
+    for spec in cfg["species"]:
+        x, y, z, px, py, pz, m = generate_species(params, spec, rng, stream_mode, seed, nthreads)
+        out.append((spec.sort, x, y, z, px, py, pz, m, spec.q_m))
+    return params, out

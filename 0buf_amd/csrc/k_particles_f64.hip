@@ -1,0 +1,705 @@
+/*
+ * k_particles_f64.hip -- particle stages of the FP64 plain form.
+ *
+ * One workgroup per tile of TX x TY x TZ cells:
+ *   1. stage E and H^{n+1/2} of the tile's node box ((TX+2)(TY+2)(TZ+2)
+ *      nodes: a cell's stencil reaches global nodes owner+0..owner+2) in LDS;
+ *   2. every lane takes particles of the tile in flat order (consecutive
+ *      lanes -> consecutive slots of a cell bucket, so loads coalesce),
+ *      gathers, pushes, and deposits its charge-conserving current into an
+ *      LDS copy of the tile's J box (ds_add_f64);
+ *   3. the particle is wrapped, its fly direction and receiving cells are
+ *      computed with the reference predicates; stayers are written back in
+ *      place, leavers go to the tile's departure region and leave a hole;
+ *   4. holes are filled from the tail of each bucket (one lane per cell);
+ *   5. the J box is added to the global arrays with FP64 atomics.
+ * The arithmetic mirrors cell.h / particle.h in the reference's evaluation
+ * order and is built with -ffp-contract=off.
+ */
+#include "pic_internal.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int TXC = 2, TYC = 4, TZC = 8;
+
+struct PushArgs {
+    Grid g;
+    TileGeom t;
+    FieldPtrs f;
+    PartPtrs p;
+    QueuePtrs q;
+    SpeciesTable sp;
+    int dep_cap;
+    DevStatus *st;
+};
+
+/* pow(v, -0.5) (particle.h:73,116) correctly rounded: r = 1/sqrt(v) refined
+ * by one Newton step whose residual 1 - v r^2 is formed exactly with FMAs. */
+__device__ __forceinline__ double pow_m05(double v) {
+    double s = sqrt(v);
+    double r = 1.0 / s;
+    double rr = r * r;
+    double rr_lo = fma(r, r, -rr);
+    double e = fma(-v, rr, 1.0);
+    e = fma(-v, rr_lo, e);
+    return fma(0.5 * r, e, r);
+}
+
+/* cell.h:527-566 */
+__device__ __forceinline__ int cell_number(double x, double x0, double h) {
+    return ((int)((x - x0) / h + 1.0) + 1);
+}
+__device__ __forceinline__ int cell_number_center(double x, double x0, double h) {
+    double t = (((x - x0) / h + 1.0) + 1.5);
+    return (int)(t);
+}
+__device__ __forceinline__ int cell_number_center_current(double x, double x0, double h) {
+    double t = ((x - x0) / h + 1.5);
+    return (int)(t + 1);
+}
+/* cell.h:1193-1236 */
+__device__ __forceinline__ double s_node(double x, double h) {
+    return (int)(x / h + 1.0) - x / h;
+}
+__device__ __forceinline__ double s_center(double x, double h) {
+    return (int)(x / h + 1.5) - 0.5 - x / h;
+}
+/* cell.h:1427-1478 */
+__device__ __forceinline__ double reflect1(double x, double xm) {
+    return (x > xm) * (x - xm) + (x < 0.0) * (xm + x) + (x > 0 && x < xm) * x;
+}
+
+template <int BY, int BZ>
+__device__ __forceinline__ double interp(const double *F, int u, int v, int w, double sx,
+                                         double sx1, double sy, double sy1, double sz,
+                                         double sz1) {
+    /* cell.h:1054-1153 Interpolate3D: t2 + t1 */
+    const double *p = F + (u * BY + v) * BZ + w;
+    double t_ilk = p[0], t_ilk1 = p[1], t_il1k = p[BZ], t_il1k1 = p[BZ + 1];
+    double t_i1lk = p[BY * BZ], t_i1lk1 = p[BY * BZ + 1];
+    double t_i1l1k = p[BY * BZ + BZ], t_i1l1k1 = p[BY * BZ + BZ + 1];
+    double t1 = sx * (sy * (sz * t_ilk + sz1 * t_ilk1) + sy1 * (sz * t_il1k + sz1 * t_il1k1));
+    double t2 = sx1 * (sy * (sz * t_i1lk + sz1 * t_i1lk1) + sy1 * (sz * t_i1l1k + sz1 * t_i1l1k1));
+    return t2 + t1;
+}
+
+struct Geo {
+    double hx, hy, hz, x0, y0, z0, tau;
+    int ca, cb, cc;   /* owner cell inside the tile */
+};
+
+/* cell.h:1511-1693 pqr: 12 node contributions of one straight segment,
+ * accumulated into the LDS J box at local node (owner + tile index - 2). */
+template <int BY, int BZ, int NB>
+__device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, int ix, int iy, int iz,
+                                        double xx, double xy, double xz, double x1x, double x1y,
+                                        double x1z, double a, unsigned &bad) {
+    if (ix < 2 || ix > 3 || iy < 2 || iy > 3 || iz < 2 || iz > 3) {
+        bad = 1;
+        return;
+    }
+    double dx = (0.50 * (xx + x1x) - c.hx * ((double)ix - 2.50) - c.x0);
+    double dy = (0.50 * (xy + x1y) - c.hy * ((double)iy - 2.50) - c.y0);
+    double dz = (0.50 * (xz + x1z) - c.hz * ((double)iz - 2.50) - c.z0);
+    double dx1 = c.hx - dx, dy1 = c.hy - dy, dz1 = c.hz - dz;
+    double su = x1x - xx, sv = x1y - xy, sw = x1z - xz;
+    double s1 = sv * sw / 12.0, s2 = su * sw / 12.0, s3 = su * sv / 12.0;
+    su = su * a / (c.tau * c.hy * c.hz);
+    sv = sv * a / (c.tau * c.hx * c.hz);
+    sw = sw * a / (c.tau * c.hx * c.hy);
+    const int u = c.ca + ix - 2, v = c.cb + iy - 2, w = c.cc + iz - 2;
+    const int n = (u * BY + v) * BZ + w;
+    double *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
+    atomicAdd(&Jx[n], su * (dy1 * dz1 + s1));
+    atomicAdd(&Jx[n + 1], su * (dy1 * dz - s1));
+    atomicAdd(&Jx[n + BZ], su * (dy * dz1 - s1));
+    atomicAdd(&Jx[n + BZ + 1], su * (dy * dz + s1));
+    atomicAdd(&Jy[n], sv * (dx1 * dz1 + s2));
+    atomicAdd(&Jy[n + 1], sv * (dx1 * dz - s2));
+    atomicAdd(&Jy[n + BY * BZ], sv * (dx * dz1 - s2));
+    atomicAdd(&Jy[n + BY * BZ + 1], sv * (dx * dz + s2));
+    atomicAdd(&Jz[n], sw * (dx1 * dy1 + s3));
+    atomicAdd(&Jz[n + BZ], sw * (dx1 * dy - s3));
+    atomicAdd(&Jz[n + BY * BZ], sw * (dx * dy1 - s3));
+    atomicAdd(&Jz[n + BY * BZ + BZ], sw * (dx * dy + s3));
+}
+
+__device__ __forceinline__ double transit_average(double h, int i1, int i2, double x0) {
+    return (h * (((double)i1 + (double)i2) * 0.5 - 2.0) + x0);
+}
+
+/* smallest pos in [from,to) whose bit (base+pos) equals `want`, else `to` */
+__device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from, int to,
+                                        bool want) {
+    int pos = from;
+    while (pos < to) {
+        int gbit = base + pos;
+        unsigned w = mask[gbit >> 5];
+        if (!want) w = ~w;
+        w >>= (gbit & 31);
+        if (w) {
+            int p = pos + __builtin_ctz(w);
+            return p < to ? p : to;
+        }
+        pos += 32 - (gbit & 31);
+    }
+    return to;
+}
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
+    constexpr int C = TX * TY * TZ;
+    constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
+    __shared__ double sF[6 * NB];
+    __shared__ double sJ[3 * NB];
+    __shared__ int sOff[C + 1];
+    __shared__ int sCnt[C];
+    __shared__ int sLeave[C];
+    __shared__ int sOut[C];
+    __shared__ int sQ;
+    __shared__ unsigned sFlags;
+    __shared__ unsigned long long sDrop, sDup;
+    extern __shared__ unsigned sMask[];
+
+    const Grid &g = a.g;
+    const int tid = threadIdx.x;
+
+    /* XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin,
+     * so each XCD gets a contiguous run of neighbouring tiles and their
+     * shared halo nodes stay in that XCD's L2. */
+    long tile;
+    {
+        long nt = a.t.ntiles, b = blockIdx.x;
+        long q = nt / 8, r = nt % 8, x = b % 8;
+        tile = x * q + (x < r ? x : r) + b / 8;
+    }
+    const int tz = (int)(tile % a.t.ntz);
+    const int ty = (int)((tile / a.t.ntz) % a.t.nty);
+    const int tx = (int)(tile / ((long)a.t.ntz * a.t.nty));
+    const int I0 = tx * TX, L0 = ty * TY, K0 = tz * TZ;
+    const int cap = a.p.cap;
+
+    for (int c = tid; c < C; c += PIC_BLOCK) {
+        int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        int cnt = 0;
+        if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+            cnt = a.p.count[((long)gi * g.Ny + gl) * g.Nz + gk];
+            cnt = cnt < cap ? cnt : cap;
+        }
+        sCnt[c] = cnt;
+        sLeave[c] = 0;
+        sOut[c] = 0;
+    }
+    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
+        int fi = e / NB, nn = e % NB;
+        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
+        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        double val = 0.0;
+        if (gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1)
+            val = a.f.f[PIC_EX + fi][(long)gu * g.sx + (long)gv * g.sy + gw];
+        sF[e] = val;
+    }
+    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) sJ[e] = 0.0;
+    if (tid == 0) {
+        sQ = 0;
+        sFlags = 0;
+        sDrop = 0;
+        sDup = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int c = 0; c < C; c++) {
+            sOff[c] = s;
+            s += sCnt[c];
+        }
+        sOff[C] = s;
+    }
+    __syncthreads();
+    const int T = sOff[C];
+    for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
+    __syncthreads();
+
+    const double hx = g.hx, hy = g.hy, hz = g.hz, tau = g.tau;
+    unsigned bad = 0;
+    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+        int lo = 0, hi = C;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
+        }
+        const int c = lo;
+        const int j = fidx - sOff[c];
+        Geo G;
+        G.ca = c / (TY * TZ);
+        G.cb = (c / TZ) % TY;
+        G.cc = c % TZ;
+        const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
+        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+        G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
+        G.x0 = (double)gi * hx;
+        G.y0 = (double)gl * hy;
+        G.z0 = (double)gk * hz;
+
+        double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
+        double pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
+        const unsigned char s = a.p.sp[pidx];
+        const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
+
+        /* GetField (cell.h:1257-1290) */
+        double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
+        if (!(x < 0 || y < 0 || z < 0)) {
+            int ix = cell_number(x, G.x0, hx), i1x = cell_number_center(x, G.x0, hx);
+            double s1 = s_node(x, hx), s2 = s_center(x, hx);
+            int iy = cell_number(y, G.y0, hy), i1y = cell_number_center(y, G.y0, hy);
+            double s3 = s_node(y, hy), s4 = s_center(y, hy);
+            int iz = cell_number(z, G.z0, hz), i1z = cell_number_center(z, G.z0, hz);
+            double s5 = s_node(z, hz), s6 = s_center(z, hz);
+            double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4, s51 = 1. - s5,
+                   s61 = 1. - s6;
+            if (ix < 2 || ix > 3 || i1x < 2 || i1x > 3 || iy < 2 || iy > 3 || i1y < 2 ||
+                i1y > 3 || iz < 2 || iz > 3 || i1z < 2 || i1z > 3) {
+                bad = 1;
+            } else {
+                const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
+                Ex = interp<BY, BZ>(sF + 0 * NB, ua + ix, va + i1y, wa + i1z, s1, s11, s4, s41, s6, s61);
+                Ey = interp<BY, BZ>(sF + 1 * NB, ua + i1x, va + iy, wa + i1z, s2, s21, s3, s31, s6, s61);
+                Ez = interp<BY, BZ>(sF + 2 * NB, ua + i1x, va + i1y, wa + iz, s2, s21, s4, s41, s5, s51);
+                Hx = interp<BY, BZ>(sF + 3 * NB, ua + i1x, va + iy, wa + iz, s2, s21, s3, s31, s5, s51);
+                Hy = interp<BY, BZ>(sF + 4 * NB, ua + ix, va + i1y, wa + iz, s1, s11, s4, s41, s5, s51);
+                Hz = interp<BY, BZ>(sF + 5 * NB, ua + ix, va + iy, wa + i1z, s1, s11, s3, s31, s6, s61);
+            }
+        }
+
+        /* Particle::Move (particle.h:160-202) */
+        const double tau1 = q_m * tau * 0.5;
+        pu += tau1 * Ex;
+        pv += tau1 * Ey;
+        pw += tau1 * Ez;
+        double ps = tau1 * pow_m05((pu * pu + pv * pv + pw * pw) * 1. + 1.0);
+        double bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
+        double su = pu + pv * bz - pw * by;
+        double sv = pv + pw * bx - pu * bz;
+        double sw = pw + pu * by - pv * bx;
+        double q1 = bx * bx, q2 = by * by, q3 = bz * bz;
+        double q4 = bx * by, q5 = by * bz, q6 = bz * bx;
+        double sden = q1 + 1. + q2 + q3;
+        double pu1 = ((q1 + 1.) * su + (q4 + bz) * sv + (q6 - by) * sw) / sden;
+        double pv1 = ((q4 - bz) * su + (q2 + 1.) * sv + (q5 + bx) * sw) / sden;
+        double pw1 = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.) * sw) / sden;
+        pu = pu1 + Ex * tau1;
+        pv = pv1 + Ey * tau1;
+        pw = pw1 + Ez * tau1;
+        ps = pow_m05(((pu * pu + pv * pv + pw * pw) + 1.0));
+        const double x1 = x + tau * (ps * pu);
+        const double y1 = y + tau * (ps * pv);
+        const double z1 = z + tau * (ps * pw);
+
+        /* CurrentToMesh (cell.h:1306-1422) */
+        {
+            int i1x = cell_number_center_current(x, G.x0, hx);
+            int i1y = cell_number_center_current(y, G.y0, hy);
+            int i1z = cell_number_center_current(z, G.z0, hz);
+            int i2x = cell_number_center_current(x1, G.x0, hx);
+            int i2y = cell_number_center_current(y1, G.y0, hy);
+            int i2z = cell_number_center_current(z1, G.z0, hz);
+            int di = abs(i2x - i1x), dl = abs(i2y - i1y), dk = abs(i2z - i1z);
+            int m = 4 * di + 2 * dl + dk;
+            double x2x = 0, x2y = 0, x2z = 0, sr;
+            switch (m) {
+            case 1:
+                x2z = transit_average(hz, i1z, i2z, G.z0);
+                sr = (x2z - z) / (z1 - z);
+                x2x = (x + (x1 - x) * sr);
+                x2y = (y + (y1 - y) * sr);
+                break;
+            case 2:
+                x2y = transit_average(hy, i1y, i2y, G.y0);
+                sr = (x2y - y) / (y1 - y);
+                x2x = (x + (x1 - x) * sr);
+                x2z = (z + (z1 - z) * sr);
+                break;
+            case 3:
+                x2y = transit_average(hy, i1y, i2y, G.y0);
+                x2z = transit_average(hz, i1z, i2z, G.z0);
+                sr = ((x2z - z) * (z1 - z) + (x2y - y) * (y1 - y)) /
+                     ((z1 - z) * (z1 - z) + (y1 - y) * (y1 - y));
+                x2x = (x + (x1 - x) * sr);
+                break;
+            case 4:
+                x2x = transit_average(hx, i1x, i2x, G.x0);
+                sr = (x2x - x) / (x1 - x);
+                x2y = (y + (y1 - y) * sr);
+                x2z = (z + (z1 - z) * sr);
+                break;
+            case 5:
+                x2x = transit_average(hx, i1x, i2x, G.x0);
+                x2z = transit_average(hz, i1z, i2z, G.z0);
+                sr = ((x2z - z) * (z1 - z) + (x2x - x) * (x1 - x)) /
+                     ((z1 - z) * (z1 - z) + (x1 - x) * (x1 - x));
+                x2y = (y + (y1 - y) * sr);
+                break;
+            case 6:
+                x2x = transit_average(hx, i1x, i2x, G.x0);
+                x2y = transit_average(hy, i1y, i2y, G.y0);
+                sr = ((x2y - y) * (y1 - y) + (x2x - x) * (x1 - x)) /
+                     ((y1 - y) * (y1 - y) + (x1 - x) * (x1 - x));
+                x2z = (z + (z1 - z) * sr);
+                break;
+            case 7:
+                x2x = transit_average(hx, i1x, i2x, G.x0);
+                x2y = transit_average(hy, i1y, i2y, G.y0);
+                x2z = transit_average(hz, i1z, i2z, G.z0);
+                break;
+            default:
+                m = 0;
+                break;
+            }
+            if (m == 0) {
+                pqr_lds<BY, BZ, NB>(sJ, G, i1x, i1y, i1z, x, y, z, x1, y1, z1, mass, bad);
+            } else {
+                pqr_lds<BY, BZ, NB>(sJ, G, i1x, i1y, i1z, x, y, z, x2x, x2y, x2z, mass, bad);
+                pqr_lds<BY, BZ, NB>(sJ, G, i2x, i2y, i2z, x2x, x2y, x2z, x1, y1, z1, mass, bad);
+            }
+        }
+
+        /* x <- x1, Reflect (cell.h:1410-1478) */
+        const double xn = reflect1(x1, g.Lx), yn = reflect1(y1, g.Ly), zn = reflect1(z1, g.Lz);
+
+        /* flyDirection (cell.h:694-701) with the owner cell's geometry */
+        int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
+                      ? 2
+                      : (((xn < G.x0) || ((xn > g.Lx - hx) && (gi == 0))) ? 0 : 1);
+        int ddy = (((yn > G.y0 + hy) && (yn < G.y0 + 2 * hy)) || ((yn < hy) && (gl == g.Ny - 1)))
+                      ? 2
+                      : (((yn < G.y0) || ((yn > g.Ly - hy) && (gl == 0))) ? 0 : 1);
+        int ddz = (((zn > G.z0 + hz) && (zn < G.z0 + 2 * hz)) || ((zn < hz) && (gk == g.Nz - 1)))
+                      ? 2
+                      : (((zn < G.z0) || ((zn > g.Lz - hz) && (gk == 0))) ? 0 : 1);
+        const int dir = ddx * 9 + ddy * 3 + ddz;
+        const bool inside = (G.x0 <= xn) && (xn < G.x0 + hx) && (G.y0 <= yn) &&
+                            (yn < G.y0 + hy) && (G.z0 <= zn) && (zn < G.z0 + hz);
+        if (!inside) atomicAdd(&sOut[c], 1);
+
+        if (dir == 13) {
+            a.p.p[0][pidx] = xn;
+            a.p.p[1][pidx] = yn;
+            a.p.p[2][pidx] = zn;
+            a.p.p[3][pidx] = pu;
+            a.p.p[4][pidx] = pv;
+            a.p.p[5][pidx] = pw;
+        } else {
+            atomicAdd(&sLeave[c], 1);
+            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
+            /* GPU_ArrangeFlights (kernels.cu:316-370): every bulk cell of the
+             * 27-neighbourhood (getWrapCellNumber) whose isPointInCell
+             * (cell.h:717) holds receives a copy; per axis the test is
+             * separable, so the receivers are a product of per-axis sets. */
+            int rx[3], ry[3], rz[3], nxr = 0, nyr = 0, nzr = 0;
+            for (int d = -1; d <= 1; d++) {
+                int r = gi + d;
+                if (r >= g.Nx) r -= g.Nx; else if (r < 0) r += g.Nx;
+                double r0 = (double)r * hx;
+                if (r0 <= xn && xn < r0 + hx) rx[nxr++] = r;
+                r = gl + d;
+                if (r >= g.Ny) r -= g.Ny; else if (r < 0) r += g.Ny;
+                r0 = (double)r * hy;
+                if (r0 <= yn && yn < r0 + hy) ry[nyr++] = r;
+                r = gk + d;
+                if (r >= g.Nz) r -= g.Nz; else if (r < 0) r += g.Nz;
+                r0 = (double)r * hz;
+                if (r0 <= zn && zn < r0 + hz) rz[nzr++] = r;
+            }
+            const int ncopy = nxr * nyr * nzr;
+            if (ncopy == 0) atomicAdd(&sDrop, 1ull);
+            if (ncopy > 1) atomicAdd(&sDup, (unsigned long long)(ncopy - 1));
+            if (ncopy > 0) {
+                int slot = atomicAdd(&sQ, ncopy);
+                const long qb = tile * (long)a.q.qcap;
+                int kq = 0;
+                for (int ax = 0; ax < nxr; ax++)
+                    for (int ay = 0; ay < nyr; ay++)
+                        for (int az = 0; az < nzr; az++, kq++) {
+                            int idx = slot + kq;
+                            if (idx >= a.q.qcap) continue;
+                            long e = qb + idx;
+                            a.q.q[0][e] = xn;
+                            a.q.q[1][e] = yn;
+                            a.q.q[2][e] = zn;
+                            a.q.q[3][e] = pu;
+                            a.q.q[4][e] = pv;
+                            a.q.q[5][e] = pw;
+                            a.q.sp[e] = s;
+                            a.q.dest[e] = (int)(((long)rx[ax] * g.Ny + ry[ay]) * g.Nz + rz[az]);
+                        }
+            }
+        }
+    }
+    if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
+    __syncthreads();
+
+    /* holes -> tail stayers, one lane per cell; the swap-with-last idea of
+     * removeParticleFromSurfaceDevice (cell.h:391-432) */
+    int maxout = 0;
+    for (int c = tid; c < C; c += PIC_BLOCK) {
+        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
+        const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+        const int n = sCnt[c], m = sLeave[c], np = n - m, base = sOff[c];
+        if (m > 0) {
+            int h = next_bit(sMask, base, 0, np, true);
+            int sidx = next_bit(sMask, base, np, n, false);
+            while (h < np && sidx < n) {
+                const long from = cell * cap + sidx, to = cell * cap + h;
+                for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
+                a.p.sp[to] = a.p.sp[from];
+                h = next_bit(sMask, base, h + 1, np, true);
+                sidx = next_bit(sMask, base, sidx + 1, n, false);
+            }
+        }
+        a.p.count[cell] = np;
+        if (sOut[c] > maxout) maxout = sOut[c];
+    }
+
+    /* tile J box -> global J (GPU_WriteAllCurrents, kernels.cu:83-112) */
+    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
+        double val = sJ[e];
+        if (val == 0.0) continue;
+        int comp = e / NB, nn = e % NB;
+        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
+        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
+        atomicAdd(&a.f.f[PIC_JX + comp][(long)gu * g.sx + (long)gv * g.sy + gw], val);
+    }
+
+    if (maxout > 0) {
+        if (a.dep_cap > 0 && maxout > a.dep_cap) atomicOr(&sFlags, (unsigned)PIC_F_TOO_MANY);
+        if (maxout > a.st->max_dep) atomicMax(&a.st->max_dep, maxout);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int nq = sQ;
+        if (nq > a.q.qcap) {
+            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_OVERFLOW);
+            nq = a.q.qcap;
+        }
+        a.q.count[tile] = nq;
+        if (sFlags) atomicOr(&a.st->flags, sFlags);
+        if (sDrop) atomicAdd(&a.st->dropped, sDrop);
+        if (sDup) atomicAdd(&a.st->duplicated, sDup);
+    }
+}
+
+/* arrivals: each queued leaver claims the next slot of its destination
+ * bucket (the Insert of GPU_ArrangeFlights, kernels.cu:357-360) */
+__global__ __launch_bounds__(PIC_BLOCK) void k_arrive(PartPtrs p, QueuePtrs q, DevStatus *st) {
+    const long tile = blockIdx.x;
+    const int n = q.count[tile];
+    const long qb = tile * (long)q.qcap;
+    bool over = false;
+    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
+        const long e = qb + j;
+        const int d = q.dest[e];
+        const int slot = atomicAdd(&p.count[d], 1);
+        q.slot[e] = slot;
+        if (slot < p.cap) {
+            const long to = (long)d * p.cap + slot;
+            for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
+            p.sp[to] = q.sp[e];
+        } else {
+            over = true;
+        }
+    }
+    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+}
+
+/* initial upload: compact cell-sorted arrays -> buckets */
+__global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load(const long *off, long ncell,
+                                                           const double *s0, const double *s1,
+                                                           const double *s2, const double *s3,
+                                                           const double *s4, const double *s5,
+                                                           const unsigned char *ssp, PartPtrs p) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    const long n = off[c + 1] - off[c];
+    if (j == 0) p.count[c] = (int)n;
+    if (j >= n) return;
+    const long src = off[c] + j;
+    p.p[0][e] = s0[src];
+    p.p[1][e] = s1[src];
+    p.p[2][e] = s2[src];
+    p.p[3][e] = s3[src];
+    p.p[4][e] = s4[src];
+    p.p[5][e] = s5[src];
+    p.sp[e] = ssp[src];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_repack(long ncell, PartPtrs from, PartPtrs to) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * from.cap) return;
+    const long c = e / from.cap;
+    const int j = (int)(e % from.cap);
+    int n = from.count[c];
+    if (n > from.cap) n = from.cap;
+    if (j >= n) return;
+    const long dst = c * to.cap + j;
+    for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
+    to.sp[dst] = from.sp[e];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals(QueuePtrs q, int oldcap, PartPtrs to) {
+    const long tile = blockIdx.x;
+    const int n = q.count[tile];
+    const long qb = tile * (long)q.qcap;
+    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
+        const long e = qb + j;
+        const int slot = q.slot[e];
+        if (slot < oldcap || slot >= to.cap) continue;
+        const long dst = (long)q.dest[e] * to.cap + slot;
+        for (int k = 0; k < 6; k++) to.p[k][dst] = q.q[k][e];
+        to.sp[dst] = q.sp[e];
+    }
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles(long ncell, const long *off,
+                                                               PartPtrs p, double *d0, double *d1,
+                                                               double *d2, double *d3, double *d4,
+                                                               double *d5, unsigned char *dsp,
+                                                               int *dcell) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    if (j >= off[c + 1] - off[c]) return;
+    const long dst = off[c] + j;
+    d0[dst] = p.p[0][e];
+    d1[dst] = p.p[1][e];
+    d2[dst] = p.p[2][e];
+    d3[dst] = p.p[3][e];
+    d4[dst] = p.p[4][e];
+    d5[dst] = p.p[5][e];
+    dsp[dst] = p.sp[e];
+    dcell[dst] = (int)c;
+}
+
+/* FP64 moments per sort: N, sum x,y,z, sum p, sum (m/q_m)(gamma-1), sum |p| */
+__global__ __launch_bounds__(PIC_BLOCK) void k_moments(long ncell, PartPtrs p, SpeciesTable sp,
+                                                      const int *slot_sort, double *partials) {
+    __shared__ double red[PIC_BLOCK];
+    double acc[27];
+    for (int k = 0; k < 27; k++) acc[k] = 0.0;
+    for (long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x; e < ncell * p.cap;
+         e += (long)gridDim.x * PIC_BLOCK) {
+        const long c = e / p.cap;
+        const int j = (int)(e % p.cap);
+        if (j >= p.count[c]) continue;
+        const int s = p.sp[e];
+        int srt = slot_sort[s];
+        srt = srt < 0 ? 0 : (srt > 2 ? 2 : srt);
+        const double px = p.p[3][e], py = p.p[4][e], pz = p.p[5][e];
+        const double gm = sqrt(1.0 + px * px + py * py + pz * pz);
+        double v[9] = {1.0, p.p[0][e], p.p[1][e], p.p[2][e], px, py, pz,
+                       (sp.m[s] / sp.q_m[s]) * (gm - 1.0), fabs(px) + fabs(py) + fabs(pz)};
+        for (int k = 0; k < 9; k++) {
+            double add = v[k];
+            acc[0 * 9 + k] += (srt == 0) ? add : 0.0;
+            acc[1 * 9 + k] += (srt == 1) ? add : 0.0;
+            acc[2 * 9 + k] += (srt == 2) ? add : 0.0;
+        }
+    }
+    for (int k = 0; k < 27; k++) {
+        red[threadIdx.x] = acc[k];
+        __syncthreads();
+        for (int w = PIC_BLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * 27 + k] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_moments_final(const double *partials, int nblocks, double *out27) {
+    const int k = threadIdx.x;
+    if (k >= 27) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partials[(long)b * 27 + k];
+    out27[k] = s;
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC_BLOCK); }
+
+}  // namespace
+
+size_t push_lds_bytes(const TileGeom &t, int cap) {
+    long bits = (long)t.TX * t.TY * t.TZ * cap;
+    return (size_t)((bits + 31) / 32) * 4;
+}
+
+void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
+                             QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
+                             hipStream_t s) {
+    PushArgs a;
+    a.g = g;
+    a.t = t;
+    a.f = f;
+    a.p = p;
+    a.q = q;
+    a.sp = sp;
+    a.dep_cap = dep_cap;
+    a.st = st;
+    hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK),
+                       push_lds_bytes(t, p.cap), s, a);
+}
+
+void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL(k_arrive, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, p, q, st);
+}
+
+void launch_scatter_load(const long *offsets, long ncell, const double *src[6],
+                         const unsigned char *srcsp, PartPtrs p, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_load, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       offsets, ncell, src[0], src[1], src[2], src[3], src[4], src[5], srcsp, p);
+}
+
+void launch_repack(long ncell, PartPtrs from, PartPtrs to, hipStream_t s) {
+    hipLaunchKernelGGL(k_repack, dim3(blocks_for(ncell * from.cap)), dim3(PIC_BLOCK), 0, s, ncell,
+                       from, to);
+}
+
+void launch_fix_arrivals(const TileGeom &t, QueuePtrs q, int oldcap, PartPtrs to, hipStream_t s) {
+    hipLaunchKernelGGL(k_fix_arrivals, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, q, oldcap,
+                       to);
+}
+
+void launch_gather_particles(long ncell, const long *offsets, PartPtrs p, double *dst[6],
+                             unsigned char *dsp, int *dcell, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_particles, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       ncell, offsets, p, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dsp,
+                       dcell);
+}
+
+void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort,
+                    double *partials, int nblocks, double *out27, hipStream_t s) {
+    hipLaunchKernelGGL(k_moments, dim3(nblocks), dim3(PIC_BLOCK), 0, s, ncell, p, sp, slot_sort,
+                       partials);
+    hipLaunchKernelGGL(k_moments_final, dim3(1), dim3(32), 0, s, partials, nblocks, out27);
+}
+
+TileGeom make_tile_geom(int Nx, int Ny, int Nz) {
+    TileGeom t;
+    t.TX = TXC;
+    t.TY = TYC;
+    t.TZ = TZC;
+    t.ntx = (Nx + TXC - 1) / TXC;
+    t.nty = (Ny + TYC - 1) / TYC;
+    t.ntz = (Nz + TZC - 1) / TZC;
+    t.ntiles = (long)t.ntx * t.nty * t.ntz;
+    return t;
+}

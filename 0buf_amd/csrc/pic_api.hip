@@ -1,0 +1,627 @@
+/*
+ * pic_api.hip -- host side of libpic_amd.so: the C ABI declared in
+ * include/pic_amd.h.  Plain HIP runtime calls; kernels live in k_*.hip.
+ *
+ * Step orchestration follows Plasma::Step (gpu_plasma.h:605-620): H half
+ * step (+ J := 0), fused gather/push/deposit/departures, arrivals, periodic
+ * J fold, H second half, E update, periodic E ghosts, energy.  The whole step
+ * is enqueued on the context stream and synchronised once; the reference
+ * synchronises five times and copies stage arrays to the host every step.
+ */
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pic_internal.h"
+
+TileGeom make_tile_geom(int Nx, int Ny, int Nz);
+
+namespace {
+
+thread_local char g_hip_err[256] = "";
+
+int record_hip(hipError_t e, const char *where) {
+    snprintf(g_hip_err, sizeof(g_hip_err), "%s: %s (%d)", where, hipGetErrorString(e), (int)e);
+    return e == hipErrorOutOfMemory ? PIC_ERR_NOMEM : PIC_ERR_HIP;
+}
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return record_hip(e_, #x);   \
+    } while (0)
+
+struct Staged {
+    std::vector<double> v[6];
+    std::vector<unsigned char> sp;
+    std::vector<int> cell;
+};
+
+}  // namespace
+
+struct pic_ctx {
+    pic_params prm;
+    Grid g;
+    TileGeom t;
+    long NC = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    FieldPtrs f{};
+    PartPtrs p{};
+    QueuePtrs q{};
+    SpeciesTable sp{};
+    int slot_sort[PIC_MAX_SLOTS] = {0};
+    int nslots = 0;
+    int *d_slot_sort = nullptr;
+    DevStatus *d_st = nullptr;
+    DevStatus *h_st = nullptr;
+    double *d_part = nullptr;
+    int npart = 1024;
+    double *d_red = nullptr;
+    double *h_red = nullptr;
+    Staged staged;
+    bool committed = false;
+    int regrows = 0;
+    pic_step_status last{};
+};
+
+namespace {
+
+void free_parts(PartPtrs &p) {
+    for (int k = 0; k < 6; k++)
+        if (p.p[k]) hipFree(p.p[k]);
+    if (p.sp) hipFree(p.sp);
+    if (p.count) hipFree(p.count);
+    p = PartPtrs{};
+}
+
+int alloc_parts(PartPtrs &p, long ncell, int cap, bool with_count) {
+    p = PartPtrs{};
+    p.cap = cap;
+    size_t n = (size_t)ncell * cap;
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&p.p[k], n * sizeof(double)));
+    HIPCHK(hipMalloc(&p.sp, n));
+    if (with_count) HIPCHK(hipMalloc(&p.count, ncell * sizeof(int)));
+    return PIC_SUCCESS;
+}
+
+void free_queue(QueuePtrs &q) {
+    for (int k = 0; k < 6; k++)
+        if (q.q[k]) hipFree(q.q[k]);
+    if (q.sp) hipFree(q.sp);
+    if (q.dest) hipFree(q.dest);
+    if (q.slot) hipFree(q.slot);
+    if (q.count) hipFree(q.count);
+    q = QueuePtrs{};
+}
+
+int alloc_queue(QueuePtrs &q, const TileGeom &t, int qcap) {
+    q = QueuePtrs{};
+    q.qcap = qcap;
+    size_t n = (size_t)t.ntiles * qcap;
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&q.q[k], n * sizeof(double)));
+    HIPCHK(hipMalloc(&q.sp, n));
+    HIPCHK(hipMalloc(&q.dest, n * sizeof(int)));
+    HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
+    HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
+    HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
+    return PIC_SUCCESS;
+}
+
+int queue_capacity(const TileGeom &t, int cap) {
+    long c = (long)t.TX * t.TY * t.TZ * cap / 2;
+    c = (c + 63) / 64 * 64;
+    return (int)std::max(64L, c);
+}
+
+int auto_capacity(int maxc) {
+    int cap = (int)(maxc * 1.5) + 8;
+    cap = (cap + 7) / 8 * 8;
+    return std::max(cap, 16);
+}
+
+/* upload staged particles into cell buckets */
+int commit(pic_ctx *c) {
+    if (c->committed) return PIC_SUCCESS;
+    const long NC = c->NC;
+    const long P = (long)c->staged.cell.size();
+    std::vector<long> off(NC + 1, 0);
+    for (long j = 0; j < P; j++) off[c->staged.cell[j] + 1]++;
+    int maxc = 0;
+    for (long k = 0; k < NC; k++) maxc = std::max(maxc, (int)off[k + 1]);
+    for (long k = 0; k < NC; k++) off[k + 1] += off[k];
+    int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
+    if (maxc > cap) cap = auto_capacity(maxc);
+    /* counting sort by cell (stable: load order kept inside a cell) */
+    std::vector<double> sorted[6];
+    std::vector<unsigned char> ssp(P);
+    for (int k = 0; k < 6; k++) sorted[k].resize(P);
+    {
+        std::vector<long> pos(off.begin(), off.end() - 1);
+        for (long j = 0; j < P; j++) {
+            long d = pos[c->staged.cell[j]]++;
+            for (int k = 0; k < 6; k++) sorted[k][d] = c->staged.v[k][j];
+            ssp[d] = c->staged.sp[j];
+        }
+    }
+    int rc = alloc_parts(c->p, NC, cap, true);
+    if (rc) return rc;
+    rc = alloc_queue(c->q, c->t, queue_capacity(c->t, cap));
+    if (rc) return rc;
+    long *d_off = nullptr;
+    double *d_src[6] = {nullptr};
+    unsigned char *d_sp = nullptr;
+    size_t Pn = (size_t)std::max(P, 1L);
+    HIPCHK(hipMalloc(&d_off, (NC + 1) * sizeof(long)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&d_src[k], Pn * sizeof(double)));
+    HIPCHK(hipMalloc(&d_sp, Pn));
+    HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
+    if (P > 0) {
+        for (int k = 0; k < 6; k++)
+            HIPCHK(hipMemcpy(d_src[k], sorted[k].data(), P * sizeof(double), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_sp, ssp.data(), P, hipMemcpyHostToDevice));
+    }
+    const double *src[6] = {d_src[0], d_src[1], d_src[2], d_src[3], d_src[4], d_src[5]};
+    launch_scatter_load(d_off, NC, src, d_sp, c->p, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    hipFree(d_off);
+    for (int k = 0; k < 6; k++) hipFree(d_src[k]);
+    hipFree(d_sp);
+    Staged empty;
+    std::swap(c->staged, empty);
+    c->committed = true;
+    c->last.bucket_capacity = cap;
+    return PIC_SUCCESS;
+}
+
+/* bucket overflow repair: grow every bucket, re-place the arrivals whose
+ * claimed slot did not fit */
+int regrow(pic_ctx *c) {
+    const long NC = c->NC;
+    std::vector<int> cnt(NC);
+    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    int maxc = 0;
+    for (long k = 0; k < NC; k++) maxc = std::max(maxc, cnt[k]);
+    int oldcap = c->p.cap;
+    int cap = std::max(auto_capacity(maxc), oldcap + 8);
+    PartPtrs np;
+    int rc = alloc_parts(np, NC, cap, false);
+    if (rc) return rc;
+    np.count = c->p.count;
+    launch_repack(NC, c->p, np, c->stream);
+    launch_fix_arrivals(c->t, c->q, oldcap, np, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->p.count = nullptr;
+    free_parts(c->p);
+    c->p = np;
+    QueuePtrs nq;
+    rc = alloc_queue(nq, c->t, queue_capacity(c->t, cap));
+    if (rc) return rc;
+    free_queue(c->q);
+    c->q = nq;
+    c->regrows++;
+    c->last.bucket_capacity = cap;
+    c->last.bucket_regrows = c->regrows;
+    return PIC_SUCCESS;
+}
+
+int check_ready(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    if (!c->committed) return commit(c);
+    return PIC_SUCCESS;
+}
+
+int finish_step(pic_ctx *c) {
+    HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const DevStatus st = *c->h_st;
+    c->last.dropped = (long long)st.dropped;
+    c->last.duplicated = (long long)st.duplicated;
+    c->last.max_cell_departures = st.max_dep;
+    c->last.flags = (int)st.flags;
+    c->last.bucket_capacity = c->p.cap;
+    c->last.bucket_regrows = c->regrows;
+    if (st.flags & PIC_F_BUCKET_OVERFLOW) {
+        int rc = regrow(c);
+        if (rc) return rc;
+    }
+    if (st.flags & PIC_F_QUEUE_OVERFLOW) return PIC_ERR_QUEUE_OVERFLOW;
+    if (st.flags & PIC_F_RANGE) return PIC_ERR_CELL_RANGE;
+    if (st.flags & PIC_F_TOO_MANY) return PIC_ERR_TOO_MANY_PARTICLES;
+    return PIC_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* ------------------------------------------------------------------ archAPI */
+int MemoryCopy(void *dst, void *src, size_t size, int dir) {
+    hipMemcpyKind k;
+    switch (dir) {
+    case HOST_TO_DEVICE: k = hipMemcpyHostToDevice; break;
+    case HOST_TO_HOST: k = hipMemcpyHostToHost; break;
+    case DEVICE_TO_HOST: k = hipMemcpyDeviceToHost; break;
+    case DEVICE_TO_DEVICE: k = hipMemcpyDeviceToDevice; break;
+    default: return (int)hipErrorInvalidValue;
+    }
+    return (int)hipMemcpy(dst, src, size, k);
+}
+int MemoryAllocate(void **dst, size_t size) { return (int)hipMalloc(dst, size); }
+int MemoryFree(void *p) { return (int)hipFree(p); }
+int GetDeviceMemory(size_t *m_free, size_t *m_total) { return (int)hipMemGetInfo(m_free, m_total); }
+int MemorySet(void *s, int c, size_t n) { return (int)hipMemset(s, c, n); }
+int ThreadSynchronize(void) { return (int)hipDeviceSynchronize(); }
+int DeviceSynchronize(void) { return (int)hipDeviceSynchronize(); }
+int getLastError(void) { return (int)hipGetLastError(); }
+const char *getErrorString(int err) { return hipGetErrorString((hipError_t)err); }
+int SetDevice(int n) { return (int)hipSetDevice(n); }
+
+/* ------------------------------------------------------------------ context */
+void pic_params_default(pic_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->nx = 32; p->ny = 8; p->nz = 8;
+    p->lx = 3.2; p->ly = 0.8; p->lz = 0.8;
+    p->ni = 1.0;
+    p->n_per_cell = 16;
+    p->ion_q_m = 1.0 / 1836.0;
+    p->tau = 0.05;
+    p->precision = PIC_FP64;
+    p->departure_cap = 50;
+    p->bucket_capacity = 0;
+    p->device = 0;
+}
+
+const char *pic_error_string(int s) {
+    switch (s) {
+    case PIC_SUCCESS: return "success";
+    case PIC_ERR_INVALID: return "invalid argument";
+    case PIC_ERR_HIP: return "HIP runtime error";
+    case PIC_ERR_NO_DEVICE: return "no HIP device";
+    case PIC_ERR_TOO_MANY_PARTICLES: return "too many particles leaving one cell";
+    case PIC_ERR_QUEUE_OVERFLOW: return "departure queue overflow";
+    case PIC_ERR_NOMEM: return "device out of memory";
+    case PIC_ERR_STATE: return "call out of order";
+    case PIC_ERR_CELL_RANGE: return "particle stencil outside its tile";
+    default: return "unknown status";
+    }
+}
+
+const char *pic_last_hip_error(void) { return g_hip_err; }
+
+int pic_create(const pic_params *prm, pic_ctx **out) {
+    if (!prm || !out) return PIC_ERR_INVALID;
+    *out = nullptr;
+    if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return PIC_ERR_INVALID;
+    if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
+    if (prm->precision != PIC_FP64) return PIC_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
+    if (prm->device < 0 || prm->device >= ndev) return PIC_ERR_INVALID;
+    HIPCHK(hipSetDevice(prm->device));
+    pic_ctx *c = new pic_ctx();
+    c->prm = *prm;
+    c->dev = prm->device;
+    Grid &g = c->g;
+    g.Nx = prm->nx; g.Ny = prm->ny; g.Nz = prm->nz;
+    g.sy = g.Nz + 2;
+    g.sx = (g.Ny + 2) * (g.Nz + 2);
+    g.G = (long)(g.Nx + 2) * g.sx;
+    g.Lx = prm->lx; g.Ly = prm->ly; g.Lz = prm->lz;
+    /* Cell ctor (cell.h:731-737): h = L / N */
+    g.hx = prm->lx / (double)prm->nx;
+    g.hy = prm->ly / (double)prm->ny;
+    g.hz = prm->lz / (double)prm->nz;
+    g.tau = prm->tau;
+    c->NC = (long)g.Nx * g.Ny * g.Nz;
+    c->t = make_tile_geom(g.Nx, g.Ny, g.Nz);
+    int rc = PIC_SUCCESS;
+    auto fail = [&](int r) {
+        pic_destroy(c);
+        return r;
+    };
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(record_hip(e, "hipStreamCreate"));
+    for (int k = 0; k < PIC_NFIELDS; k++) {
+        e = hipMalloc(&c->f.f[k], g.G * sizeof(double));
+        if (e != hipSuccess) return fail(record_hip(e, "hipMalloc fields"));
+        e = hipMemset(c->f.f[k], 0, g.G * sizeof(double));
+        if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
+    }
+    if ((e = hipMalloc(&c->d_st, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
+    if ((e = hipMemset(c->d_st, 0, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
+    if ((e = hipHostMalloc(&c->h_st, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
+    if ((e = hipMalloc(&c->d_part, (size_t)c->npart * 27 * sizeof(double))) != hipSuccess)
+        return fail(record_hip(e, "partials"));
+    if ((e = hipMalloc(&c->d_red, 64 * sizeof(double))) != hipSuccess) return fail(record_hip(e, "red"));
+    if ((e = hipHostMalloc(&c->h_red, 64 * sizeof(double))) != hipSuccess) return fail(record_hip(e, "red"));
+    if ((e = hipMalloc(&c->d_slot_sort, PIC_MAX_SLOTS * sizeof(int))) != hipSuccess)
+        return fail(record_hip(e, "slots"));
+    (void)rc;
+    *out = c;
+    return PIC_SUCCESS;
+}
+
+int pic_destroy(pic_ctx *c) {
+    if (!c) return PIC_SUCCESS;
+    hipSetDevice(c->dev);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (int k = 0; k < PIC_NFIELDS; k++)
+        if (c->f.f[k]) hipFree(c->f.f[k]);
+    free_parts(c->p);
+    free_queue(c->q);
+    if (c->d_st) hipFree(c->d_st);
+    if (c->h_st) hipHostFree(c->h_st);
+    if (c->d_part) hipFree(c->d_part);
+    if (c->d_red) hipFree(c->d_red);
+    if (c->h_red) hipHostFree(c->h_red);
+    if (c->d_slot_sort) hipFree(c->d_slot_sort);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+    return PIC_SUCCESS;
+}
+
+int pic_load_particles(pic_ctx *c, int sort, const double *x, const double *y, const double *z,
+                       const double *px, const double *py, const double *pz, long n, double m,
+                       double q_m, long *n_dropped) {
+    if (!c || n < 0 || (n > 0 && (!x || !y || !z || !px || !py || !pz))) return PIC_ERR_INVALID;
+    if (sort < 0 || sort > 2 || q_m == 0.0) return PIC_ERR_INVALID;
+    if (c->committed) return PIC_ERR_STATE;
+    int slot = -1;
+    for (int s = 0; s < c->nslots; s++)
+        if (c->slot_sort[s] == sort && c->sp.m[s] == m && c->sp.q_m[s] == q_m) slot = s;
+    if (slot < 0) {
+        if (c->nslots >= PIC_MAX_SLOTS) return PIC_ERR_INVALID;
+        slot = c->nslots++;
+        c->slot_sort[slot] = sort;
+        c->sp.m[slot] = m;
+        c->sp.q_m[slot] = q_m;
+        HIPCHK(hipMemcpy(c->d_slot_sort, c->slot_sort, sizeof(c->slot_sort), hipMemcpyHostToDevice));
+    }
+    const Grid &g = c->g;
+    long dropped = 0;
+    Staged &S = c->staged;
+    for (long j = 0; j < n; j++) {
+        /* getPointCell (cell.h:593-598): (int)((x - 0.0)/h) per axis */
+        int i = (int)((x[j] - 0.0) / g.hx);
+        int l = (int)((y[j] - 0.0) / g.hy);
+        int k = (int)((z[j] - 0.0) / g.hz);
+        if (i < 0 || l < 0 || k < 0 || i >= g.Nx || l >= g.Ny || k >= g.Nz) {
+            dropped++;
+            continue;
+        }
+        /* Cell::Insert -> isPointInCell (cell.h:717, 1754-1768) */
+        double x0 = (double)i * g.hx, y0 = (double)l * g.hy, z0 = (double)k * g.hz;
+        if (!((x0 <= x[j]) && (x[j] < x0 + g.hx) && (y0 <= y[j]) && (y[j] < y0 + g.hy) &&
+              (z0 <= z[j]) && (z[j] < z0 + g.hz))) {
+            dropped++;
+            continue;
+        }
+        S.v[0].push_back(x[j]);
+        S.v[1].push_back(y[j]);
+        S.v[2].push_back(z[j]);
+        S.v[3].push_back(px[j]);
+        S.v[4].push_back(py[j]);
+        S.v[5].push_back(pz[j]);
+        S.sp.push_back((unsigned char)slot);
+        S.cell.push_back((int)(((long)i * g.Ny + l) * g.Nz + k));
+    }
+    if (n_dropped) *n_dropped = dropped;
+    return PIC_SUCCESS;
+}
+
+int pic_load_field(pic_ctx *c, int which, const double *host) {
+    if (!c || which < 0 || which >= PIC_NFIELDS || !host) return PIC_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(c->f.f[which], host, c->g.G * sizeof(double), hipMemcpyHostToDevice));
+    return PIC_SUCCESS;
+}
+
+int pic_get_field(pic_ctx *c, int which, double *host) {
+    if (!c || which < 0 || which >= PIC_NFIELDS || !host) return PIC_ERR_INVALID;
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(host, c->f.f[which], c->g.G * sizeof(double), hipMemcpyDeviceToHost));
+    return PIC_SUCCESS;
+}
+
+/* ------------------------------------------------------------------ stages */
+int pic_yee_h_half(pic_ctx *c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_reset_status(c->d_st, c->stream);
+    launch_h_half(c->g, c->f, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_push_deposit(pic_ctx *c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap, c->d_st,
+                            c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_migrate(pic_ctx *c) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_fold_j(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    launch_fold_j(c->g, c->f, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_yee_h2(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    launch_h2(c->g, c->f, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_yee_e(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    launch_e(c->g, c->f, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_e_periodic(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    launch_e_periodic(c->g, c->f, c->stream);
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int pic_energy(pic_ctx *c, double *energy) {
+    if (!c) return PIC_ERR_INVALID;
+    launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (energy) *energy = c->h_red[0];
+    return PIC_SUCCESS;
+}
+
+int pic_sync(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    if (!c->committed) return PIC_SUCCESS;
+    return finish_step(c);
+}
+
+int pic_step(pic_ctx *c, int nt, double *energy) {
+    (void)nt;
+    int rc = check_ready(c);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->dev));
+    launch_reset_status(c->d_st, c->stream);
+    launch_h_half(c->g, c->f, c->stream);
+    launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap, c->d_st,
+                            c->stream);
+    launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    launch_fold_j(c->g, c->f, c->stream);
+    launch_h2(c->g, c->f, c->stream);
+    launch_e(c->g, c->f, c->stream);
+    launch_e_periodic(c->g, c->f, c->stream);
+    launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    rc = finish_step(c);
+    if (energy) *energy = c->h_red[0];
+    return rc;
+}
+
+int pic_run(pic_ctx *c, int nt0, int nsteps, double *energies) {
+    for (int s = 0; s < nsteps; s++) {
+        int rc = pic_step(c, nt0 + s, energies ? &energies[s] : nullptr);
+        if (rc) return rc;
+    }
+    return PIC_SUCCESS;
+}
+
+int pic_get_status(pic_ctx *c, pic_step_status *st) {
+    if (!c || !st) return PIC_ERR_INVALID;
+    *st = c->last;
+    if (c->committed) {
+        HIPCHK(hipStreamSynchronize(c->stream));
+        std::vector<int> qc(c->t.ntiles);
+        HIPCHK(hipMemcpy(qc.data(), c->q.count, c->t.ntiles * sizeof(int), hipMemcpyDeviceToHost));
+        long long s = 0;
+        for (int v : qc) s += v;
+        st->departures = s;
+    }
+    return PIC_SUCCESS;
+}
+
+long pic_particle_count(pic_ctx *c) {
+    if (!c) return -1;
+    if (!c->committed) return (long)c->staged.cell.size();
+    std::vector<int> cnt(c->NC);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
+    if (hipMemcpy(cnt.data(), c->p.count, c->NC * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        return -1;
+    long n = 0;
+    for (int v : cnt) n += v;
+    return n;
+}
+
+int pic_get_cell_counts(pic_ctx *c, int *counts) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!counts) return PIC_ERR_INVALID;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(counts, c->p.count, c->NC * sizeof(int), hipMemcpyDeviceToHost));
+    return PIC_SUCCESS;
+}
+
+int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, double *py,
+                      double *pz, double *m, double *q_m, int *sort, int *cell, long cap,
+                      long *n_out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long NC = c->NC;
+    std::vector<int> cnt(NC);
+    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<long> off(NC + 1, 0);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    const long P = off[NC];
+    if (n_out) *n_out = P;
+    if (P > cap) return PIC_ERR_INVALID;
+    size_t Pn = (size_t)std::max(P, 1L);
+    long *d_off = nullptr;
+    double *d[6] = {nullptr};
+    unsigned char *d_sp = nullptr;
+    int *d_cell = nullptr;
+    HIPCHK(hipMalloc(&d_off, (NC + 1) * sizeof(long)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&d[k], Pn * sizeof(double)));
+    HIPCHK(hipMalloc(&d_sp, Pn));
+    HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
+    HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
+    launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double *outs[6] = {x, y, z, px, py, pz};
+    for (int k = 0; k < 6; k++)
+        if (outs[k] && P) HIPCHK(hipMemcpy(outs[k], d[k], P * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<unsigned char> sp(Pn);
+    if (P) HIPCHK(hipMemcpy(sp.data(), d_sp, P, hipMemcpyDeviceToHost));
+    if (cell && P) HIPCHK(hipMemcpy(cell, d_cell, P * sizeof(int), hipMemcpyDeviceToHost));
+    for (long j = 0; j < P; j++) {
+        int s = sp[j];
+        if (sort) sort[j] = c->slot_sort[s];
+        if (m) m[j] = c->sp.m[s];
+        if (q_m) q_m[j] = c->sp.q_m[s];
+    }
+    hipFree(d_off);
+    for (int k = 0; k < 6; k++) hipFree(d[k]);
+    hipFree(d_sp);
+    hipFree(d_cell);
+    return PIC_SUCCESS;
+}
+
+int pic_moments(pic_ctx *c, double *out27) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out27) return PIC_ERR_INVALID;
+    launch_moments(c->NC, c->p, c->sp, c->d_slot_sort, c->d_part, c->npart, c->d_red, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, 27 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::memcpy(out27, c->h_red, 27 * sizeof(double));
+    return PIC_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+/*
+ * pic_internal.h -- context layout and kernel launch interfaces shared by
+ * the host API (pic_api.hip) and the kernel files (k_*.hip).  Not part of the
+ * public ABI.
+ *
+ * HBM layout (one GPU):
+ *   fields  12 ghosted arrays (nx+2)(ny+2)(nz+2), k fastest, the reference's
+ *           flat index n = (i(ny+2) + l)(nz+2) + k (cell.h:796-801);
+ *   particles  bucketed SoA: bulk cell c = (i*ny + l)*nz + k owns slots
+ *           [c*cap, c*cap + count[c]) of x, y, z, px, py, pz (FP64) and a
+ *           one-byte species slot; per-species weight and q/m live in a table;
+ *   departures  one fixed region of qcap entries per tile of cells.
+ */
+#ifndef PIC_INTERNAL_H
+#define PIC_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pic_amd.h"
+
+#define PIC_MAX_SLOTS 8
+#define PIC_BLOCK 256
+
+/* status flags raised on the device */
+enum {
+    PIC_F_TOO_MANY = 1,
+    PIC_F_QUEUE_OVERFLOW = 2,
+    PIC_F_BUCKET_OVERFLOW = 4,
+    PIC_F_RANGE = 8
+};
+
+struct DevStatus {
+    unsigned int flags;
+    int max_dep;
+    unsigned long long departures;
+    unsigned long long dropped;
+    unsigned long long duplicated;
+};
+
+/* geometry shared by every kernel */
+struct Grid {
+    int Nx, Ny, Nz;      /* bulk cells */
+    int sy, sx;          /* ghosted strides: sy = Nz+2, sx = (Ny+2)(Nz+2) */
+    long G;              /* ghosted nodes */
+    double Lx, Ly, Lz, hx, hy, hz, tau;
+};
+
+struct FieldPtrs {
+    double *f[PIC_NFIELDS];
+};
+
+/* particle storage (FP64 plain form) */
+struct PartPtrs {
+    double *p[6];            /* x, y, z, pu, pv, pw */
+    unsigned char *sp;       /* species slot */
+    int *count;              /* per bulk cell */
+    int cap;
+};
+
+struct QueuePtrs {
+    double *q[6];
+    unsigned char *sp;
+    int *dest;               /* destination bulk cell */
+    int *slot;               /* slot claimed by the arrival kernel */
+    int *count;              /* per tile */
+    int qcap;                /* entries per tile */
+};
+
+struct TileGeom {
+    int TX, TY, TZ;
+    int ntx, nty, ntz;
+    long ntiles;
+};
+
+struct SpeciesTable {
+    double m[PIC_MAX_SLOTS];
+    double q_m[PIC_MAX_SLOTS];
+};
+
+/* kernel launchers (k_fields.hip) */
+void launch_h_half(const Grid &g, FieldPtrs f, hipStream_t s);
+void launch_h2(const Grid &g, FieldPtrs f, hipStream_t s);
+void launch_e(const Grid &g, FieldPtrs f, hipStream_t s);
+void launch_e_periodic(const Grid &g, FieldPtrs f, hipStream_t s);
+void launch_fold_j(const Grid &g, FieldPtrs f, hipStream_t s);
+void launch_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks, double *out,
+                   hipStream_t s);
+void launch_reset_status(DevStatus *st, hipStream_t s);
+TileGeom make_tile_geom(int Nx, int Ny, int Nz);
+
+/* particle launchers (k_particles_f64.hip) */
+size_t push_lds_bytes(const TileGeom &t, int cap);
+void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
+                             QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
+                             hipStream_t s);
+void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s);
+void launch_scatter_load(const long *offsets, long ncell, const double *src[6],
+                         const unsigned char *srcsp, PartPtrs p, hipStream_t s);
+void launch_repack(long ncell, PartPtrs from, PartPtrs to, hipStream_t s);
+void launch_fix_arrivals(const TileGeom &t, QueuePtrs q, int oldcap, PartPtrs to, hipStream_t s);
+void launch_gather_particles(long ncell, const long *offsets, PartPtrs p, double *dst[6],
+                             unsigned char *dsp, int *dcell, hipStream_t s);
+void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort,
+                    double *partials, int nblocks, double *out27, hipStream_t s);
+
+#endif

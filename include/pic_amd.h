@@ -1,0 +1,219 @@
+/*
+ * pic_amd.h -- C ABI of the MI355X-native EM-PIC time step (libpic_amd.so).
+ *
+ * Drop-in boundary for the reference's Plasma::Step (gpu_plasma.h:605-620).
+ * Plain C types only: pointers, sizes, ints and doubles.  Every entry point
+ * returns an int status (0 = success) and never exits the process.
+ *
+ * Ownership: a pic_ctx owns all device memory it allocates and frees it in
+ * pic_destroy.  Host buffers passed in or out belong to the caller; copies
+ * are synchronous.  One host thread per context; a context is not
+ * re-entrant.  All device work of a context runs on its own HIP stream.
+ */
+#ifndef PIC_AMD_H
+#define PIC_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PIC_API_VERSION 1
+
+/* ------------------------------------------------------------------------
+ * archAPI: the reference's backend seam, same names, constants and int codes
+ * (archAPI.h:8-11, 92-125), implemented directly on the HIP runtime.
+ * ---------------------------------------------------------------------- */
+#define HOST_TO_DEVICE   -131313   /* archAPI.h:8  */
+#define HOST_TO_HOST     -131314   /* archAPI.h:9  */
+#define DEVICE_TO_HOST   -131315   /* archAPI.h:10 */
+#define DEVICE_TO_DEVICE -131316   /* archAPI.h:11 */
+
+/* archAPI.h:103 / archAPI.cu:37-55 -- returns the HIP error code (0 ok) */
+int MemoryCopy(void *dst, void *src, size_t size, int dir);
+/* archAPI.h:107 / archAPI.cu:58-63 -- device allocation */
+int MemoryAllocate(void **dst, size_t size);
+/* archAPI.h:109 -- free and total device memory in bytes */
+int GetDeviceMemory(size_t *m_free, size_t *m_total);
+/* archAPI.h:111 -- device memset */
+int MemorySet(void *s, int c, size_t n);
+/* archAPI.h:114 / archAPI.cu:96-100 */
+int ThreadSynchronize(void);
+/* archAPI.h:117 -- host-side device synchronize */
+int DeviceSynchronize(void);
+/* archAPI.h:118 / archAPI.cu:111-120 */
+int getLastError(void);
+/* archAPI.h:92 / archAPI.cu:142-146 */
+const char *getErrorString(int err);
+/* archAPI.h:94 / archAPI.cu:15-19 */
+int SetDevice(int n);
+/* frees memory from MemoryAllocate (the reference never frees: gpu_plasma.h:1629) */
+int MemoryFree(void *p);
+
+/* ------------------------------------------------------------------------
+ * Status codes.  Positive codes are this library's; a HIP runtime error is
+ * reported as PIC_ERR_HIP and its text by pic_last_hip_error().
+ * ---------------------------------------------------------------------- */
+enum pic_status {
+    PIC_SUCCESS = 0,
+    PIC_ERR_INVALID = 1,            /* bad argument or unsupported shape */
+    PIC_ERR_HIP = 2,                /* HIP runtime call failed */
+    PIC_ERR_NO_DEVICE = 3,          /* no GPU visible */
+    PIC_ERR_TOO_MANY_PARTICLES = 4, /* > departure_cap leavers in one cell: the
+                                       reference's TOO_MANY_PARTICLES exit
+                                       (cell.h:27, kernels.cu:193-203,
+                                       gpu_plasma.h:1953-1957) */
+    PIC_ERR_QUEUE_OVERFLOW = 5,     /* departure queue of a tile overflowed */
+    PIC_ERR_NOMEM = 6,              /* device allocation failed */
+    PIC_ERR_STATE = 7,              /* call out of order (e.g. no particles) */
+    PIC_ERR_CELL_RANGE = 8          /* a particle's stencil left its tile box */
+};
+
+/* field ids: the 12 ghosted arrays of init.cu:310-331 */
+enum pic_field {
+    PIC_EX = 0, PIC_EY, PIC_EZ, PIC_HX, PIC_HY, PIC_HZ,
+    PIC_JX, PIC_JY, PIC_JZ, PIC_QX, PIC_QY, PIC_QZ, PIC_NFIELDS
+};
+
+/* particle sorts, particle_types.h:11 */
+enum pic_sort { PIC_ION = 0, PIC_PLASMA_ELECTRON = 1, PIC_BEAM_ELECTRON = 2 };
+
+enum pic_precision {
+    PIC_FP64 = 0,   /* plain form: reproduces the reference's FP64 step */
+    PIC_FP32 = 1    /* precision form: FP32 push/gather/curl, int64 J */
+};
+
+/* Constructor arguments of Plasma (gpu_plasma.h:1574-1589) plus the build's
+ * own layout and precision controls. */
+typedef struct pic_params {
+    int nx, ny, nz;          /* bulk cells per axis (Plasma ctor nx,ny,nz) */
+    double lx, ly, lz;       /* domain size (Plasma ctor lx,ly,lz) */
+    double ni;               /* plasma density (Plasma ctor ni) */
+    int n_per_cell;          /* particles per cell (Plasma ctor n_per_cell) */
+    double ion_q_m;          /* ion charge/mass (Plasma ctor q_m) */
+    double tau;              /* time step (Plasma ctor TAU) */
+    int precision;           /* enum pic_precision */
+    int departure_cap;       /* PARTICLES_FLYING_ONE_DIRECTION (cell.h:25) = 50;
+                                <= 0 disables the check */
+    int bucket_capacity;     /* particle slots per cell; 0 = automatic */
+    int device;              /* HIP device ordinal */
+} pic_params;
+
+typedef struct pic_ctx pic_ctx;
+
+/* Step diagnostics, filled by pic_get_status after each step. */
+typedef struct pic_step_status {
+    long long departures;     /* leavers queued this step (copies included) */
+    long long dropped;        /* leavers accepted by no cell (reference drops
+                                 them: kernels.cu:316-370 + Cell::Insert) */
+    long long duplicated;     /* extra copies of leavers accepted by 2+ cells */
+    int max_cell_departures;  /* max leavers of one cell this step */
+    int bucket_regrows;       /* bucket capacity increases so far */
+    int bucket_capacity;      /* current particle slots per cell */
+    int flags;                /* bit0 too many, bit1 queue overflow,
+                                 bit2 bucket overflow (repaired), bit3 range */
+} pic_step_status;
+
+void pic_params_default(pic_params *p);
+int pic_create(const pic_params *p, pic_ctx **out);
+int pic_destroy(pic_ctx *ctx);
+const char *pic_error_string(int status);
+const char *pic_last_hip_error(void);
+
+/* Plasma::addParticleListToCells (init.cu:602-630): each particle goes to
+ * the cell getPointCell gives (cell.h:593-598) if Cell::Insert accepts it
+ * (isPointInCell, cell.h:717); otherwise it is dropped and counted.
+ * m is the charge weight (Particle::m), q_m the charge/mass ratio. */
+int pic_load_particles(pic_ctx *ctx, int sort, const double *x, const double *y,
+                       const double *z, const double *px, const double *py,
+                       const double *pz, long n, double m, double q_m,
+                       long *n_dropped);
+/* Load one ghosted field array of (nx+2)(ny+2)(nz+2) doubles, k fastest
+ * (cell.h:796-801).  Replaces InitFields / copyFieldsToGPU
+ * (init.cu:355-372, service_functions.cu:221-337). */
+int pic_load_field(pic_ctx *ctx, int which, const double *host);
+int pic_get_field(pic_ctx *ctx, int which, double *host);
+
+/* One full time step: Plasma::Step(nt) (gpu_plasma.h:605-620).  Returns
+ * after the stream completes; *energy (may be NULL) receives the
+ * getElectricEnergy() value that Diagnose(nt) writes (gpu_plasma.h:621-686). */
+int pic_step(pic_ctx *ctx, int nt, double *energy);
+/* nsteps steps from nt0 with one synchronisation at the end; energies[i]
+ * (may be NULL) receives the energy after step nt0+i. */
+int pic_run(pic_ctx *ctx, int nt0, int nsteps, double *energies);
+
+/* Stage entry points: the launch seam of Plasma::Step, in this order.
+ * Each enqueues on the context stream and returns without synchronising. */
+int pic_yee_h_half(pic_ctx *ctx);   /* MagneticStageOne + zero J
+                                       (gpu_plasma.h:524-538, 2058-2069) */
+int pic_push_deposit(pic_ctx *ctx); /* tile gather + Boris push + current
+                                       deposit + tile->grid J + departures
+                                       (gpu_plasma.h:1006-1036, 1757-1855,
+                                       1857-1967) */
+int pic_migrate(pic_ctx *ctx);      /* arrivals (gpu_plasma.h:1969-2033) */
+int pic_fold_j(pic_ctx *ctx);       /* SetPeriodicCurrents (gpu_plasma.h:980-993) */
+int pic_yee_h2(pic_ctx *ctx);       /* MagneticFieldStageTwo (gpu_plasma.h:540-551) */
+int pic_yee_e(pic_ctx *ctx);        /* ElectricFieldTrace x3 (gpu_plasma.h:455-481) */
+int pic_e_periodic(pic_ctx *ctx);   /* periodic E ghosts (gpu_plasma.h:432-452, 486-505) */
+int pic_energy(pic_ctx *ctx, double *energy); /* getElectricEnergy, synchronises */
+int pic_sync(pic_ctx *ctx);         /* wait for the stream; check step status */
+
+int pic_get_status(pic_ctx *ctx, pic_step_status *st);
+long pic_particle_count(pic_ctx *ctx);
+/* per bulk cell particle counts, index (i*ny + l)*nz + k */
+int pic_get_cell_counts(pic_ctx *ctx, int *counts);
+/* particles in cell order; any output pointer may be NULL.  cell is the bulk
+ * index of the owning cell.  *n_out receives the count (error if > cap). */
+int pic_get_particles(pic_ctx *ctx, double *x, double *y, double *z, double *px,
+                      double *py, double *pz, double *m, double *q_m, int *sort,
+                      int *cell, long cap, long *n_out);
+/* per sort (3) x 9 moments, reduced in FP64: N, sum x, y, z, sum px, py, pz,
+ * sum (m/q_m)(gamma-1), sum |px|+|py|+|pz| */
+int pic_moments(pic_ctx *ctx, double *out27);
+
+/* ------------------------------------------------------------------------
+ * Seeded initial conditions (host only; the step never calls these).
+ * Generators restate the reference's maxwell.cu: Wichmann-Hill uniform
+ * g05cae_ (147-179) and the table-driven normal g05dde_ (77-144).
+ * ---------------------------------------------------------------------- */
+typedef struct pic_rng {
+    int ix, iy, iz;     /* Wichmann-Hill state (cag05a_, maxwell.cu:52) */
+    double store1;      /* g05dde_ carry (cag05b_, maxwell.cu:47) */
+} pic_rng;
+
+void pic_rng_seed(pic_rng *r, int ix, int iy, int iz);    /* store1 = 1.0 */
+double pic_rng_uniform(pic_rng *r);                       /* g05cae_ */
+double pic_rng_normal(pic_rng *r, double mean, double sd);/* g05dde_ */
+
+typedef struct pic_species_spec {
+    int sort;                 /* enum pic_sort */
+    int ppc;                  /* particles per cell (peak count if profiled) */
+    double density;           /* |charge density| n of the species */
+    double charge;            /* +1 ions, -1 electrons */
+    double q_m;               /* charge/mass */
+    double drift[3];          /* mean velocity (units of c) */
+    double vth[3];            /* velocity spread (units of c) */
+    int profile;              /* 0 uniform; 1 transverse Gaussian in y,z */
+    double sigma_cells;       /* Gaussian width in cells (profile 1) */
+    double perturb[8];        /* amplitudes a_k of x-displacement
+                                 a_k sin(2 pi k x / Lx), k = 1..8 */
+} pic_species_spec;
+
+/* Number of particles pic_generate_species will produce for this grid. */
+long pic_species_count(const pic_params *grid, const pic_species_spec *s);
+/* Fills x..pz (capacity cap) for one species and returns its weight m in
+ * *m_out.  stream_mode 0: one Wichmann-Hill stream continued through *rng in
+ * cell order (i-major, k fastest), per particle x,y,z then ux,uy,uz;
+ * stream_mode 1: an independent stream per cell seeded from (seed, cell),
+ * generated on nthreads host threads (same result for any thread count). */
+int pic_generate_species(const pic_params *grid, const pic_species_spec *s,
+                         pic_rng *rng, int stream_mode, unsigned long long seed,
+                         int nthreads, double *x, double *y, double *z,
+                         double *px, double *py, double *pz, long cap,
+                         long *n_out, double *m_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PIC_AMD_H */
