@@ -73,8 +73,8 @@ EXPORTS = [
     "pic_step", "pic_run", "pic_yee_h_half", "pic_push_deposit", "pic_migrate",
     "pic_fold_j", "pic_yee_h2", "pic_yee_e", "pic_e_periodic", "pic_energy",
     "pic_sync",
-    # readback
-    "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
+    # readback and timing
+    "pic_profile", "pic_profile_read", "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
     "pic_get_particles", "pic_moments",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
@@ -128,6 +128,8 @@ def load_library(path=LIB_PATH):
         "pic_energy": (C.c_int, [vp, dp]),
         "pic_sync": (C.c_int, [vp]),
         "pic_get_status": (C.c_int, [vp, C.POINTER(PicStepStatus)]),
+        "pic_profile": (C.c_int, [vp, C.c_int]),
+        "pic_profile_read": (C.c_int, [vp, dp, lp, C.c_int]),
         "pic_particle_count": (C.c_long, [vp]),
         "pic_get_cell_counts": (C.c_int, [vp, ip]),
         "pic_get_particles": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip,
@@ -262,6 +264,20 @@ class Pic:
         st = PicStepStatus()
         _check(self.lib.pic_get_status(self._ctx, C.byref(st)), "pic_get_status")
         return {f: getattr(st, f) for f, _ in PicStepStatus._fields_}
+
+    STAGES = ["h_half", "push_deposit", "migrate", "fold_j", "h2", "e", "e_periodic",
+              "energy"]
+
+    def profile(self, enable=True):
+        _check(self.lib.pic_profile(self._ctx, 1 if enable else 0), "pic_profile")
+
+    def profile_read(self):
+        ms = np.zeros(len(self.STAGES))
+        n = np.zeros(len(self.STAGES), dtype=np.int64)
+        _check(self.lib.pic_profile_read(self._ctx, _dptr(ms),
+                                         n.ctypes.data_as(C.POINTER(C.c_long)), len(ms)),
+               "pic_profile_read")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.STAGES)}
 
     def particle_count(self):
         return int(self.lib.pic_particle_count(self._ctx))

@@ -65,6 +65,11 @@ struct pic_ctx {
     bool committed = false;
     int regrows = 0;
     pic_step_status last{};
+    /* stage timing */
+    bool prof = false;
+    hipEvent_t ev[PIC_NSTAGES + 1] = {};
+    double prof_ms[PIC_NSTAGES] = {0};
+    long prof_n[PIC_NSTAGES] = {0};
 };
 
 namespace {
@@ -215,6 +220,21 @@ int check_ready(pic_ctx *c) {
     return PIC_SUCCESS;
 }
 
+inline void mark(pic_ctx *c, int k) {
+    if (c->prof) (void)hipEventRecord(c->ev[k], c->stream);
+}
+
+void accumulate_profile(pic_ctx *c) {
+    if (!c->prof) return;
+    for (int k = 0; k < PIC_NSTAGES; k++) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+            c->prof_ms[k] += ms;
+            c->prof_n[k] += 1;
+        }
+    }
+}
+
 int finish_step(pic_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -360,6 +380,8 @@ int pic_destroy(pic_ctx *c) {
     if (c->d_red) hipFree(c->d_red);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->d_slot_sort) hipFree(c->d_slot_sort);
+    for (int k = 0; k <= PIC_NSTAGES; k++)
+        if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return PIC_SUCCESS;
@@ -507,18 +529,28 @@ int pic_step(pic_ctx *c, int nt, double *energy) {
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->dev));
     launch_reset_status(c->d_st, c->stream);
+    mark(c, PIC_STAGE_H_HALF);
     launch_h_half(c->g, c->f, c->stream);
+    mark(c, PIC_STAGE_PUSH_DEPOSIT);
     launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap, c->d_st,
                             c->stream);
+    mark(c, PIC_STAGE_MIGRATE);
     launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    mark(c, PIC_STAGE_FOLD_J);
     launch_fold_j(c->g, c->f, c->stream);
+    mark(c, PIC_STAGE_H2);
     launch_h2(c->g, c->f, c->stream);
+    mark(c, PIC_STAGE_E);
     launch_e(c->g, c->f, c->stream);
+    mark(c, PIC_STAGE_E_PERIODIC);
     launch_e_periodic(c->g, c->f, c->stream);
+    mark(c, PIC_STAGE_ENERGY);
     launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
+    mark(c, PIC_NSTAGES);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     rc = finish_step(c);
+    accumulate_profile(c);
     if (energy) *energy = c->h_red[0];
     return rc;
 }
@@ -541,6 +573,27 @@ int pic_get_status(pic_ctx *c, pic_step_status *st) {
         long long s = 0;
         for (int v : qc) s += v;
         st->departures = s;
+    }
+    return PIC_SUCCESS;
+}
+
+int pic_profile(pic_ctx *c, int enable) {
+    if (!c) return PIC_ERR_INVALID;
+    if (enable && !c->ev[0])
+        for (int k = 0; k <= PIC_NSTAGES; k++) HIPCHK(hipEventCreate(&c->ev[k]));
+    c->prof = enable != 0;
+    for (int k = 0; k < PIC_NSTAGES; k++) {
+        c->prof_ms[k] = 0.0;
+        c->prof_n[k] = 0;
+    }
+    return PIC_SUCCESS;
+}
+
+int pic_profile_read(pic_ctx *c, double *ms, long *launches, int nstages) {
+    if (!c || nstages < 0) return PIC_ERR_INVALID;
+    for (int k = 0; k < nstages && k < PIC_NSTAGES; k++) {
+        if (ms) ms[k] = c->prof_ms[k];
+        if (launches) launches[k] = c->prof_n[k];
     }
     return PIC_SUCCESS;
 }

@@ -160,6 +160,17 @@ int pic_energy(pic_ctx *ctx, double *energy); /* getElectricEnergy, synchronises
 int pic_sync(pic_ctx *ctx);         /* wait for the stream; check step status */
 
 int pic_get_status(pic_ctx *ctx, pic_step_status *st);
+
+/* Stage timing with HIP events recorded on the context stream around every
+ * stage that pic_step launches (ids below).  pic_profile(ctx, 1) enables and
+ * clears it; pic_profile_read fills total milliseconds and launch counts per
+ * stage for the steps run since. */
+enum pic_stage_id {
+    PIC_STAGE_H_HALF = 0, PIC_STAGE_PUSH_DEPOSIT, PIC_STAGE_MIGRATE, PIC_STAGE_FOLD_J,
+    PIC_STAGE_H2, PIC_STAGE_E, PIC_STAGE_E_PERIODIC, PIC_STAGE_ENERGY, PIC_NSTAGES
+};
+int pic_profile(pic_ctx *ctx, int enable);
+int pic_profile_read(pic_ctx *ctx, double *ms, long *launches, int nstages);
 long pic_particle_count(pic_ctx *ctx);
 /* per bulk cell particle counts, index (i*ny + l)*nz + k */
 int pic_get_cell_counts(pic_ctx *ctx, int *counts);

@@ -1,0 +1,130 @@
+"""C-ABI surface and the host-side seeded initializer (no GPU needed).
+
+The library is loaded on the CPU: every function include/pic_amd.h declares
+must be exported; only host-only entry points are called here.
+"""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pic_amd.h")
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    text = re.sub(r"typedef\s+struct\s+\w*\s*\{.*?\}\s*\w+\s*;", "", text, flags=re.S)
+    text = re.sub(r"enum\s+\w*\s*\{.*?\}\s*;", "", text, flags=re.S)
+    names = re.findall(r"\b([A-Za-z_]\w*)\s*\([^;{}]*\)\s*;", text)
+    return sorted(set(names))
+
+
+def test_header_declares_expected_entry_points():
+    names = header_functions()
+    for must in ("MemoryCopy", "MemoryAllocate", "getErrorString", "pic_create", "pic_step",
+                 "pic_push_deposit", "pic_migrate", "pic_get_particles", "pic_generate_species"):
+        assert must in names
+    assert len(names) >= 39
+
+
+def test_library_exports_every_declared_symbol(picmod):
+    lib = picmod.load_library()
+    names = header_functions()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    out = subprocess.run(["nm", "-D", "--defined-only", picmod.LIB_PATH],
+                         stdout=subprocess.PIPE, text=True, check=True).stdout
+    exported = {l.split()[-1] for l in out.splitlines() if " T " in l}
+    assert set(names) <= exported
+    assert set(picmod.EXPORTS) == set(names)
+
+
+def test_archapi_constants_match_reference(picmod):
+    # archAPI.h:8-11
+    assert (picmod.HOST_TO_DEVICE, picmod.HOST_TO_HOST, picmod.DEVICE_TO_HOST,
+            picmod.DEVICE_TO_DEVICE) == (-131313, -131314, -131315, -131316)
+    text = open(HEADER).read()
+    for name, val in (("HOST_TO_DEVICE", -131313), ("DEVICE_TO_DEVICE", -131316)):
+        assert re.search(rf"#define {name}\s+{val}\b", text)
+
+
+def test_params_struct_layout(picmod):
+    p = picmod.default_params()
+    assert (p.nx, p.ny, p.nz, p.departure_cap) == (32, 8, 8, 50)
+    assert p.tau == 0.05 and p.precision == picmod.FP64
+    assert C.sizeof(picmod.PicParams) == 88
+
+
+def test_host_error_strings(picmod):
+    lib = picmod.load_library()
+    assert lib.pic_error_string(4).decode().startswith("too many")
+    assert lib.pic_error_string(0).decode() == "success"
+
+
+def test_generate_single_stream_draw_order(picmod):
+    """stream_mode 0 starts from the reference seeds (1,255,25555) and draws
+    x,y,z uniforms first: the first particle's x is (0 + u1) * hx."""
+    g = np.load(os.path.join(GOLDEN, "rng_ref.npz"))
+    cfg = picmod.beam_plasma_config(0)
+    params = picmod.make_params(cfg["n"], cfg["l"], cfg["tau"])
+    x, y, z, px, py, pz, m = picmod.generate_species(params, cfg["species"][0], None, 0)
+    hx = params.lx / params.nx
+    hy = params.ly / params.ny
+    assert x[0] == (0.0 + g["uniform"][0]) * hx
+    assert y[0] == (0.0 + g["uniform"][1]) * hy
+    assert m == pytest.approx(1.01 / 16, rel=1e-15)
+    assert np.all(px == 0.0)     # cold ions
+
+
+def test_generate_per_cell_streams_thread_independent(picmod):
+    cfg = picmod.beam_plasma_config(0)
+    params = picmod.make_params(cfg["n"], cfg["l"], cfg["tau"])
+    a = picmod.generate_species(params, cfg["species"][1], None, 1, seed=99, nthreads=1)
+    b = picmod.generate_species(params, cfg["species"][1], None, 1, seed=99, nthreads=5)
+    for u, v in zip(a[:6], b[:6]):
+        assert np.array_equal(u, v)
+    # every particle inside its generating cell (cell-major order)
+    n = cfg["n"]
+    cells = np.repeat(np.arange(n[0] * n[1] * n[2]), 16)
+    ci = np.floor(a[0] / (params.lx / n[0])).astype(int)
+    assert np.array_equal(ci, cells // (n[1] * n[2]))
+
+
+def test_generate_beam_momentum_and_weights(picmod):
+    cfg = picmod.beam_plasma_config(0)
+    params = picmod.make_params(cfg["n"], cfg["l"], cfg["tau"])
+    x, y, z, px, py, pz, m = picmod.generate_species(params, cfg["species"][2], None, 1, seed=1)
+    # u = 0.3 +- 0.005 -> p = u / sqrt(1 - u^2) (maxwell.cu:521-531)
+    assert px.mean() == pytest.approx(0.3 / np.sqrt(1 - 0.09), rel=2e-3)
+    assert py.std() == pytest.approx(0.005 / np.sqrt(1 - 0.09), rel=5e-2)
+    assert m == pytest.approx(-0.01 / 16, rel=1e-15)
+
+
+def test_generate_gaussian_profile_counts(picmod):
+    cfg = picmod.beam_plasma_config(4)
+    params = picmod.make_params((4, 64, 64), (0.4, 6.4, 6.4), 0.05)
+    spec = cfg["species"][2]
+    lib = picmod.load_library()
+    n = lib.pic_species_count(C.byref(params), C.byref(spec))
+    yc = np.arange(64) + 0.5 - 32
+    f = np.exp(-(yc[:, None] ** 2 + yc[None, :] ** 2) / (2 * 8.0 ** 2))
+    assert n == 4 * int(np.floor(spec.ppc * f + 0.5).sum())
+
+
+def test_create_without_gpu_reports_no_device(picmod):
+    """pic_create only counts devices before allocating; on a host without a
+    GPU it must fail with PIC_ERR_NO_DEVICE, not crash."""
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = picmod.load_library()
+    p = picmod.default_params()
+    ctx = C.c_void_p()
+    assert lib.pic_create(C.byref(p), C.byref(ctx)) == 3

@@ -1,0 +1,394 @@
+"""Pin the oracle's restatement with mathematics that fixes each stage.
+
+Each test is chosen so that a plausible restatement mistake (a swapped
+index, a dropped term, a wrong sign or staggering) fails it:
+  Boris push     closed forms (uniform E; uniform B rotation; E x B drift)
+  gather         a field linear in each coordinate, on each component's own
+                 staggering, interpolates exactly; H recovered from rotation
+  deposit        textbook Villasenor-Buneman current of a straight segment
+                 (no crossing and single-face crossings), and the discrete
+                 continuity equation for every crossing case
+  Yee            the leapfrog recurrence of a discrete eigenmode, the Gauss
+                 law residual conserved, div H preserved in the interior
+  migration      brute-force cell membership, exact per-cell counts
+  energy         fsum of E^2
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import div_staggered, load_workload, rho_cic
+
+TAU = 0.05
+
+
+def small_grid(n=(8, 6, 5), h=0.1):
+    return n, tuple(k * h for k in n), tuple(h for _ in n)
+
+
+# ---------------------------------------------------------------- Boris push
+def test_boris_uniform_E_closed_form(orc):
+    E = np.array([0.3, -0.2, 0.1])
+    p = np.array([0.01, 0.02, -0.03])
+    x = np.array([1.0, 1.0, 1.0])
+    q_m = -1.0
+    p1 = p.copy()
+    for _ in range(50):
+        p1, _x = orc.boris(E, np.zeros(3), TAU, q_m, x, p1)
+    assert np.allclose(p1, p + 50 * q_m * TAU * E, rtol=0, atol=1e-14)
+
+
+def test_boris_uniform_B_rotation(orc):
+    H = np.array([0.0, 0.0, 0.8])
+    p = np.array([0.2, 0.0, 0.05])
+    q_m = -1.0
+    gam = math.sqrt(1 + p @ p)
+    b = 0.5 * q_m * TAU / gam * np.linalg.norm(H)
+    theta = 2 * math.atan(abs(b))
+    p1, _ = orc.boris(np.zeros(3), H, TAU, q_m, np.zeros(3), p)
+    assert np.linalg.norm(p1) == pytest.approx(np.linalg.norm(p), rel=1e-15)
+    ang = math.atan2(p1[1], p1[0]) - math.atan2(p[1], p[0])
+    assert abs(ang) == pytest.approx(theta, rel=1e-12)
+    assert p1[2] == p[2]
+    # electrons (q_m<0) gyrate counter-clockwise about +z
+    assert ang > 0
+
+
+def test_boris_exb_drift(orc):
+    E = np.array([0.0, 0.01, 0.0])
+    H = np.array([0.0, 0.0, 1.0])
+    p = np.zeros(3)
+    x = np.zeros(3)
+    xs = []
+    for _ in range(4000):
+        p, x1 = orc.boris(E, H, TAU, -1.0, x, p)
+        x = x1
+        xs.append(x[0])
+    v = (xs[-1] - xs[999]) / (3000 * TAU)
+    assert v == pytest.approx(0.01, rel=2e-2)   # |E x B|/B^2 along +x
+
+
+# -------------------------------------------------------------- field gather
+# staggering of each component: 'n' node (g h), 'c' centre ((g-0.5) h),
+# restated from the index pairs of GetElectricField / GetMagneticField
+STAG = {"Ex": "ncc", "Ey": "cnc", "Ez": "ccn", "Hx": "cnn", "Hy": "ncn", "Hz": "nnc"}
+
+
+def linear_field(n, h, stag, alpha, beta):
+    g = [np.arange(k + 2, dtype=np.float64) for k in n]
+    pos = [(g[d] - (0.5 if stag[d] == "c" else 0.0)) * h[d] for d in range(3)]
+    X, Y, Z = np.meshgrid(*pos, indexing="ij")
+    return (alpha + beta[0] * X + beta[1] * Y + beta[2] * Z).reshape(-1)
+
+
+def one_particle(target, pos, p=(0.0, 0.0, 0.0), q_m=-1.0, m=-0.01):
+    c = [np.array([v]) for v in pos]
+    target.load_particles(1, c[0], c[1], c[2], np.array([p[0]]), np.array([p[1]]),
+                          np.array([p[2]]), m, q_m)
+
+
+def test_gather_linear_E_exact(orc):
+    n, L, h = small_grid()
+    rng = np.random.default_rng(3)
+    beta = {c: rng.normal(0, 1, 3) for c in ("Ex", "Ey", "Ez")}
+    alpha = {"Ex": 0.3, "Ey": -0.1, "Ez": 0.2}
+    for trial in range(20):
+        pos = [rng.uniform(0.05, Lk - 0.05) for Lk in L]
+        o = orc.Oracle(n, L, TAU)
+        for i, c in enumerate(("Ex", "Ey", "Ez")):
+            o.load_field(i, linear_field(n, h, STAG[c], alpha[c], beta[c]))
+        one_particle(o, pos)
+        o.stage("push")
+        got = np.array([o.particles()[k][0] for k in ("px", "py", "pz")]) / (-1.0 * TAU)
+        want = np.array([alpha[c] + beta[c] @ np.array(pos) for c in ("Ex", "Ey", "Ez")])
+        assert np.allclose(got, want, rtol=0, atol=1e-12), (trial, got, want)
+
+
+def recover_b(p0s, p1s):
+    """Boris rotation p1 - p0 = (p1 + p0) x b, solved for b from two momenta."""
+    rows, rhs = [], []
+    for p0, p1 in zip(p0s, p1s):
+        s = p0 + p1
+        cross = np.array([[0, -s[2], s[1]], [s[2], 0, -s[0]], [-s[1], s[0], 0]])   # s x .
+        rows.append(cross)
+        rhs.append(p1 - p0)
+    b, *_ = np.linalg.lstsq(np.vstack(rows), np.concatenate(rhs), rcond=None)
+    return b
+
+
+def test_gather_linear_H_exact(orc):
+    n, L, h = small_grid()
+    rng = np.random.default_rng(5)
+    beta = {c: rng.normal(0, 1, 3) for c in ("Hx", "Hy", "Hz")}
+    alpha = {"Hx": 0.5, "Hy": -0.4, "Hz": 0.3}
+    pmag = 0.05
+    for trial in range(10):
+        pos = [rng.uniform(0.05, Lk - 0.05) for Lk in L]
+        o = orc.Oracle(n, L, TAU)
+        for i, c in enumerate(("Hx", "Hy", "Hz")):
+            o.load_field(3 + i, linear_field(n, h, STAG[c], alpha[c], beta[c]))
+        p0s = [np.array([pmag, 0, 0]), np.array([0, pmag, 0]), np.array([0, 0, pmag])]
+        c = [np.array([v] * 3) for v in pos]
+        o.load_particles(1, c[0], c[1], c[2], np.array([pmag, 0, 0]), np.array([0, pmag, 0]),
+                         np.array([0, 0, pmag]), -0.01, -1.0)
+        o.stage("push")
+        pp = o.particles()
+        p1s = [np.array([pp["px"][j], pp["py"][j], pp["pz"][j]]) for j in range(3)]
+        # match particles by their dominant initial axis
+        p1s = sorted(p1s, key=lambda v: -abs(v[0]))[:1] + \
+            sorted(p1s, key=lambda v: -abs(v[1]))[:1] + sorted(p1s, key=lambda v: -abs(v[2]))[:1]
+        b = recover_b(p0s, p1s)
+        gam = math.sqrt(1 + pmag * pmag)
+        Hg = b / (0.5 * -1.0 * TAU / gam)
+        want = np.array([alpha[k] + beta[k] @ np.array(pos) for k in ("Hx", "Hy", "Hz")])
+        assert np.allclose(Hg, want, rtol=0, atol=1e-10), (trial, Hg, want)
+
+
+# ------------------------------------------------------------------- deposit
+def vb_segment(J, r0, r1, a, h, n, g):
+    """Textbook Villasenor-Buneman current of a straight segment inside the
+    dual cell [c_g, c_g+1] (c_g = (g - 0.5) h):
+    Jx = (a dx / tau)(Wy Wz +- dy dz / 12 hy hz) on its 4 x-edges."""
+    sh = (n[0] + 2, n[1] + 2, n[2] + 2)
+    mid = 0.5 * (r0 + r1)
+    d = r1 - r0
+    w1 = (mid - (g - 0.5) * h) / h               # weight of the upper centre
+    w0 = 1.0 - w1
+    for comp in range(3):
+        o1, o2 = [k for k in range(3) if k != comp]
+        corr = d[o1] * d[o2] / (12 * h[o1] * h[o2])
+        base = a * d[comp] / TAU
+        for s1 in (0, 1):
+            for s2 in (0, 1):
+                wa = w1[o1] if s1 else w0[o1]
+                wb = w1[o2] if s2 else w0[o2]
+                sgn = 1 if s1 == s2 else -1
+                idx = [int(v) for v in g]
+                idx[o1] += s1
+                idx[o2] += s2
+                J[comp].reshape(sh)[tuple(idx)] += base * (wa * wb + sgn * corr)
+
+
+def deposit_only(orc, n, L, pos, p, m=-0.01):
+    o = orc.Oracle(n, L, TAU)
+    one_particle(o, pos, p=p, m=m)
+    o.stage("h_half")
+    o.stage("push")
+    o.stage("deposit")
+    return o, [o.get_field(6 + k) for k in range(3)]
+
+
+def test_deposit_vb_closed_form_no_crossing(orc):
+    n, L, h = small_grid()
+    h = np.array(h)
+    rng = np.random.default_rng(11)
+    done = 0
+    while done < 30:
+        pos = rng.uniform(0.3, 0.45, 3)
+        p = rng.normal(0, 0.05, 3)
+        gam = math.sqrt(1 + p @ p)
+        r1 = pos + TAU * p / gam
+        if np.any(np.floor(pos / h + 0.5) != np.floor(r1 / h + 0.5)):
+            continue
+        _, J = deposit_only(orc, n, L, pos, p)
+        want = [np.zeros_like(J[0]) for _ in range(3)]
+        vb_segment(want, pos, r1, -0.01, h, n, np.floor(pos / h + 0.5))
+        for k in range(3):
+            assert np.allclose(J[k], want[k], rtol=0, atol=1e-15), (done, k)
+        done += 1
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_deposit_vb_single_face_crossing_straight_split(orc, axis):
+    """m = 4, 2, 1: the relay point is where the straight path crosses the
+    dual-cell face (cell.h:1346-1373), and each piece is a VB segment."""
+    n, L, h = small_grid()
+    h = np.array(h)
+    rng = np.random.default_rng(20 + axis)
+    done = 0
+    while done < 20:
+        pos = rng.uniform(0.3, 0.45, 3)
+        p = rng.normal(0, 0.02, 3)
+        p[axis] = rng.choice([-1, 1]) * rng.uniform(0.3, 0.6)
+        gam = math.sqrt(1 + p @ p)
+        r1 = pos + TAU * p / gam
+        g0, g1 = np.floor(pos / h + 0.5), np.floor(r1 / h + 0.5)
+        if not (g0[axis] != g1[axis] and all(g0[k] == g1[k] for k in range(3) if k != axis)):
+            continue
+        face = (max(g0[axis], g1[axis]) - 0.5) * h[axis]
+        s = (face - pos[axis]) / (r1[axis] - pos[axis])
+        mid = pos + (r1 - pos) * s
+        mid[axis] = face
+        _, J = deposit_only(orc, n, L, pos, p)
+        want = [np.zeros_like(J[0]) for _ in range(3)]
+        vb_segment(want, pos, mid, -0.01, h, n, g0)
+        vb_segment(want, mid, r1, -0.01, h, n, g1)
+        for k in range(3):
+            assert np.allclose(J[k], want[k], rtol=0, atol=1e-12), (axis, done, k)
+        done += 1
+
+
+def test_continuity_every_crossing_case(orc, picmod):
+    """rho^{n+1} - rho^n + tau div J = 0 on the periodic centre lattice for a
+    full config-0 step (all relay cases m = 0..7 occur among 98k particles)."""
+    params, parts = picmod.generate_workload(0)
+    n = (params.nx, params.ny, params.nz)
+    L = (params.lx, params.ly, params.lz)
+    h = tuple(a / b for a, b in zip(L, n))
+    o = orc.Oracle(n, L, params.tau)
+    load_workload(o, parts)
+    for nt in range(1, 4):
+        o.step(nt)
+    for nt in range(4, 7):
+        r0 = rho_cic(o.particles(), n, h)
+        o.step(nt)
+        r1 = rho_cic(o.particles(), n, h)
+        J = [o.get_field(6 + k) for k in range(3)]
+        res = r1 - r0 + params.tau * div_staggered(*J, n, h)
+        assert np.abs(res).max() < 1e-13 * np.abs(r1).max()
+
+
+def test_gauss_law_residual_conserved(orc, picmod):
+    params, parts = picmod.generate_workload(0)
+    n = (params.nx, params.ny, params.nz)
+    L = (params.lx, params.ly, params.lz)
+    h = tuple(a / b for a, b in zip(L, n))
+    o = orc.Oracle(n, L, params.tau)
+    load_workload(o, parts)
+    g0 = None
+    for nt in range(1, 11):
+        o.step(nt)
+        E = [o.get_field(k) for k in range(3)]
+        g = div_staggered(*E, n, h) - rho_cic(o.particles(), n, h)
+        if g0 is None:
+            g0 = g
+        assert np.abs(g - g0).max() < 1e-13
+
+
+# ----------------------------------------------------------------------- Yee
+def yee_mode_series(orc, comp, axis, steps=40, mode=2):
+    n = (12, 10, 14)
+    L = tuple(0.1 * k for k in n)
+    h = 0.1
+    o = orc.Oracle(n, L, TAU)
+    ci = {"Ex": 0, "Ey": 1, "Ez": 2}[comp]
+    g = np.arange(n[axis] + 2, dtype=np.float64)
+    prof = np.sin(2 * np.pi * mode * (g - 0.5) / n[axis])
+    shape = [1, 1, 1]
+    shape[axis] = n[axis] + 2
+    arr = np.broadcast_to(prof.reshape(shape), tuple(k + 2 for k in n)).copy()
+    o.load_field(ci, arr)
+    lam = 4 * math.sin(np.pi * mode / n[axis]) ** 2 / h ** 2
+    probe = [3, 4, 5]
+    probe[axis] = 2
+    idx = np.ravel_multi_index(tuple(probe), tuple(k + 2 for k in n))
+    series = [o.get_field(ci)[idx]]
+    for nt in range(1, steps):
+        o.step(nt)
+        series.append(o.get_field(ci)[idx])
+    s = np.array(series)
+    return s, s[2:] + s[:-2] - (2 - TAU ** 2 * lam) * s[1:-1]
+
+
+@pytest.mark.parametrize("comp,axis", [("Ey", 0), ("Ey", 2)])
+def test_yee_mode_leapfrog_recurrence(orc, comp, axis):
+    """A discrete eigenmode obeys e^{n+1} + e^{n-1} = (2 - tau^2 lam) e^n with
+    lam = 4 sin^2(k h/2)/h^2 (Yee dispersion relation).  Ey is the component
+    whose ghosts the reference keeps periodic in every direction it reads
+    them (x and z copies, gpu_plasma.h:486-505)."""
+    s, res = yee_mode_series(orc, comp, axis)
+    assert np.abs(res).max() < 1e-12
+    assert np.abs(s).max() > 0.1
+
+
+@pytest.mark.parametrize("comp,axis", [("Ez", 0), ("Ex", 2)])
+def test_yee_q2_stale_y_ghosts_break_mode(orc, comp, axis):
+    """Reference quirk Q2: Ez is copied along x only and Ex along z only, so
+    their y ghost layers keep their initial values and feed Hx / Hz at l = 0;
+    a y-uniform mode therefore stops obeying the recurrence once the boundary
+    disturbance reaches the probe.  The restatement keeps this."""
+    s, res = yee_mode_series(orc, comp, axis)
+    assert np.abs(res[:2]).max() < 1e-12
+    assert np.abs(res).max() > 1e-4
+
+
+def test_div_h_preserved_interior(orc):
+    n = (9, 8, 7)
+    L = tuple(0.1 * k for k in n)
+    h = 0.1
+    o = orc.Oracle(n, L, TAU)
+    rng = np.random.default_rng(2)
+    for k in range(3):
+        o.load_field(k, rng.normal(0, 1, (n[0] + 2) * (n[1] + 2) * (n[2] + 2)))
+    for nt in range(1, 6):
+        o.step(nt)
+        sh = tuple(k + 2 for k in n)
+        Hx, Hy, Hz = (o.get_field(3 + k).reshape(sh) for k in range(3))
+        s = np.s_[1:n[0], 1:n[1], 1:n[2]]
+        div = ((Hx[2:n[0] + 1, 1:n[1], 1:n[2]] - Hx[s]) + (Hy[1:n[0], 2:n[1] + 1, 1:n[2]] - Hy[s]) +
+               (Hz[1:n[0], 1:n[1], 2:n[2] + 1] - Hz[s])) / h
+        assert np.abs(div).max() < 1e-12, nt
+
+
+# ----------------------------------------------------------------- migration
+def test_migration_counts_match_brute_force(orc, picmod):
+    params, parts = picmod.generate_workload(0)
+    n = (params.nx, params.ny, params.nz)
+    L = (params.lx, params.ly, params.lz)
+    h = [a / b for a, b in zip(L, n)]
+    o = orc.Oracle(n, L, params.tau)
+    load_workload(o, parts)
+    total = o.particle_count()
+    for nt in range(1, 8):
+        o.step(nt)
+        p = o.particles()
+        assert len(p["x"]) == total
+        idx = []
+        for d, key in enumerate(("x", "y", "z")):
+            c = np.floor(p[key] / h[d]).astype(int)
+            # isPointInCell uses x0 = (double)i*h and x0 + h (cell.h:717)
+            lo = c * h[d]
+            c = np.where(p[key] < lo, c - 1, c)
+            c = np.where(p[key] >= c * h[d] + h[d], c + 1, c)
+            idx.append(c)
+        brute = np.zeros(n, dtype=np.int32)
+        np.add.at(brute, tuple(idx), 1)
+        assert np.array_equal(brute.reshape(-1), o.cell_counts())
+        own = np.ravel_multi_index(tuple(idx), n)
+        assert np.array_equal(own, p["cell"])
+
+
+def test_energy_is_sum_of_squares(orc, picmod):
+    params, parts = picmod.generate_workload(0)
+    n = (params.nx, params.ny, params.nz)
+    L = (params.lx, params.ly, params.lz)
+    o = orc.Oracle(n, L, params.tau)
+    load_workload(o, parts)
+    for nt in range(1, 4):
+        e, _ = o.step(nt)
+    E = [o.get_field(k) for k in range(3)]
+    want = math.fsum((E[0] ** 2 + E[1] ** 2 + E[2] ** 2).tolist())
+    assert e == pytest.approx(want, rel=1e-14)
+
+
+def test_departure_cap_flags_too_many(orc):
+    n, L, h = small_grid()
+    for count, flagged in ((50, 0), (51, 1)):
+        o = orc.Oracle(n, L, TAU, departure_cap=50)
+        x = np.full(count, 0.295)
+        y = np.linspace(0.31, 0.39, count)
+        z = np.full(count, 0.35)
+        o.load_particles(1, x, y, z, np.full(count, 0.9), np.zeros(count), np.zeros(count),
+                         -0.01, -1.0)
+        _, st = o.step(1)
+        assert st == flagged
+
+
+def test_load_drops_outside_particles(orc):
+    n, L, h = small_grid()
+    o = orc.Oracle(n, L, TAU)
+    x = np.array([-0.01, L[0], 0.05, L[0] - 1e-9])
+    d = o.load_particles(0, x, np.full(4, 0.05), np.full(4, 0.05), np.zeros(4), np.zeros(4),
+                         np.zeros(4), 0.1, 1.0 / 1836)
+    assert d == 2 and o.particle_count() == 2
