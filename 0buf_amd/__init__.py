@@ -53,6 +53,18 @@ class PicRng(C.Structure):
                 ("store1", C.c_double)]
 
 
+class PicParamFile(C.Structure):
+    _fields_ = [("tex0", C.c_double), ("tey0", C.c_double), ("tez0", C.c_double),
+                ("rimp", C.c_double), ("Tb", C.c_double), ("rbd", C.c_double),
+                ("ni", C.c_double), ("B0", C.c_double), ("lx", C.c_double),
+                ("ly", C.c_double), ("lz", C.c_double), ("py", C.c_double),
+                ("pz", C.c_double), ("bx", C.c_double), ("by", C.c_double),
+                ("bz", C.c_double), ("lp", C.c_int), ("nx", C.c_int), ("ny", C.c_int),
+                ("nz", C.c_int), ("tau", C.c_double), ("beam_plasma", C.c_int),
+                ("start_from_file", C.c_int), ("phase", C.c_int), ("ts", C.c_int),
+                ("ms", C.c_int)]
+
+
 class PicSpeciesSpec(C.Structure):
     _fields_ = [("sort", C.c_int), ("ppc", C.c_int), ("density", C.c_double),
                 ("charge", C.c_double), ("q_m", C.c_double),
@@ -75,10 +87,12 @@ EXPORTS = [
     "pic_sync",
     # readback and timing
     "pic_profile", "pic_profile_read", "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
-    "pic_get_particles", "pic_moments",
+    "pic_get_particles", "pic_moments", "pic_gauss_residual",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
     "pic_generate_species",
+    # parameter file (000_params.dat)
+    "pic_read_param_file", "pic_write_param_file", "pic_species_from_param_file",
 ]
 
 _lib = None
@@ -135,6 +149,7 @@ def load_library(path=LIB_PATH):
         "pic_get_particles": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip,
                                         C.c_long, lp]),
         "pic_moments": (C.c_int, [vp, dp]),
+        "pic_gauss_residual": (C.c_int, [vp, dp, dp, C.c_int]),
         "pic_rng_seed": (None, [C.POINTER(PicRng), C.c_int, C.c_int, C.c_int]),
         "pic_rng_uniform": (C.c_double, [C.POINTER(PicRng)]),
         "pic_rng_normal": (C.c_double, [C.POINTER(PicRng), C.c_double, C.c_double]),
@@ -143,6 +158,10 @@ def load_library(path=LIB_PATH):
                                            C.POINTER(PicRng), C.c_int, C.c_ulonglong,
                                            C.c_int, dp, dp, dp, dp, dp, dp, C.c_long, lp,
                                            dp]),
+        "pic_read_param_file": (C.c_int, [C.c_char_p, C.POINTER(PicParamFile)]),
+        "pic_write_param_file": (C.c_int, [C.c_char_p, C.POINTER(PicParamFile)]),
+        "pic_species_from_param_file": (C.c_int, [C.POINTER(PicParamFile), C.POINTER(PicParams),
+                                                  C.POINTER(PicSpeciesSpec)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -304,6 +323,14 @@ class Pic:
         res["cell"] = cell[:n]
         return res
 
+    def gauss_residual(self, set_reference=False, want_array=False):
+        """(max |res - reference|, res array or None), res = div E - rho."""
+        res = np.empty(self.ncell, dtype=np.float64) if want_array else None
+        d = C.c_double(0.0)
+        _check(self.lib.pic_gauss_residual(self._ctx, _dptr(res), C.byref(d),
+                                           1 if set_reference else 0), "pic_gauss_residual")
+        return d.value, res
+
     def moments(self):
         a = np.empty(27, dtype=np.float64)
         _check(self.lib.pic_moments(self._ctx, _dptr(a)), "pic_moments")
@@ -350,6 +377,28 @@ def beam_plasma_config(index):
                         profile=1 if index == 4 else 0, sigma_cells=8.0)
     return {"n": n, "l": tuple(k * h for k in n), "tau": 0.05, "steps": steps,
             "species": [ions, elec, beam], "bx0": 0.5 if index == 2 else 0.0}
+
+
+def read_param_file(path):
+    """Parse a 000_params.dat file (the format of writeParamsFile)."""
+    pf = PicParamFile()
+    _check(load_library().pic_read_param_file(os.fsencode(path), C.byref(pf)),
+           "pic_read_param_file")
+    return pf
+
+
+def write_param_file(path, pf):
+    _check(load_library().pic_write_param_file(os.fsencode(path), C.byref(pf)),
+           "pic_write_param_file")
+
+
+def species_from_param_file(pf):
+    """(grid params, [ion, electron, beam] specs) for a parsed parameter file."""
+    grid = default_params()
+    specs = (PicSpeciesSpec * 3)()
+    _check(load_library().pic_species_from_param_file(C.byref(pf), C.byref(grid), specs),
+           "pic_species_from_param_file")
+    return grid, list(specs)
 
 
 def make_params(n, l, tau, precision=FP64, **kw):

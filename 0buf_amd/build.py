@@ -31,6 +31,8 @@ HIP_FLAGS = [
 HIP_SOURCES = {
     "k_fields.hip": ["-ffp-contract=off"],
     "k_particles_f64.hip": ["-ffp-contract=off"],
+    "k_particles_f32.hip": ["-ffp-contract=fast"],
+    "k_gauss.hip": ["-ffp-contract=off"],
     "pic_api.hip": ["-ffp-contract=off"],
 }
 CXX_SOURCES = {"pic_init.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"]}

@@ -1,17 +1,52 @@
 /*
- * k_fields.hip -- Yee field stages of the FP64 plain form as coalesced 3-D
- * stencils over the ghosted arrays (k fastest), one node per lane.
+ * k_fields.hip -- Yee field stages as coalesced 3-D stencils over the
+ * ghosted arrays (k fastest), one node per lane.
  *
- * Built with -ffp-contract=off: each update keeps the reference's rounding
- * sequence, so these kernels are bit-identical to the CPU restatement.
- * HBM roofline: every kernel is a streaming stencil; the neighbour reads at
- * +-1 in k, +-(nz+2) and +-(ny+2)(nz+2) hit L2.
+ * Each kernel is a template over the field type T and the current type JT:
+ *   FP64 plain form   T = double, JT = double  (built with -ffp-contract=off:
+ *                     every update keeps the reference's rounding sequence,
+ *                     bit-identical to the CPU restatement);
+ *   FP32 form         T = float,  JT = int64 fixed point (2^-40), so that the
+ *                     fold and the E update read an exact, order-free J.
+ * HBM roofline: streaming stencils; neighbour reads at +-1 in k, +-(nz+2) and
+ * +-(ny+2)(nz+2) hit L2.
  */
 #include "pic_internal.h"
 
 #pragma clang fp contract(off)
 
 namespace {
+
+template <typename T, typename JT>
+struct FV {
+    T *E[3], *H[3], *Q[3];
+    JT *J[3];
+};
+
+inline FV<double, double> view(FieldPtrs f) {
+    FV<double, double> v;
+    for (int c = 0; c < 3; c++) {
+        v.E[c] = f.f[PIC_EX + c];
+        v.H[c] = f.f[PIC_HX + c];
+        v.Q[c] = f.f[PIC_QX + c];
+        v.J[c] = f.f[PIC_JX + c];
+    }
+    return v;
+}
+
+inline FV<float, long long> view(FieldPtrs32 f) {
+    FV<float, long long> v;
+    for (int c = 0; c < 3; c++) {
+        v.E[c] = f.E[c];
+        v.H[c] = f.H[c];
+        v.Q[c] = f.Q[c];
+        v.J[c] = f.J[c];
+    }
+    return v;
+}
+
+__device__ __forceinline__ double jval(double j) { return j; }
+__device__ __forceinline__ float jval(long long j) { return (float)((double)j * PIC_J_INV); }
 
 __device__ __forceinline__ void decode(const Grid &g, long n, int &i, int &l, int &k) {
     k = (int)(n % g.sy);
@@ -25,76 +60,81 @@ __device__ __forceinline__ void decode(const Grid &g, long n, int &i, int &l, in
  * MagneticFieldTrace (gpu_plasma.h:735-767), shifts from
  * getMagneticFieldTraceShifts (722-733); fused with the J := 0 of
  * SetCurrentsToZero (gpu_plasma.h:2058-2069). */
-__global__ __launch_bounds__(PIC_BLOCK) void k_h_half(Grid g, FieldPtrs f) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_h_half(Grid g, FV<T, JT> f) {
     long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (n >= g.G) return;
     int i, l, k;
     decode(g, n, i, l, k);
-    const double c1x = g.tau / g.hx, c1y = g.tau / g.hy, c1z = g.tau / g.hz;
-    const double *Ex = f.f[PIC_EX], *Ey = f.f[PIC_EY], *Ez = f.f[PIC_EZ];
+    const T c1x = (T)(g.tau / g.hx), c1y = (T)(g.tau / g.hy), c1z = (T)(g.tau / g.hz);
+    const T *Ex = f.E[0], *Ey = f.E[1], *Ez = f.E[2];
     if (l <= g.Ny && k <= g.Nz) {   /* Qx over [0,Nx+1]x[0,Ny]x[0,Nz] */
-        double e1_n1 = Ey[n + 1], e1_n = Ey[n], e2_n2 = Ez[n + g.sy], e2_n = Ez[n];
-        double t = 0.5 * (c1z * (e1_n1 - e1_n) - c1y * (e2_n2 - e2_n));
-        f.f[PIC_QX][n] = t;
-        f.f[PIC_HX][n] += t;
+        T e1_n1 = Ey[n + 1], e1_n = Ey[n], e2_n2 = Ez[n + g.sy], e2_n = Ez[n];
+        T t = (T)0.5 * (c1z * (e1_n1 - e1_n) - c1y * (e2_n2 - e2_n));
+        f.Q[0][n] = t;
+        f.H[0][n] += t;
     }
     if (i <= g.Nx && k <= g.Nz) {   /* Qy over [0,Nx]x[0,Ny+1]x[0,Nz] */
-        double e1_n1 = Ez[n + g.sx], e1_n = Ez[n], e2_n2 = Ex[n + 1], e2_n = Ex[n];
-        double t = 0.5 * (c1x * (e1_n1 - e1_n) - c1z * (e2_n2 - e2_n));
-        f.f[PIC_QY][n] = t;
-        f.f[PIC_HY][n] += t;
+        T e1_n1 = Ez[n + g.sx], e1_n = Ez[n], e2_n2 = Ex[n + 1], e2_n = Ex[n];
+        T t = (T)0.5 * (c1x * (e1_n1 - e1_n) - c1z * (e2_n2 - e2_n));
+        f.Q[1][n] = t;
+        f.H[1][n] += t;
     }
     if (i <= g.Nx && l <= g.Ny) {   /* Qz over [0,Nx]x[0,Ny]x[0,Nz+1] */
-        double e1_n1 = Ex[n + g.sy], e1_n = Ex[n], e2_n2 = Ey[n + g.sx], e2_n = Ey[n];
-        double t = 0.5 * (c1y * (e1_n1 - e1_n) - c1x * (e2_n2 - e2_n));
-        f.f[PIC_QZ][n] = t;
-        f.f[PIC_HZ][n] += t;
+        T e1_n1 = Ex[n + g.sy], e1_n = Ex[n], e2_n2 = Ey[n + g.sx], e2_n = Ey[n];
+        T t = (T)0.5 * (c1y * (e1_n1 - e1_n) - c1x * (e2_n2 - e2_n));
+        f.Q[2][n] = t;
+        f.H[2][n] += t;
     }
-    f.f[PIC_JX][n] = 0.0;
-    f.f[PIC_JY][n] = 0.0;
-    f.f[PIC_JZ][n] = 0.0;
+    f.J[0][n] = (JT)0;
+    f.J[1][n] = (JT)0;
+    f.J[2][n] = (JT)0;
 }
 
 /* MagneticFieldStageTwo (gpu_plasma.h:540-551) -> GPU_emh2 (kernels.cu:1024-1049) */
-__global__ __launch_bounds__(PIC_BLOCK) void k_h2(Grid g, FieldPtrs f) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_h2(Grid g, FV<T, JT> f) {
     long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (n >= g.G) return;
     int i, l, k;
     decode(g, n, i, l, k);
-    if (l <= g.Ny && k <= g.Nz) f.f[PIC_HX][n] += f.f[PIC_QX][n];
-    if (i <= g.Nx && k <= g.Nz) f.f[PIC_HY][n] += f.f[PIC_QY][n];
-    if (i <= g.Nx && l <= g.Ny) f.f[PIC_HZ][n] += f.f[PIC_QZ][n];
+    if (l <= g.Ny && k <= g.Nz) f.H[0][n] += f.Q[0][n];
+    if (i <= g.Nx && k <= g.Nz) f.H[1][n] += f.Q[1][n];
+    if (i <= g.Nx && l <= g.Ny) f.H[2][n] += f.Q[2][n];
 }
 
 /* ElectricFieldEvaluate (gpu_plasma.h:455-481) -> GPU_eme / emeElement
  * (kernels.cu:1091-1181); starts and shifts from GetElectricFieldStartsDirs
  * (gpu_plasma.h:286-304), finish (Nx,Ny,Nz). */
-__global__ __launch_bounds__(PIC_BLOCK) void k_e(Grid g, FieldPtrs f) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_e(Grid g, FV<T, JT> f) {
     long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (n >= g.G) return;
     int i, l, k;
     decode(g, n, i, l, k);
     if (i > g.Nx || l > g.Ny || k > g.Nz) return;
-    const double c1x = g.tau / g.hx, c1y = g.tau / g.hy, c1z = g.tau / g.hz, tau = g.tau;
-    const double *Hx = f.f[PIC_HX], *Hy = f.f[PIC_HY], *Hz = f.f[PIC_HZ];
+    const T c1x = (T)(g.tau / g.hx), c1y = (T)(g.tau / g.hy), c1z = (T)(g.tau / g.hz),
+            tau = (T)g.tau;
+    const T *Hx = f.H[0], *Hy = f.H[1], *Hz = f.H[2];
     if (l >= 1 && k >= 1) {   /* Ex: [0,Nx]x[1,Ny]x[1,Nz], H1=Hz(l-1), H2=Hy(k-1) */
-        double *E = f.f[PIC_EX];
-        E[n] += c1y * (Hz[n] - Hz[n - g.sy]) - c1z * (Hy[n] - Hy[n - 1]) - tau * f.f[PIC_JX][n];
+        T *E = f.E[0];
+        E[n] += c1y * (Hz[n] - Hz[n - g.sy]) - c1z * (Hy[n] - Hy[n - 1]) - tau * jval(f.J[0][n]);
     }
     if (i >= 1 && k >= 1) {   /* Ey: [1,Nx]x[0,Ny]x[1,Nz], H1=Hx(k-1), H2=Hz(i-1) */
-        double *E = f.f[PIC_EY];
-        E[n] += c1z * (Hx[n] - Hx[n - 1]) - c1x * (Hz[n] - Hz[n - g.sx]) - tau * f.f[PIC_JY][n];
+        T *E = f.E[1];
+        E[n] += c1z * (Hx[n] - Hx[n - 1]) - c1x * (Hz[n] - Hz[n - g.sx]) - tau * jval(f.J[1][n]);
     }
     if (i >= 1 && l >= 1) {   /* Ez: [1,Nx]x[1,Ny]x[0,Nz], H1=Hy(i-1), H2=Hx(l-1) */
-        double *E = f.f[PIC_EZ];
-        E[n] += c1x * (Hy[n] - Hy[n - g.sx]) - c1y * (Hx[n] - Hx[n - g.sy]) - tau * f.f[PIC_JZ][n];
+        T *E = f.E[2];
+        E[n] += c1x * (Hy[n] - Hy[n - g.sx]) - c1y * (Hx[n] - Hx[n - g.sy]) - tau * jval(f.J[2][n]);
     }
 }
 
 /* PeriodicBoundaries along x (gpu_plasma.h:802-870 with the arguments at
  * 486-505): Ey over l in [0,Ny], k in [1,Nz]; Ez over l in [1,Ny], k in
  * [0,Nz]; E[0]=E[Nx] and E[Nx+1]=E[1] (periodicElement, kernels.cu:1104). */
-__global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_x(Grid g, FieldPtrs f) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_x(Grid g, FV<T, JT> f) {
     long t = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (t >= (long)(g.Ny + 2) * g.sy) return;
     int l = (int)(t / g.sy), k = (int)(t % g.sy);
@@ -102,12 +142,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_x(Grid g, FieldPtrs f)
     long n0 = base, n1 = (long)g.sx + base, nN = (long)g.Nx * g.sx + base,
          nN1 = (long)(g.Nx + 1) * g.sx + base;
     if (l <= g.Ny && k >= 1 && k <= g.Nz) {
-        double *E = f.f[PIC_EY];
+        T *E = f.E[1];
         E[n0] = E[nN];
         E[nN1] = E[n1];
     }
     if (l >= 1 && l <= g.Ny && k <= g.Nz) {
-        double *E = f.f[PIC_EZ];
+        T *E = f.E[2];
         E[n0] = E[nN];
         E[nN1] = E[n1];
     }
@@ -115,19 +155,20 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_x(Grid g, FieldPtrs f)
 
 /* PeriodicBoundaries along z: Ex over i in [0,Nx], l in [0,Ny+1]; Ey over
  * i in [0,Nx+1], l in [0,Ny] (after the x copies). */
-__global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_z(Grid g, FieldPtrs f) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_z(Grid g, FV<T, JT> f) {
     long t = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (t >= (long)(g.Nx + 2) * (g.Ny + 2)) return;
     int i = (int)(t / (g.Ny + 2)), l = (int)(t % (g.Ny + 2));
     long base = (long)i * g.sx + (long)l * g.sy;
     long n0 = base, n1 = base + 1, nN = base + g.Nz, nN1 = base + g.Nz + 1;
     if (i <= g.Nx) {
-        double *E = f.f[PIC_EX];
+        T *E = f.E[0];
         E[n0] = E[nN];
         E[nN1] = E[n1];
     }
     if (l <= g.Ny) {
-        double *E = f.f[PIC_EY];
+        T *E = f.E[1];
         E[n0] = E[nN];
         E[nN1] = E[n1];
     }
@@ -137,7 +178,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_z(Grid g, FieldPtrs f)
  * periodicCurrentElement (kernels.cu:1124-1159), all three components for one
  * face direction `dir`; the three directions run as three launches in the
  * reference's order. */
-__global__ __launch_bounds__(PIC_BLOCK) void k_fold_j(Grid g, FieldPtrs f, int dir) {
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_fold_j(Grid g, FV<T, JT> f, int dir) {
     long t = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     int na, nb, N;
     if (dir == 0) { na = g.Ny + 2; nb = g.Nz + 2; N = g.Nx + 2; }
@@ -152,7 +194,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fold_j(Grid g, FieldPtrs f, int d
     long n1 = base + stride, nNm1 = base + (long)(N - 1) * stride;
     long nNm2 = base + (long)(N - 2) * stride, n0 = base;
     for (int dirE = 0; dirE < 3; dirE++) {
-        double *E = f.f[PIC_JX + dirE];
+        JT *E = f.J[dirE];
         if (dir != dirE) E[n1] += E[nNm1];
         if (dir != 1 || dirE != 1) E[nNm1] = E[n1];
         E[n0] += E[nNm2];
@@ -161,15 +203,17 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fold_j(Grid g, FieldPtrs f, int d
 }
 
 /* getElectricEnergy (gpu_plasma.h:621-657) -> GPU_getCellEnergy
- * (wrap_kernel.cu:14-36): sum over every ghosted node of Ex^2+Ey^2+Ez^2.
- * Deterministic two-level tree instead of one-address atomics. */
-__global__ __launch_bounds__(PIC_BLOCK) void k_energy(Grid g, FieldPtrs f, double *partials) {
+ * (wrap_kernel.cu:14-36): sum over every ghosted node of Ex^2+Ey^2+Ez^2,
+ * accumulated in FP64 by a deterministic two-level tree instead of
+ * one-address atomics. */
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_energy(Grid g, FV<T, JT> f, double *partials) {
     __shared__ double red[PIC_BLOCK];
-    const double *ex = f.f[PIC_EX], *ey = f.f[PIC_EY], *ez = f.f[PIC_EZ];
+    const T *ex = f.E[0], *ey = f.E[1], *ez = f.E[2];
     double s = 0.0;
     for (long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x; n < g.G;
          n += (long)gridDim.x * PIC_BLOCK) {
-        double a = ex[n], b = ey[n], c = ez[n];
+        double a = (double)ex[n], b = (double)ey[n], c = (double)ez[n];
         s += a * a + b * b + c * c;
     }
     red[threadIdx.x] = s;
@@ -205,35 +249,64 @@ __global__ void k_reset_status(DevStatus *st) {
 
 inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC_BLOCK); }
 
-}  // namespace
-
-void launch_h_half(const Grid &g, FieldPtrs f, hipStream_t s) {
-    hipLaunchKernelGGL(k_h_half, dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
+template <typename T, typename JT>
+void h_half(const Grid &g, FV<T, JT> f, hipStream_t s) {
+    hipLaunchKernelGGL((k_h_half<T, JT>), dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
 }
-void launch_h2(const Grid &g, FieldPtrs f, hipStream_t s) {
-    hipLaunchKernelGGL(k_h2, dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
+template <typename T, typename JT>
+void h2(const Grid &g, FV<T, JT> f, hipStream_t s) {
+    hipLaunchKernelGGL((k_h2<T, JT>), dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
 }
-void launch_e(const Grid &g, FieldPtrs f, hipStream_t s) {
-    hipLaunchKernelGGL(k_e, dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
+template <typename T, typename JT>
+void e_update(const Grid &g, FV<T, JT> f, hipStream_t s) {
+    hipLaunchKernelGGL((k_e<T, JT>), dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g, f);
 }
-void launch_e_periodic(const Grid &g, FieldPtrs f, hipStream_t s) {
-    hipLaunchKernelGGL(k_e_periodic_x, dim3(blocks_for((long)(g.Ny + 2) * g.sy)), dim3(PIC_BLOCK),
-                       0, s, g, f);
-    hipLaunchKernelGGL(k_e_periodic_z, dim3(blocks_for((long)(g.Nx + 2) * (g.Ny + 2))),
+template <typename T, typename JT>
+void e_periodic(const Grid &g, FV<T, JT> f, hipStream_t s) {
+    hipLaunchKernelGGL((k_e_periodic_x<T, JT>), dim3(blocks_for((long)(g.Ny + 2) * g.sy)),
+                       dim3(PIC_BLOCK), 0, s, g, f);
+    hipLaunchKernelGGL((k_e_periodic_z<T, JT>), dim3(blocks_for((long)(g.Nx + 2) * (g.Ny + 2))),
                        dim3(PIC_BLOCK), 0, s, g, f);
 }
-void launch_fold_j(const Grid &g, FieldPtrs f, hipStream_t s) {
+template <typename T, typename JT>
+void fold_j(const Grid &g, FV<T, JT> f, hipStream_t s) {
     long faces[3] = {(long)(g.Ny + 2) * (g.Nz + 2), (long)(g.Nx + 2) * (g.Nz + 2),
                      (long)(g.Nx + 2) * (g.Ny + 2)};
     for (int dir = 0; dir < 3; dir++)
-        hipLaunchKernelGGL(k_fold_j, dim3(blocks_for(faces[dir])), dim3(PIC_BLOCK), 0, s, g, f,
-                           dir);
+        hipLaunchKernelGGL((k_fold_j<T, JT>), dim3(blocks_for(faces[dir])), dim3(PIC_BLOCK), 0, s,
+                           g, f, dir);
 }
-void launch_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks, double *out,
-                   hipStream_t s) {
-    hipLaunchKernelGGL(k_energy, dim3(nblocks), dim3(PIC_BLOCK), 0, s, g, f, partials);
+template <typename T, typename JT>
+void energy(const Grid &g, FV<T, JT> f, double *partials, int nblocks, double *out,
+            hipStream_t s) {
+    hipLaunchKernelGGL((k_energy<T, JT>), dim3(nblocks), dim3(PIC_BLOCK), 0, s, g, f, partials);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(PIC_BLOCK), 0, s, partials, nblocks, out);
 }
+
+}  // namespace
+
+void launch_h_half(const Grid &g, FieldPtrs f, hipStream_t s) { h_half(g, view(f), s); }
+void launch_h2(const Grid &g, FieldPtrs f, hipStream_t s) { h2(g, view(f), s); }
+void launch_e(const Grid &g, FieldPtrs f, hipStream_t s) { e_update(g, view(f), s); }
+void launch_e_periodic(const Grid &g, FieldPtrs f, hipStream_t s) { e_periodic(g, view(f), s); }
+void launch_fold_j(const Grid &g, FieldPtrs f, hipStream_t s) { fold_j(g, view(f), s); }
+void launch_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks, double *out,
+                   hipStream_t s) {
+    energy(g, view(f), partials, nblocks, out, s);
+}
+
+void launch_h_half32(const Grid &g, FieldPtrs32 f, hipStream_t s) { h_half(g, view(f), s); }
+void launch_h2_32(const Grid &g, FieldPtrs32 f, hipStream_t s) { h2(g, view(f), s); }
+void launch_e32(const Grid &g, FieldPtrs32 f, hipStream_t s) { e_update(g, view(f), s); }
+void launch_e_periodic32(const Grid &g, FieldPtrs32 f, hipStream_t s) {
+    e_periodic(g, view(f), s);
+}
+void launch_fold_j32(const Grid &g, FieldPtrs32 f, hipStream_t s) { fold_j(g, view(f), s); }
+void launch_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks, double *out,
+                     hipStream_t s) {
+    energy(g, view(f), partials, nblocks, out, s);
+}
+
 void launch_reset_status(DevStatus *st, hipStream_t s) {
     hipLaunchKernelGGL(k_reset_status, dim3(1), dim3(1), 0, s, st);
 }

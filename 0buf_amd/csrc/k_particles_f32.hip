@@ -1,0 +1,564 @@
+/*
+ * k_particles_f32.hip -- particle stages of the FP32 precision form.
+ *
+ * Same tile structure as the FP64 plain form (one workgroup per 2x4x8 cells,
+ * E/H node box staged in LDS, flat particle order, in-place stayers, tile
+ * departure regions, holes filled from the bucket tail), with:
+ *   * positions held as FP32 offsets in [0,1) inside the owning cell, so the
+ *     resolution is h * 2^-24 everywhere in the box;
+ *   * FP32 gather and Boris push (the reference's Cayley form);
+ *   * the reference's charge-conserving deposit (Villasenor-Buneman segments,
+ *     relay on the dual-cell faces as in CurrentToMesh, cell.h:1306-1422)
+ *     written in cell units, each contribution rounded once to int64 fixed
+ *     point (2^-40) and summed with native 64-bit integer atomics (ds_add_u64
+ *     in LDS, then global): the current is exact given the per-particle
+ *     contributions and independent of summation order;
+ *   * migration by the cell-local offset: a particle whose offset leaves
+ *     [0,1) moves to the neighbouring cell (periodic), exactly one receiver.
+ */
+#include "pic_internal.h"
+
+namespace {
+
+constexpr int TXC = 2, TYC = 4, TZC = 8;
+
+struct PushArgs32 {
+    Grid g;
+    TileGeom t;
+    FieldPtrs32 f;
+    PartPtrs32 p;
+    QueuePtrs32 q;
+    float qm[PIC_MAX_SLOTS];       /* charge/mass per species slot */
+    float cj[PIC_MAX_SLOTS][3];    /* m h_d / tau * 2^40: fixed-point current per
+                                      unit cell displacement along d */
+    float tdh[3];                  /* tau / h_d */
+    float hh[3];                   /* h_d^2, weights of the two-face relay projection */
+    float tau;
+    int dep_cap;
+    DevStatus *st;
+};
+
+/* round-to-nearest int64 of v (|v| < 2^51) by the 1.5*2^52 shifter */
+__device__ __forceinline__ long long fix64(float v) {
+    double d = (double)v + 6755399441055744.0;
+    return (long long)(__double_as_longlong(d) - 0x4338000000000000LL);
+}
+
+__device__ __forceinline__ void add64(long long *p, long long v) {
+    atomicAdd((unsigned long long *)p, (unsigned long long)v);
+}
+
+template <int BY, int BZ>
+__device__ __forceinline__ float tri(const float *F, int u, int v, int w, float x0, float x1,
+                                     float y0, float y1, float z0, float z1) {
+    const float *p = F + (u * BY + v) * BZ + w;
+    float a = z0 * p[0] + z1 * p[1];
+    float b = z0 * p[BZ] + z1 * p[BZ + 1];
+    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
+    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
+    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
+}
+
+/* One straight segment a -> b inside dual cell rd (relative to the owner
+ * cell; the dual cell spans centres rd-1/2 .. rd+1/2): the 12 VB node
+ * contributions of pqr (cell.h:1511-1693) in cell units,
+ *   Jx = (m hx/tau) du (Wy Wz +- dv dw/12)  on (x dual, y/z centres), etc. */
+template <int BY, int BZ, int NB>
+__device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, int rdx, int rdy,
+                                       int rdz, float ax, float ay, float az, float bx, float by,
+                                       float bz, const float *cj) {
+    const float dx = bx - ax, dy = by - ay, dz = bz - az;
+    const float mx = 0.5f * (ax + bx) - ((float)rdx - 0.5f);
+    const float my = 0.5f * (ay + by) - ((float)rdy - 0.5f);
+    const float mz = 0.5f * (az + bz) - ((float)rdz - 0.5f);
+    const float lx = 1.0f - mx, ly = 1.0f - my, lz = 1.0f - mz;
+    const float syz = dy * dz * (1.0f / 12.0f), sxz = dx * dz * (1.0f / 12.0f),
+                sxy = dx * dy * (1.0f / 12.0f);
+    const float qx = cj[0] * dx, qy = cj[1] * dy, qz = cj[2] * dz;
+    const int n = ((ca + rdx) * BY + (cb + rdy)) * BZ + (cc + rdz);
+    long long *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
+    add64(&Jx[n], fix64(qx * (ly * lz + syz)));
+    add64(&Jx[n + 1], fix64(qx * (ly * mz - syz)));
+    add64(&Jx[n + BZ], fix64(qx * (my * lz - syz)));
+    add64(&Jx[n + BZ + 1], fix64(qx * (my * mz + syz)));
+    add64(&Jy[n], fix64(qy * (lx * lz + sxz)));
+    add64(&Jy[n + 1], fix64(qy * (lx * mz - sxz)));
+    add64(&Jy[n + BY * BZ], fix64(qy * (mx * lz - sxz)));
+    add64(&Jy[n + BY * BZ + 1], fix64(qy * (mx * mz + sxz)));
+    add64(&Jz[n], fix64(qz * (lx * ly + sxy)));
+    add64(&Jz[n + BZ], fix64(qz * (lx * my - sxy)));
+    add64(&Jz[n + BY * BZ], fix64(qz * (mx * ly - sxy)));
+    add64(&Jz[n + BY * BZ + BZ], fix64(qz * (mx * my + sxy)));
+}
+
+__device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from, int to,
+                                        bool want) {
+    int pos = from;
+    while (pos < to) {
+        int gbit = base + pos;
+        unsigned w = mask[gbit >> 5];
+        if (!want) w = ~w;
+        w >>= (gbit & 31);
+        if (w) {
+            int p = pos + __builtin_ctz(w);
+            return p < to ? p : to;
+        }
+        pos += 32 - (gbit & 31);
+    }
+    return to;
+}
+
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
+    constexpr int C = TX * TY * TZ;
+    constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
+    __shared__ float sF[6 * NB];
+    __shared__ long long sJ[3 * NB];
+    __shared__ int sOff[C + 1];
+    __shared__ int sCnt[C];
+    __shared__ int sLeave[C];
+    __shared__ int sQ;
+    __shared__ unsigned sFlags;
+    extern __shared__ unsigned sMask[];
+
+    const Grid &g = a.g;
+    const int tid = threadIdx.x;
+    long tile;
+    {
+        long nt = a.t.ntiles, b = blockIdx.x;
+        long q = nt / 8, r = nt % 8, x = b % 8;
+        tile = x * q + (x < r ? x : r) + b / 8;
+    }
+    const int tz = (int)(tile % a.t.ntz);
+    const int ty = (int)((tile / a.t.ntz) % a.t.nty);
+    const int tx = (int)(tile / ((long)a.t.ntz * a.t.nty));
+    const int I0 = tx * TX, L0 = ty * TY, K0 = tz * TZ;
+    const int cap = a.p.cap;
+
+    for (int c = tid; c < C; c += PIC_BLOCK) {
+        int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        int cnt = 0;
+        if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+            cnt = a.p.count[((long)gi * g.Ny + gl) * g.Nz + gk];
+            cnt = cnt < cap ? cnt : cap;
+        }
+        sCnt[c] = cnt;
+        sLeave[c] = 0;
+    }
+    const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
+    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
+        int fi = e / NB, nn = e % NB;
+        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
+        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        float val = 0.0f;
+        if (gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1)
+            val = Fsrc[fi][(long)gu * g.sx + (long)gv * g.sy + gw];
+        sF[e] = val;
+    }
+    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
+    if (tid == 0) {
+        sQ = 0;
+        sFlags = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int s = 0;
+        for (int c = 0; c < C; c++) {
+            sOff[c] = s;
+            s += sCnt[c];
+        }
+        sOff[C] = s;
+    }
+    __syncthreads();
+    const int T = sOff[C];
+    for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
+    __syncthreads();
+
+    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+        int lo = 0, hi = C;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
+        }
+        const int c = lo;
+        const int j = fidx - sOff[c];
+        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+
+        const float ox = a.p.p[0][pidx], oy = a.p.p[1][pidx], oz = a.p.p[2][pidx];
+        float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
+        const unsigned char s = a.p.sp[pidx];
+
+        /* gather: node-type axes interpolate between faces i, i+1 with
+         * weights (1-o, o); centre-type axes between the two nearest centres
+         * (rd = 1 when o >= 1/2; rd is also the start's dual cell) */
+        const int rdx = ox >= 0.5f, rdy = oy >= 0.5f, rdz = oz >= 0.5f;
+        const float cx0 = rdx ? 1.5f - ox : 0.5f - ox, cx1 = 1.0f - cx0;
+        const float cy0 = rdy ? 1.5f - oy : 0.5f - oy, cy1 = 1.0f - cy0;
+        const float cz0 = rdz ? 1.5f - oz : 0.5f - oz, cz1 = 1.0f - cz0;
+        const float nx0 = 1.0f - ox, ny0 = 1.0f - oy, nz0 = 1.0f - oz;
+        const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
+        const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
+        const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
+        const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
+        const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
+        const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
+
+        /* Boris push, Cayley form (particle.h:160-202) */
+        const float tau1 = a.qm[s] * a.tau * 0.5f;
+        pu += tau1 * Ex;
+        pv += tau1 * Ey;
+        pw += tau1 * Ez;
+        const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+        const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
+        const float su = pu + pv * bz - pw * by;
+        const float sv = pv + pw * bx - pu * bz;
+        const float sw = pw + pu * by - pv * bx;
+        const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
+        const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
+        const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
+        pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
+        pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
+        pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
+        const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+        const float dux = a.tdh[0] * gi2 * pu, duy = a.tdh[1] * gi2 * pv,
+                    duz = a.tdh[2] * gi2 * pw;
+        const float o1x = ox + dux, o1y = oy + duy, o1z = oz + duz;
+
+        /* deposit: dual cell of each end (floor(o + 1/2)), crossing code m
+         * and relay on the crossed dual faces (o = 1/2), as CurrentToMesh */
+        {
+            const int rx1 = o1x >= 0.5f, ry1 = o1y >= 0.5f, rz1 = o1z >= 0.5f;
+            const int m = 4 * (rdx != rx1) + 2 * (rdy != ry1) + (rdz != rz1);
+            float num = 0.0f, den = 1.0f;
+            switch (m) {
+            case 1: num = 0.5f - oz; den = duz; break;
+            case 2: num = 0.5f - oy; den = duy; break;
+            case 4: num = 0.5f - ox; den = dux; break;
+            case 3:
+                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - oy) * duy * a.hh[1];
+                den = duz * duz * a.hh[2] + duy * duy * a.hh[1];
+                break;
+            case 5:
+                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - ox) * dux * a.hh[0];
+                den = duz * duz * a.hh[2] + dux * dux * a.hh[0];
+                break;
+            case 6:
+                num = (0.5f - oy) * duy * a.hh[1] + (0.5f - ox) * dux * a.hh[0];
+                den = duy * duy * a.hh[1] + dux * dux * a.hh[0];
+                break;
+            default: break;
+            }
+            const float sr = num / den;
+            const float x2x = (m & 4) ? 0.5f : ox + dux * sr;
+            const float x2y = (m & 2) ? 0.5f : oy + duy * sr;
+            const float x2z = (m & 1) ? 0.5f : oz + duz * sr;
+            const float *cj = a.cj[s];
+            const float e1x = m ? x2x : o1x, e1y = m ? x2y : o1y, e1z = m ? x2z : o1z;
+            vb_seg<BY, BZ, NB>(sJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
+            if (m)
+                vb_seg<BY, BZ, NB>(sJ, ca, cb, cc, rx1, ry1, rz1, x2x, x2y, x2z, o1x, o1y, o1z,
+                                   cj);
+        }
+
+        /* migration by the cell-local offset */
+        int shx = (int)floorf(o1x), shy = (int)floorf(o1y), shz = (int)floorf(o1z);
+        float nxo = o1x - (float)shx, nyo = o1y - (float)shy, nzo = o1z - (float)shz;
+        if (nxo >= 1.0f) { nxo -= 1.0f; shx += 1; }
+        if (nyo >= 1.0f) { nyo -= 1.0f; shy += 1; }
+        if (nzo >= 1.0f) { nzo -= 1.0f; shz += 1; }
+        if ((shx | shy | shz) == 0) {
+            a.p.p[0][pidx] = nxo;
+            a.p.p[1][pidx] = nyo;
+            a.p.p[2][pidx] = nzo;
+            a.p.p[3][pidx] = pu;
+            a.p.p[4][pidx] = pv;
+            a.p.p[5][pidx] = pw;
+        } else {
+            atomicAdd(&sLeave[c], 1);
+            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
+            int rx = gi + shx, ry = gl + shy, rz = gk + shz;
+            if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
+            if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
+            if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
+            const int slot = atomicAdd(&sQ, 1);
+            if (slot < a.q.qcap) {
+                const long e = tile * (long)a.q.qcap + slot;
+                a.q.q[0][e] = nxo;
+                a.q.q[1][e] = nyo;
+                a.q.q[2][e] = nzo;
+                a.q.q[3][e] = pu;
+                a.q.q[4][e] = pv;
+                a.q.q[5][e] = pw;
+                a.q.sp[e] = s;
+                a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
+            }
+        }
+    }
+    __syncthreads();
+
+    int maxout = 0;
+    for (int c = tid; c < C; c += PIC_BLOCK) {
+        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
+        const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+        const int n = sCnt[c], m = sLeave[c], np = n - m, base = sOff[c];
+        if (m > 0) {
+            int h = next_bit(sMask, base, 0, np, true);
+            int sidx = next_bit(sMask, base, np, n, false);
+            while (h < np && sidx < n) {
+                const long from = cell * cap + sidx, to = cell * cap + h;
+                for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
+                a.p.sp[to] = a.p.sp[from];
+                h = next_bit(sMask, base, h + 1, np, true);
+                sidx = next_bit(sMask, base, sidx + 1, n, false);
+            }
+        }
+        a.p.count[cell] = np;
+        if (m > maxout) maxout = m;
+    }
+
+    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
+        const long long val = sJ[e];
+        if (val == 0) continue;
+        int comp = e / NB, nn = e % NB;
+        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
+        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
+        add64(&a.f.J[comp][(long)gu * g.sx + (long)gv * g.sy + gw], val);
+    }
+
+    if (maxout > 0) {
+        if (a.dep_cap > 0 && maxout > a.dep_cap) atomicOr(&sFlags, (unsigned)PIC_F_TOO_MANY);
+        if (maxout > a.st->max_dep) atomicMax(&a.st->max_dep, maxout);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int nq = sQ;
+        if (nq > a.q.qcap) {
+            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_OVERFLOW);
+            nq = a.q.qcap;
+        }
+        a.q.count[tile] = nq;
+        if (sFlags) atomicOr(&a.st->flags, sFlags);
+    }
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(PartPtrs32 p, QueuePtrs32 q,
+                                                       DevStatus *st) {
+    const long tile = blockIdx.x;
+    const int n = q.count[tile];
+    const long qb = tile * (long)q.qcap;
+    bool over = false;
+    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
+        const long e = qb + j;
+        const int d = q.dest[e];
+        const int slot = atomicAdd(&p.count[d], 1);
+        q.slot[e] = slot;
+        if (slot < p.cap) {
+            const long to = (long)d * p.cap + slot;
+            for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
+            p.sp[to] = q.sp[e];
+        } else {
+            over = true;
+        }
+    }
+    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load32(const long *off, long ncell,
+                                                             const float *s0, const float *s1,
+                                                             const float *s2, const float *s3,
+                                                             const float *s4, const float *s5,
+                                                             const unsigned char *ssp,
+                                                             PartPtrs32 p) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    const long n = off[c + 1] - off[c];
+    if (j == 0) p.count[c] = (int)n;
+    if (j >= n) return;
+    const long src = off[c] + j;
+    p.p[0][e] = s0[src];
+    p.p[1][e] = s1[src];
+    p.p[2][e] = s2[src];
+    p.p[3][e] = s3[src];
+    p.p[4][e] = s4[src];
+    p.p[5][e] = s5[src];
+    p.sp[e] = ssp[src];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_repack32(long ncell, PartPtrs32 from,
+                                                       PartPtrs32 to) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * from.cap) return;
+    const long c = e / from.cap;
+    const int j = (int)(e % from.cap);
+    int n = from.count[c];
+    if (n > from.cap) n = from.cap;
+    if (j >= n) return;
+    const long dst = c * to.cap + j;
+    for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
+    to.sp[dst] = from.sp[e];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(QueuePtrs32 q, int oldcap,
+                                                             PartPtrs32 to) {
+    const long tile = blockIdx.x;
+    const int n = q.count[tile];
+    const long qb = tile * (long)q.qcap;
+    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
+        const long e = qb + j;
+        const int slot = q.slot[e];
+        if (slot < oldcap || slot >= to.cap) continue;
+        const long dst = (long)q.dest[e] * to.cap + slot;
+        for (int k = 0; k < 6; k++) to.p[k][dst] = q.q[k][e];
+        to.sp[dst] = q.sp[e];
+    }
+}
+
+/* global coordinates x = (i + o) h in FP64 */
+__global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles32(Grid g, long ncell,
+                                                                 const long *off, PartPtrs32 p,
+                                                                 double *d0, double *d1,
+                                                                 double *d2, double *d3,
+                                                                 double *d4, double *d5,
+                                                                 unsigned char *dsp, int *dcell) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    if (j >= off[c + 1] - off[c]) return;
+    const long dst = off[c] + j;
+    const int i = (int)(c / ((long)g.Ny * g.Nz)), l = (int)((c / g.Nz) % g.Ny),
+              k = (int)(c % g.Nz);
+    d0[dst] = ((double)i + (double)p.p[0][e]) * g.hx;
+    d1[dst] = ((double)l + (double)p.p[1][e]) * g.hy;
+    d2[dst] = ((double)k + (double)p.p[2][e]) * g.hz;
+    d3[dst] = p.p[3][e];
+    d4[dst] = p.p[4][e];
+    d5[dst] = p.p[5][e];
+    dsp[dst] = p.sp[e];
+    dcell[dst] = (int)c;
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_moments32(Grid g, long ncell, PartPtrs32 p,
+                                                        SpeciesTable sp, const int *slot_sort,
+                                                        double *partials) {
+    __shared__ double red[PIC_BLOCK];
+    double acc[27];
+    for (int k = 0; k < 27; k++) acc[k] = 0.0;
+    for (long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x; e < ncell * p.cap;
+         e += (long)gridDim.x * PIC_BLOCK) {
+        const long c = e / p.cap;
+        const int j = (int)(e % p.cap);
+        if (j >= p.count[c]) continue;
+        const int s = p.sp[e];
+        int srt = slot_sort[s];
+        srt = srt < 0 ? 0 : (srt > 2 ? 2 : srt);
+        const int i = (int)(c / ((long)g.Ny * g.Nz)), l = (int)((c / g.Nz) % g.Ny),
+                  k = (int)(c % g.Nz);
+        const double px = p.p[3][e], py = p.p[4][e], pz = p.p[5][e];
+        const double gm = sqrt(1.0 + px * px + py * py + pz * pz);
+        double v[9] = {1.0,
+                       ((double)i + (double)p.p[0][e]) * g.hx,
+                       ((double)l + (double)p.p[1][e]) * g.hy,
+                       ((double)k + (double)p.p[2][e]) * g.hz,
+                       px, py, pz,
+                       (sp.m[s] / sp.q_m[s]) * (gm - 1.0),
+                       fabs(px) + fabs(py) + fabs(pz)};
+        for (int q = 0; q < 9; q++) {
+            acc[0 * 9 + q] += (srt == 0) ? v[q] : 0.0;
+            acc[1 * 9 + q] += (srt == 1) ? v[q] : 0.0;
+            acc[2 * 9 + q] += (srt == 2) ? v[q] : 0.0;
+        }
+    }
+    for (int q = 0; q < 27; q++) {
+        red[threadIdx.x] = acc[q];
+        __syncthreads();
+        for (int w = PIC_BLOCK / 2; w > 0; w >>= 1) {
+            if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) partials[(long)blockIdx.x * 27 + q] = red[0];
+        __syncthreads();
+    }
+}
+
+__global__ void k_moments_final32(const double *partials, int nblocks, double *out27) {
+    const int k = threadIdx.x;
+    if (k >= 27) return;
+    double s = 0.0;
+    for (int b = 0; b < nblocks; b++) s += partials[(long)b * 27 + k];
+    out27[k] = s;
+}
+
+inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC_BLOCK); }
+
+}  // namespace
+
+void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
+                             QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
+                             hipStream_t s) {
+    PushArgs32 a;
+    a.g = g;
+    a.t = t;
+    a.f = f;
+    a.p = p;
+    a.q = q;
+    const double h[3] = {g.hx, g.hy, g.hz};
+    for (int k = 0; k < PIC_MAX_SLOTS; k++) {
+        a.qm[k] = (float)sp.q_m[k];
+        for (int d = 0; d < 3; d++) a.cj[k][d] = (float)(sp.m[k] * h[d] / g.tau * PIC_J_SCALE);
+    }
+    for (int d = 0; d < 3; d++) {
+        a.tdh[d] = (float)(g.tau / h[d]);
+        a.hh[d] = (float)(h[d] * h[d]);
+    }
+    a.tau = (float)g.tau;
+    a.dep_cap = dep_cap;
+    a.st = st;
+    hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                       dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+}
+
+void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(k_arrive32, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, p, q, st);
+}
+
+void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
+                           const unsigned char *srcsp, PartPtrs32 p, hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_load32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       offsets, ncell, src[0], src[1], src[2], src[3], src[4], src[5], srcsp, p);
+}
+
+void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s) {
+    hipLaunchKernelGGL(k_repack32, dim3(blocks_for(ncell * from.cap)), dim3(PIC_BLOCK), 0, s,
+                       ncell, from, to);
+}
+
+void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtrs32 to,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_fix_arrivals32, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, q,
+                       oldcap, to);
+}
+
+void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
+                               double *dst[6], unsigned char *dsp, int *dcell, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_particles32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0,
+                       s, g, ncell, offsets, p, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5],
+                       dsp, dcell);
+}
+
+void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,
+                      const int *slot_sort, double *partials, int nblocks, double *out27,
+                      hipStream_t s) {
+    hipLaunchKernelGGL(k_moments32, dim3(nblocks), dim3(PIC_BLOCK), 0, s, g, ncell, p, sp,
+                       slot_sort, partials);
+    hipLaunchKernelGGL(k_moments_final32, dim3(1), dim3(32), 0, s, partials, nblocks, out27);
+}

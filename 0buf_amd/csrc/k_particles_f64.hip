@@ -24,8 +24,20 @@ namespace {
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
 
+/* Divisors that are constant over a run, with their correctly rounded
+ * reciprocals: q = divc(x, d, r) returns the correctly rounded x/d in three
+ * instructions (Markstein: q0 = x r, exact remainder x - q0 d by FMA, one
+ * FMA correction), bit-identical to IEEE division. */
+struct DivConst {
+    double rhx, rhy, rhz;     /* 1/hx, 1/hy, 1/hz */
+    double r12;               /* 1/12 */
+    double Dx, Dy, Dz;        /* (tau hy) hz, (tau hx) hz, (tau hx) hy (cell.h:1578-1580) */
+    double rDx, rDy, rDz;
+};
+
 struct PushArgs {
     Grid g;
+    DivConst dc;
     TileGeom t;
     FieldPtrs f;
     PartPtrs p;
@@ -47,28 +59,17 @@ __device__ __forceinline__ double pow_m05(double v) {
     return fma(0.5 * r, e, r);
 }
 
-/* cell.h:527-566 */
-__device__ __forceinline__ int cell_number(double x, double x0, double h) {
-    return ((int)((x - x0) / h + 1.0) + 1);
+__device__ __forceinline__ double divc(double x, double d, double r) {
+    double q0 = x * r;
+    double e = fma(-q0, d, x);
+    return fma(e, r, q0);
 }
-__device__ __forceinline__ int cell_number_center(double x, double x0, double h) {
-    double t = (((x - x0) / h + 1.0) + 1.5);
-    return (int)(t);
-}
-__device__ __forceinline__ int cell_number_center_current(double x, double x0, double h) {
-    double t = ((x - x0) / h + 1.5);
-    return (int)(t + 1);
-}
-/* cell.h:1193-1236 */
-__device__ __forceinline__ double s_node(double x, double h) {
-    return (int)(x / h + 1.0) - x / h;
-}
-__device__ __forceinline__ double s_center(double x, double h) {
-    return (int)(x / h + 1.5) - 0.5 - x / h;
-}
-/* cell.h:1427-1478 */
+
+/* cell.h:1427-1478 Reflect: (x>xm)(x-xm) + (x<0)(xm+x) + (0<x<xm) x.  The
+ * select form returns the same bits in every case, including exact 0 and xm
+ * (both map to +0) and the signed zeros of the products. */
 __device__ __forceinline__ double reflect1(double x, double xm) {
-    return (x > xm) * (x - xm) + (x < 0.0) * (xm + x) + (x > 0 && x < xm) * x;
+    return x > xm ? x - xm : (x < 0.0 ? xm + x : ((x > 0 && x < xm) ? x : 0.0));
 }
 
 template <int BY, int BZ>
@@ -93,9 +94,11 @@ struct Geo {
 /* cell.h:1511-1693 pqr: 12 node contributions of one straight segment,
  * accumulated into the LDS J box at local node (owner + tile index - 2). */
 template <int BY, int BZ, int NB>
-__device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, int ix, int iy, int iz,
-                                        double xx, double xy, double xz, double x1x, double x1y,
-                                        double x1z, double a, unsigned &bad) {
+__device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, const DivConst &dc, int ix,
+                                        int iy, int iz, double xx, double xy, double xz,
+                                        double x1x, double x1y, double x1z, double a,
+                                        unsigned &bad, bool active) {
+    if (!active) return;
     if (ix < 2 || ix > 3 || iy < 2 || iy > 3 || iz < 2 || iz > 3) {
         bad = 1;
         return;
@@ -105,10 +108,11 @@ __device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, int ix, int iy
     double dz = (0.50 * (xz + x1z) - c.hz * ((double)iz - 2.50) - c.z0);
     double dx1 = c.hx - dx, dy1 = c.hy - dy, dz1 = c.hz - dz;
     double su = x1x - xx, sv = x1y - xy, sw = x1z - xz;
-    double s1 = sv * sw / 12.0, s2 = su * sw / 12.0, s3 = su * sv / 12.0;
-    su = su * a / (c.tau * c.hy * c.hz);
-    sv = sv * a / (c.tau * c.hx * c.hz);
-    sw = sw * a / (c.tau * c.hx * c.hy);
+    double s1 = divc(sv * sw, 12.0, dc.r12), s2 = divc(su * sw, 12.0, dc.r12),
+           s3 = divc(su * sv, 12.0, dc.r12);
+    su = divc(su * a, dc.Dx, dc.rDx);
+    sv = divc(sv * a, dc.Dy, dc.rDy);
+    sw = divc(sw * a, dc.Dz, dc.rDz);
     const int u = c.ca + ix - 2, v = c.cb + iy - 2, w = c.cc + iz - 2;
     const int n = (u * BY + v) * BZ + w;
     double *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
@@ -249,15 +253,22 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         const unsigned char s = a.p.sp[pidx];
         const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
 
-        /* GetField (cell.h:1257-1290) */
+        /* GetField (cell.h:1257-1290): tile indices from (x - x0)/h
+         * (getCellNumber, getCellNumberCenter: cell.h:527-549), weights from
+         * the global x/h (s1..s6_interpolate: cell.h:1193-1236) */
+        const double frx = divc(x - G.x0, hx, a.dc.rhx);
+        const double fry = divc(y - G.y0, hy, a.dc.rhy);
+        const double frz = divc(z - G.z0, hz, a.dc.rhz);
         double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
         if (!(x < 0 || y < 0 || z < 0)) {
-            int ix = cell_number(x, G.x0, hx), i1x = cell_number_center(x, G.x0, hx);
-            double s1 = s_node(x, hx), s2 = s_center(x, hx);
-            int iy = cell_number(y, G.y0, hy), i1y = cell_number_center(y, G.y0, hy);
-            double s3 = s_node(y, hy), s4 = s_center(y, hy);
-            int iz = cell_number(z, G.z0, hz), i1z = cell_number_center(z, G.z0, hz);
-            double s5 = s_node(z, hz), s6 = s_center(z, hz);
+            const double xh = divc(x, hx, a.dc.rhx), yh = divc(y, hy, a.dc.rhy),
+                         zh = divc(z, hz, a.dc.rhz);
+            int ix = (int)(frx + 1.0) + 1, i1x = (int)((frx + 1.0) + 1.5);
+            double s1 = (int)(xh + 1.0) - xh, s2 = (int)(xh + 1.5) - 0.5 - xh;
+            int iy = (int)(fry + 1.0) + 1, i1y = (int)((fry + 1.0) + 1.5);
+            double s3 = (int)(yh + 1.0) - yh, s4 = (int)(yh + 1.5) - 0.5 - yh;
+            int iz = (int)(frz + 1.0) + 1, i1z = (int)((frz + 1.0) + 1.5);
+            double s5 = (int)(zh + 1.0) - zh, s6 = (int)(zh + 1.5) - 0.5 - zh;
             double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4, s51 = 1. - s5,
                    s61 = 1. - s6;
             if (ix < 2 || ix > 3 || i1x < 2 || i1x > 3 || iy < 2 || iy > 3 || i1y < 2 ||
@@ -287,9 +298,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         double q1 = bx * bx, q2 = by * by, q3 = bz * bz;
         double q4 = bx * by, q5 = by * bz, q6 = bz * bx;
         double sden = q1 + 1. + q2 + q3;
-        double pu1 = ((q1 + 1.) * su + (q4 + bz) * sv + (q6 - by) * sw) / sden;
-        double pv1 = ((q4 - bz) * su + (q2 + 1.) * sv + (q5 + bx) * sw) / sden;
-        double pw1 = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.) * sw) / sden;
+        const double rden = 1.0 / sden;   /* one reciprocal for the three quotients */
+        double pu1 = divc((q1 + 1.) * su + (q4 + bz) * sv + (q6 - by) * sw, sden, rden);
+        double pv1 = divc((q4 - bz) * su + (q2 + 1.) * sv + (q5 + bx) * sw, sden, rden);
+        double pw1 = divc((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.) * sw, sden, rden);
         pu = pu1 + Ex * tau1;
         pv = pv1 + Ey * tau1;
         pw = pw1 + Ez * tau1;
@@ -298,72 +310,58 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         const double y1 = y + tau * (ps * pv);
         const double z1 = z + tau * (ps * pw);
 
-        /* CurrentToMesh (cell.h:1306-1422) */
+        /* CurrentToMesh (cell.h:1306-1422).  Tile indices of both ends
+         * (getCellNumberCenterCurrent, cell.h:560-566), crossing code
+         * m = 4|di| + 2|dl| + |dk|; each case's relay point is formed with
+         * cheap selects and the one division of the case is done once. */
         {
-            int i1x = cell_number_center_current(x, G.x0, hx);
-            int i1y = cell_number_center_current(y, G.y0, hy);
-            int i1z = cell_number_center_current(z, G.z0, hz);
-            int i2x = cell_number_center_current(x1, G.x0, hx);
-            int i2y = cell_number_center_current(y1, G.y0, hy);
-            int i2z = cell_number_center_current(z1, G.z0, hz);
-            int di = abs(i2x - i1x), dl = abs(i2y - i1y), dk = abs(i2z - i1z);
-            int m = 4 * di + 2 * dl + dk;
-            double x2x = 0, x2y = 0, x2z = 0, sr;
+            const double t1x = divc(x1 - G.x0, hx, a.dc.rhx);
+            const double t1y = divc(y1 - G.y0, hy, a.dc.rhy);
+            const double t1z = divc(z1 - G.z0, hz, a.dc.rhz);
+            const int i1x = (int)((frx + 1.5) + 1), i1y = (int)((fry + 1.5) + 1),
+                      i1z = (int)((frz + 1.5) + 1);
+            const int i2x = (int)((t1x + 1.5) + 1), i2y = (int)((t1y + 1.5) + 1),
+                      i2z = (int)((t1z + 1.5) + 1);
+            int m = 4 * abs(i2x - i1x) + 2 * abs(i2y - i1y) + abs(i2z - i1z);
+            if (m > 7) m = 0;
+            const double dxs = x1 - x, dys = y1 - y, dzs = z1 - z;
+            /* relay coordinates on the crossed dual-cell faces
+             * (getCellTransitAverage, cell.h:600-602) */
+            const double fx = transit_average(hx, i1x, i2x, G.x0);
+            const double fy = transit_average(hy, i1y, i2y, G.y0);
+            const double fz = transit_average(hz, i1z, i2z, G.z0);
+            double num = 0.0, den = 1.0;
             switch (m) {
-            case 1:
-                x2z = transit_average(hz, i1z, i2z, G.z0);
-                sr = (x2z - z) / (z1 - z);
-                x2x = (x + (x1 - x) * sr);
-                x2y = (y + (y1 - y) * sr);
-                break;
-            case 2:
-                x2y = transit_average(hy, i1y, i2y, G.y0);
-                sr = (x2y - y) / (y1 - y);
-                x2x = (x + (x1 - x) * sr);
-                x2z = (z + (z1 - z) * sr);
-                break;
+            case 1: num = fz - z; den = dzs; break;
+            case 2: num = fy - y; den = dys; break;
+            case 4: num = fx - x; den = dxs; break;
             case 3:
-                x2y = transit_average(hy, i1y, i2y, G.y0);
-                x2z = transit_average(hz, i1z, i2z, G.z0);
-                sr = ((x2z - z) * (z1 - z) + (x2y - y) * (y1 - y)) /
-                     ((z1 - z) * (z1 - z) + (y1 - y) * (y1 - y));
-                x2x = (x + (x1 - x) * sr);
-                break;
-            case 4:
-                x2x = transit_average(hx, i1x, i2x, G.x0);
-                sr = (x2x - x) / (x1 - x);
-                x2y = (y + (y1 - y) * sr);
-                x2z = (z + (z1 - z) * sr);
+                num = (fz - z) * dzs + (fy - y) * dys;
+                den = (z1 - z) * (z1 - z) + (y1 - y) * (y1 - y);
                 break;
             case 5:
-                x2x = transit_average(hx, i1x, i2x, G.x0);
-                x2z = transit_average(hz, i1z, i2z, G.z0);
-                sr = ((x2z - z) * (z1 - z) + (x2x - x) * (x1 - x)) /
-                     ((z1 - z) * (z1 - z) + (x1 - x) * (x1 - x));
-                x2y = (y + (y1 - y) * sr);
+                num = (fz - z) * dzs + (fx - x) * dxs;
+                den = (z1 - z) * (z1 - z) + (x1 - x) * (x1 - x);
                 break;
             case 6:
-                x2x = transit_average(hx, i1x, i2x, G.x0);
-                x2y = transit_average(hy, i1y, i2y, G.y0);
-                sr = ((x2y - y) * (y1 - y) + (x2x - x) * (x1 - x)) /
-                     ((y1 - y) * (y1 - y) + (x1 - x) * (x1 - x));
-                x2z = (z + (z1 - z) * sr);
+                num = (fy - y) * dys + (fx - x) * dxs;
+                den = (y1 - y) * (y1 - y) + (x1 - x) * (x1 - x);
                 break;
-            case 7:
-                x2x = transit_average(hx, i1x, i2x, G.x0);
-                x2y = transit_average(hy, i1y, i2y, G.y0);
-                x2z = transit_average(hz, i1z, i2z, G.z0);
-                break;
-            default:
-                m = 0;
-                break;
+            default: break;
             }
-            if (m == 0) {
-                pqr_lds<BY, BZ, NB>(sJ, G, i1x, i1y, i1z, x, y, z, x1, y1, z1, mass, bad);
-            } else {
-                pqr_lds<BY, BZ, NB>(sJ, G, i1x, i1y, i1z, x, y, z, x2x, x2y, x2z, mass, bad);
-                pqr_lds<BY, BZ, NB>(sJ, G, i2x, i2y, i2z, x2x, x2y, x2z, x1, y1, z1, mass, bad);
-            }
+            const double sr = num / den;
+            /* crossed axes take the face value, the others the point of the
+             * straight path at parameter sr (getRatioBasedX, cell.h:617) */
+            const double x2x = (m & 4) ? fx : (x + dxs * sr);
+            const double x2y = (m & 2) ? fy : (y + dys * sr);
+            const double x2z = (m & 1) ? fz : (z + dzs * sr);
+            /* one code path for every lane: without a crossing the first
+             * segment is the whole path (x2 = x1) and the second is skipped */
+            const double e1x = m ? x2x : x1, e1y = m ? x2y : y1, e1z = m ? x2z : z1;
+            pqr_lds<BY, BZ, NB>(sJ, G, a.dc, i1x, i1y, i1z, x, y, z, e1x, e1y, e1z, mass, bad,
+                                true);
+            pqr_lds<BY, BZ, NB>(sJ, G, a.dc, i2x, i2y, i2z, x2x, x2y, x2z, x1, y1, z1, mass, bad,
+                                m != 0);
         }
 
         /* x <- x1, Reflect (cell.h:1410-1478) */
@@ -647,6 +645,16 @@ void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, Part
                              hipStream_t s) {
     PushArgs a;
     a.g = g;
+    a.dc.rhx = 1.0 / g.hx;
+    a.dc.rhy = 1.0 / g.hy;
+    a.dc.rhz = 1.0 / g.hz;
+    a.dc.r12 = 1.0 / 12.0;
+    a.dc.Dx = g.tau * g.hy * g.hz;
+    a.dc.Dy = g.tau * g.hx * g.hz;
+    a.dc.Dz = g.tau * g.hx * g.hy;
+    a.dc.rDx = 1.0 / a.dc.Dx;
+    a.dc.rDy = 1.0 / a.dc.Dy;
+    a.dc.rDz = 1.0 / a.dc.Dz;
     a.t = t;
     a.f = f;
     a.p = p;

@@ -9,9 +9,11 @@
  * synchronises five times and copies stage arrays to the host every step.
  */
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pic_internal.h"
@@ -48,9 +50,16 @@ struct pic_ctx {
     long NC = 0;
     int dev = 0;
     hipStream_t stream = nullptr;
-    FieldPtrs f{};
+    bool fp32 = false;
+    FieldPtrs f{};          /* FP64 plain form */
     PartPtrs p{};
     QueuePtrs q{};
+    FieldPtrs32 f32{};      /* FP32 precision form */
+    PartPtrs32 p32{};
+    QueuePtrs32 q32{};
+    long long *d_rho = nullptr;   /* Gauss residual scratch */
+    double *d_gres = nullptr;
+    std::vector<double> gres_ref;
     SpeciesTable sp{};
     int slot_sort[PIC_MAX_SLOTS] = {0};
     int nslots = 0;
@@ -74,39 +83,45 @@ struct pic_ctx {
 
 namespace {
 
-void free_parts(PartPtrs &p) {
+template <class P>
+void free_parts(P &p) {
     for (int k = 0; k < 6; k++)
-        if (p.p[k]) hipFree(p.p[k]);
-    if (p.sp) hipFree(p.sp);
-    if (p.count) hipFree(p.count);
-    p = PartPtrs{};
+        if (p.p[k]) (void)hipFree(p.p[k]);
+    if (p.sp) (void)hipFree(p.sp);
+    if (p.count) (void)hipFree(p.count);
+    p = P{};
 }
 
-int alloc_parts(PartPtrs &p, long ncell, int cap, bool with_count) {
-    p = PartPtrs{};
+template <class P>
+int alloc_parts(P &p, long ncell, int cap, bool with_count) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(p.p[0])>>;
+    p = P{};
     p.cap = cap;
     size_t n = (size_t)ncell * cap;
-    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&p.p[k], n * sizeof(double)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&p.p[k], n * sizeof(T)));
     HIPCHK(hipMalloc(&p.sp, n));
     if (with_count) HIPCHK(hipMalloc(&p.count, ncell * sizeof(int)));
     return PIC_SUCCESS;
 }
 
-void free_queue(QueuePtrs &q) {
+template <class Q>
+void free_queue(Q &q) {
     for (int k = 0; k < 6; k++)
-        if (q.q[k]) hipFree(q.q[k]);
-    if (q.sp) hipFree(q.sp);
-    if (q.dest) hipFree(q.dest);
-    if (q.slot) hipFree(q.slot);
-    if (q.count) hipFree(q.count);
-    q = QueuePtrs{};
+        if (q.q[k]) (void)hipFree(q.q[k]);
+    if (q.sp) (void)hipFree(q.sp);
+    if (q.dest) (void)hipFree(q.dest);
+    if (q.slot) (void)hipFree(q.slot);
+    if (q.count) (void)hipFree(q.count);
+    q = Q{};
 }
 
-int alloc_queue(QueuePtrs &q, const TileGeom &t, int qcap) {
-    q = QueuePtrs{};
+template <class Q>
+int alloc_queue(Q &q, const TileGeom &t, int qcap) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(q.q[0])>>;
+    q = Q{};
     q.qcap = qcap;
     size_t n = (size_t)t.ntiles * qcap;
-    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&q.q[k], n * sizeof(double)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&q.q[k], n * sizeof(T)));
     HIPCHK(hipMalloc(&q.sp, n));
     HIPCHK(hipMalloc(&q.dest, n * sizeof(int)));
     HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
@@ -127,7 +142,69 @@ int auto_capacity(int maxc) {
     return std::max(cap, 16);
 }
 
-/* upload staged particles into cell buckets */
+/* upload staged particles into cell buckets (cell-sorted, stable) */
+template <class P, class Q>
+int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long> &off) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(parts.p[0])>>;
+    const long NC = c->NC;
+    const long P_ = (long)c->staged.cell.size();
+    std::vector<T> sorted[6];
+    std::vector<unsigned char> ssp(P_);
+    for (int k = 0; k < 6; k++) sorted[k].resize(P_);
+    {
+        const Grid &g = c->g;
+        std::vector<long> pos(off.begin(), off.end() - 1);
+        for (long j = 0; j < P_; j++) {
+            const int cell = c->staged.cell[j];
+            const long d = pos[cell]++;
+            if (std::is_same<T, float>::value) {
+                /* FP32 form: offset inside the owning cell, kept below 1 */
+                const int ci = cell / (g.Ny * g.Nz), cl = (cell / g.Nz) % g.Ny, ck = cell % g.Nz;
+                const double hh[3] = {g.hx, g.hy, g.hz};
+                const int cc[3] = {ci, cl, ck};
+                for (int k = 0; k < 3; k++) {
+                    float o = (float)((c->staged.v[k][j] - (double)cc[k] * hh[k]) / hh[k]);
+                    if (o >= 1.0f) o = 0.99999994f;
+                    if (o < 0.0f) o = 0.0f;
+                    sorted[k][d] = (T)o;
+                }
+                for (int k = 3; k < 6; k++) sorted[k][d] = (T)c->staged.v[k][j];
+            } else {
+                for (int k = 0; k < 6; k++) sorted[k][d] = (T)c->staged.v[k][j];
+            }
+            ssp[d] = c->staged.sp[j];
+        }
+    }
+    int rc = alloc_parts(parts, NC, cap, true);
+    if (rc) return rc;
+    rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap));
+    if (rc) return rc;
+    long *d_off = nullptr;
+    T *d_src[6] = {nullptr};
+    unsigned char *d_sp = nullptr;
+    size_t Pn = (size_t)std::max(P_, 1L);
+    HIPCHK(hipMalloc(&d_off, (NC + 1) * sizeof(long)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&d_src[k], Pn * sizeof(T)));
+    HIPCHK(hipMalloc(&d_sp, Pn));
+    HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
+    if (P_ > 0) {
+        for (int k = 0; k < 6; k++)
+            HIPCHK(hipMemcpy(d_src[k], sorted[k].data(), P_ * sizeof(T), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(d_sp, ssp.data(), P_, hipMemcpyHostToDevice));
+    }
+    const T *src[6] = {d_src[0], d_src[1], d_src[2], d_src[3], d_src[4], d_src[5]};
+    if constexpr (std::is_same<T, float>::value)
+        launch_scatter_load32(d_off, NC, src, d_sp, parts, c->stream);
+    else
+        launch_scatter_load(d_off, NC, src, d_sp, parts, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(d_off);
+    for (int k = 0; k < 6; k++) (void)hipFree(d_src[k]);
+    (void)hipFree(d_sp);
+    return PIC_SUCCESS;
+}
+
 int commit(pic_ctx *c) {
     if (c->committed) return PIC_SUCCESS;
     const long NC = c->NC;
@@ -139,42 +216,9 @@ int commit(pic_ctx *c) {
     for (long k = 0; k < NC; k++) off[k + 1] += off[k];
     int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
     if (maxc > cap) cap = auto_capacity(maxc);
-    /* counting sort by cell (stable: load order kept inside a cell) */
-    std::vector<double> sorted[6];
-    std::vector<unsigned char> ssp(P);
-    for (int k = 0; k < 6; k++) sorted[k].resize(P);
-    {
-        std::vector<long> pos(off.begin(), off.end() - 1);
-        for (long j = 0; j < P; j++) {
-            long d = pos[c->staged.cell[j]]++;
-            for (int k = 0; k < 6; k++) sorted[k][d] = c->staged.v[k][j];
-            ssp[d] = c->staged.sp[j];
-        }
-    }
-    int rc = alloc_parts(c->p, NC, cap, true);
+    int rc = c->fp32 ? commit_into(c, c->p32, c->q32, cap, off)
+                     : commit_into(c, c->p, c->q, cap, off);
     if (rc) return rc;
-    rc = alloc_queue(c->q, c->t, queue_capacity(c->t, cap));
-    if (rc) return rc;
-    long *d_off = nullptr;
-    double *d_src[6] = {nullptr};
-    unsigned char *d_sp = nullptr;
-    size_t Pn = (size_t)std::max(P, 1L);
-    HIPCHK(hipMalloc(&d_off, (NC + 1) * sizeof(long)));
-    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&d_src[k], Pn * sizeof(double)));
-    HIPCHK(hipMalloc(&d_sp, Pn));
-    HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
-    if (P > 0) {
-        for (int k = 0; k < 6; k++)
-            HIPCHK(hipMemcpy(d_src[k], sorted[k].data(), P * sizeof(double), hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(d_sp, ssp.data(), P, hipMemcpyHostToDevice));
-    }
-    const double *src[6] = {d_src[0], d_src[1], d_src[2], d_src[3], d_src[4], d_src[5]};
-    launch_scatter_load(d_off, NC, src, d_sp, c->p, c->stream);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(c->stream));
-    hipFree(d_off);
-    for (int k = 0; k < 6; k++) hipFree(d_src[k]);
-    hipFree(d_sp);
     Staged empty;
     std::swap(c->staged, empty);
     c->committed = true;
@@ -184,34 +228,86 @@ int commit(pic_ctx *c) {
 
 /* bucket overflow repair: grow every bucket, re-place the arrivals whose
  * claimed slot did not fit */
-int regrow(pic_ctx *c) {
+template <class P, class Q>
+int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     const long NC = c->NC;
     std::vector<int> cnt(NC);
-    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), parts.count, NC * sizeof(int), hipMemcpyDeviceToHost));
     int maxc = 0;
     for (long k = 0; k < NC; k++) maxc = std::max(maxc, cnt[k]);
-    int oldcap = c->p.cap;
-    int cap = std::max(auto_capacity(maxc), oldcap + 8);
-    PartPtrs np;
+    const int oldcap = parts.cap;
+    const int cap = std::max(auto_capacity(maxc), oldcap + 8);
+    P np;
     int rc = alloc_parts(np, NC, cap, false);
     if (rc) return rc;
-    np.count = c->p.count;
-    launch_repack(NC, c->p, np, c->stream);
-    launch_fix_arrivals(c->t, c->q, oldcap, np, c->stream);
+    np.count = parts.count;
+    if constexpr (std::is_same<P, PartPtrs32>::value) {
+        launch_repack32(NC, parts, np, c->stream);
+        launch_fix_arrivals32(c->t, queue, oldcap, np, c->stream);
+    } else {
+        launch_repack(NC, parts, np, c->stream);
+        launch_fix_arrivals(c->t, queue, oldcap, np, c->stream);
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->p.count = nullptr;
-    free_parts(c->p);
-    c->p = np;
-    QueuePtrs nq;
+    parts.count = nullptr;
+    free_parts(parts);
+    parts = np;
+    Q nq;
     rc = alloc_queue(nq, c->t, queue_capacity(c->t, cap));
     if (rc) return rc;
-    free_queue(c->q);
-    c->q = nq;
+    free_queue(queue);
+    queue = nq;
     c->regrows++;
     c->last.bucket_capacity = cap;
     c->last.bucket_regrows = c->regrows;
     return PIC_SUCCESS;
+}
+
+int regrow(pic_ctx *c) {
+    return c->fp32 ? regrow_parts(c, c->p32, c->q32) : regrow_parts(c, c->p, c->q);
+}
+
+int *count_ptr(pic_ctx *c) { return c->fp32 ? c->p32.count : c->p.count; }
+int bucket_cap(pic_ctx *c) { return c->fp32 ? c->p32.cap : c->p.cap; }
+int *qcount_ptr(pic_ctx *c) { return c->fp32 ? c->q32.count : c->q.count; }
+
+/* stage launchers dispatching on the form */
+void enq_h_half(pic_ctx *c) {
+    if (c->fp32) launch_h_half32(c->g, c->f32, c->stream);
+    else launch_h_half(c->g, c->f, c->stream);
+}
+void enq_push(pic_ctx *c) {
+    if (c->fp32)
+        launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->q32, c->sp, c->prm.departure_cap,
+                                c->d_st, c->stream);
+    else
+        launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap,
+                                c->d_st, c->stream);
+}
+void enq_arrive(pic_ctx *c) {
+    if (c->fp32) launch_arrive_f32(c->t, c->p32, c->q32, c->d_st, c->stream);
+    else launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+}
+void enq_fold(pic_ctx *c) {
+    if (c->fp32) launch_fold_j32(c->g, c->f32, c->stream);
+    else launch_fold_j(c->g, c->f, c->stream);
+}
+void enq_h2(pic_ctx *c) {
+    if (c->fp32) launch_h2_32(c->g, c->f32, c->stream);
+    else launch_h2(c->g, c->f, c->stream);
+}
+void enq_e(pic_ctx *c) {
+    if (c->fp32) launch_e32(c->g, c->f32, c->stream);
+    else launch_e(c->g, c->f, c->stream);
+}
+void enq_e_periodic(pic_ctx *c) {
+    if (c->fp32) launch_e_periodic32(c->g, c->f32, c->stream);
+    else launch_e_periodic(c->g, c->f, c->stream);
+}
+void enq_energy(pic_ctx *c) {
+    if (c->fp32) launch_energy32(c->g, c->f32, c->d_part, c->npart, c->d_red, c->stream);
+    else launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
 }
 
 int check_ready(pic_ctx *c) {
@@ -243,7 +339,7 @@ int finish_step(pic_ctx *c) {
     c->last.duplicated = (long long)st.duplicated;
     c->last.max_cell_departures = st.max_dep;
     c->last.flags = (int)st.flags;
-    c->last.bucket_capacity = c->p.cap;
+    c->last.bucket_capacity = bucket_cap(c);
     c->last.bucket_regrows = c->regrows;
     if (st.flags & PIC_F_BUCKET_OVERFLOW) {
         int rc = regrow(c);
@@ -318,7 +414,7 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     *out = nullptr;
     if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return PIC_ERR_INVALID;
     if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
-    if (prm->precision != PIC_FP64) return PIC_ERR_INVALID;
+    if (prm->precision != PIC_FP64 && prm->precision != PIC_FP32) return PIC_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
     if (prm->device < 0 || prm->device >= ndev) return PIC_ERR_INVALID;
@@ -346,10 +442,23 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     };
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(record_hip(e, "hipStreamCreate"));
+    c->fp32 = prm->precision == PIC_FP32;
     for (int k = 0; k < PIC_NFIELDS; k++) {
-        e = hipMalloc(&c->f.f[k], g.G * sizeof(double));
+        void **ptr;
+        size_t esz = sizeof(double);
+        if (!c->fp32) {
+            ptr = (void **)&c->f.f[k];
+        } else if (k >= PIC_JX && k <= PIC_JZ) {
+            ptr = (void **)&c->f32.J[k - PIC_JX];
+            esz = sizeof(long long);
+        } else {
+            esz = sizeof(float);
+            ptr = k < PIC_HX ? (void **)&c->f32.E[k] : k < PIC_JX ? (void **)&c->f32.H[k - PIC_HX]
+                                                               : (void **)&c->f32.Q[k - PIC_QX];
+        }
+        e = hipMalloc(ptr, g.G * esz);
         if (e != hipSuccess) return fail(record_hip(e, "hipMalloc fields"));
-        e = hipMemset(c->f.f[k], 0, g.G * sizeof(double));
+        e = hipMemset(*ptr, 0, g.G * esz);
         if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
     }
     if ((e = hipMalloc(&c->d_st, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
@@ -371,9 +480,19 @@ int pic_destroy(pic_ctx *c) {
     hipSetDevice(c->dev);
     if (c->stream) hipStreamSynchronize(c->stream);
     for (int k = 0; k < PIC_NFIELDS; k++)
-        if (c->f.f[k]) hipFree(c->f.f[k]);
+        if (c->f.f[k]) (void)hipFree(c->f.f[k]);
+    for (int k = 0; k < 3; k++) {
+        if (c->f32.E[k]) (void)hipFree(c->f32.E[k]);
+        if (c->f32.H[k]) (void)hipFree(c->f32.H[k]);
+        if (c->f32.Q[k]) (void)hipFree(c->f32.Q[k]);
+        if (c->f32.J[k]) (void)hipFree(c->f32.J[k]);
+    }
     free_parts(c->p);
     free_queue(c->q);
+    free_parts(c->p32);
+    free_queue(c->q32);
+    if (c->d_rho) (void)hipFree(c->d_rho);
+    if (c->d_gres) (void)hipFree(c->d_gres);
     if (c->d_st) hipFree(c->d_st);
     if (c->h_st) hipHostFree(c->h_st);
     if (c->d_part) hipFree(c->d_part);
@@ -407,6 +526,9 @@ int pic_load_particles(pic_ctx *c, int sort, const double *x, const double *y, c
     const Grid &g = c->g;
     long dropped = 0;
     Staged &S = c->staged;
+    for (int k = 0; k < 6; k++) S.v[k].reserve(S.v[k].size() + n);
+    S.sp.reserve(S.sp.size() + n);
+    S.cell.reserve(S.cell.size() + n);
     for (long j = 0; j < n; j++) {
         /* getPointCell (cell.h:593-598): (int)((x - 0.0)/h) per axis */
         int i = (int)((x[j] - 0.0) / g.hx);
@@ -440,7 +562,20 @@ int pic_load_field(pic_ctx *c, int which, const double *host) {
     if (!c || which < 0 || which >= PIC_NFIELDS || !host) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(c->f.f[which], host, c->g.G * sizeof(double), hipMemcpyHostToDevice));
+    if (!c->fp32) {
+        HIPCHK(hipMemcpy(c->f.f[which], host, c->g.G * sizeof(double), hipMemcpyHostToDevice));
+    } else if (which >= PIC_JX && which <= PIC_JZ) {
+        std::vector<long long> t(c->g.G);
+        for (long n = 0; n < c->g.G; n++) t[n] = (long long)llrint(host[n] * PIC_J_SCALE);
+        HIPCHK(hipMemcpy(c->f32.J[which - PIC_JX], t.data(), c->g.G * sizeof(long long),
+                         hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> t(c->g.G);
+        for (long n = 0; n < c->g.G; n++) t[n] = (float)host[n];
+        float *dst = which < PIC_HX ? c->f32.E[which] : which < PIC_JX ? c->f32.H[which - PIC_HX]
+                                                                       : c->f32.Q[which - PIC_QX];
+        HIPCHK(hipMemcpy(dst, t.data(), c->g.G * sizeof(float), hipMemcpyHostToDevice));
+    }
     return PIC_SUCCESS;
 }
 
@@ -448,7 +583,20 @@ int pic_get_field(pic_ctx *c, int which, double *host) {
     if (!c || which < 0 || which >= PIC_NFIELDS || !host) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(c->dev));
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(host, c->f.f[which], c->g.G * sizeof(double), hipMemcpyDeviceToHost));
+    if (!c->fp32) {
+        HIPCHK(hipMemcpy(host, c->f.f[which], c->g.G * sizeof(double), hipMemcpyDeviceToHost));
+    } else if (which >= PIC_JX && which <= PIC_JZ) {
+        std::vector<long long> t(c->g.G);
+        HIPCHK(hipMemcpy(t.data(), c->f32.J[which - PIC_JX], c->g.G * sizeof(long long),
+                         hipMemcpyDeviceToHost));
+        for (long n = 0; n < c->g.G; n++) host[n] = (double)t[n] * PIC_J_INV;
+    } else {
+        std::vector<float> t(c->g.G);
+        const float *src = which < PIC_HX ? c->f32.E[which]
+                           : which < PIC_JX ? c->f32.H[which - PIC_HX] : c->f32.Q[which - PIC_QX];
+        HIPCHK(hipMemcpy(t.data(), src, c->g.G * sizeof(float), hipMemcpyDeviceToHost));
+        for (long n = 0; n < c->g.G; n++) host[n] = (double)t[n];
+    }
     return PIC_SUCCESS;
 }
 
@@ -457,7 +605,7 @@ int pic_yee_h_half(pic_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
     launch_reset_status(c->d_st, c->stream);
-    launch_h_half(c->g, c->f, c->stream);
+    enq_h_half(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
@@ -465,8 +613,7 @@ int pic_yee_h_half(pic_ctx *c) {
 int pic_push_deposit(pic_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
-    launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap, c->d_st,
-                            c->stream);
+    enq_push(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
@@ -474,42 +621,42 @@ int pic_push_deposit(pic_ctx *c) {
 int pic_migrate(pic_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
-    launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    enq_arrive(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int pic_fold_j(pic_ctx *c) {
     if (!c) return PIC_ERR_INVALID;
-    launch_fold_j(c->g, c->f, c->stream);
+    enq_fold(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int pic_yee_h2(pic_ctx *c) {
     if (!c) return PIC_ERR_INVALID;
-    launch_h2(c->g, c->f, c->stream);
+    enq_h2(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int pic_yee_e(pic_ctx *c) {
     if (!c) return PIC_ERR_INVALID;
-    launch_e(c->g, c->f, c->stream);
+    enq_e(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int pic_e_periodic(pic_ctx *c) {
     if (!c) return PIC_ERR_INVALID;
-    launch_e_periodic(c->g, c->f, c->stream);
+    enq_e_periodic(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int pic_energy(pic_ctx *c, double *energy) {
     if (!c) return PIC_ERR_INVALID;
-    launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
+    enq_energy(c);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -530,22 +677,21 @@ int pic_step(pic_ctx *c, int nt, double *energy) {
     HIPCHK(hipSetDevice(c->dev));
     launch_reset_status(c->d_st, c->stream);
     mark(c, PIC_STAGE_H_HALF);
-    launch_h_half(c->g, c->f, c->stream);
+    enq_h_half(c);
     mark(c, PIC_STAGE_PUSH_DEPOSIT);
-    launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap, c->d_st,
-                            c->stream);
+    enq_push(c);
     mark(c, PIC_STAGE_MIGRATE);
-    launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    enq_arrive(c);
     mark(c, PIC_STAGE_FOLD_J);
-    launch_fold_j(c->g, c->f, c->stream);
+    enq_fold(c);
     mark(c, PIC_STAGE_H2);
-    launch_h2(c->g, c->f, c->stream);
+    enq_h2(c);
     mark(c, PIC_STAGE_E);
-    launch_e(c->g, c->f, c->stream);
+    enq_e(c);
     mark(c, PIC_STAGE_E_PERIODIC);
-    launch_e_periodic(c->g, c->f, c->stream);
+    enq_e_periodic(c);
     mark(c, PIC_STAGE_ENERGY);
-    launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
+    enq_energy(c);
     mark(c, PIC_NSTAGES);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -569,7 +715,7 @@ int pic_get_status(pic_ctx *c, pic_step_status *st) {
     if (c->committed) {
         HIPCHK(hipStreamSynchronize(c->stream));
         std::vector<int> qc(c->t.ntiles);
-        HIPCHK(hipMemcpy(qc.data(), c->q.count, c->t.ntiles * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(qc.data(), qcount_ptr(c), c->t.ntiles * sizeof(int), hipMemcpyDeviceToHost));
         long long s = 0;
         for (int v : qc) s += v;
         st->departures = s;
@@ -603,7 +749,7 @@ long pic_particle_count(pic_ctx *c) {
     if (!c->committed) return (long)c->staged.cell.size();
     std::vector<int> cnt(c->NC);
     if (hipStreamSynchronize(c->stream) != hipSuccess) return -1;
-    if (hipMemcpy(cnt.data(), c->p.count, c->NC * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpy(cnt.data(), count_ptr(c), c->NC * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
         return -1;
     long n = 0;
     for (int v : cnt) n += v;
@@ -615,7 +761,7 @@ int pic_get_cell_counts(pic_ctx *c, int *counts) {
     if (rc) return rc;
     if (!counts) return PIC_ERR_INVALID;
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(counts, c->p.count, c->NC * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(counts, count_ptr(c), c->NC * sizeof(int), hipMemcpyDeviceToHost));
     return PIC_SUCCESS;
 }
 
@@ -627,9 +773,9 @@ int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, d
     HIPCHK(hipStreamSynchronize(c->stream));
     const long NC = c->NC;
     std::vector<int> cnt(NC);
-    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), count_ptr(c), NC * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<long> off(NC + 1, 0);
-    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], bucket_cap(c));
     const long P = off[NC];
     if (n_out) *n_out = P;
     if (P > cap) return PIC_ERR_INVALID;
@@ -643,7 +789,8 @@ int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, d
     HIPCHK(hipMalloc(&d_sp, Pn));
     HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
     HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
-    launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, c->stream);
+    if (c->fp32) launch_gather_particles32(c->g, NC, d_off, c->p32, d, d_sp, d_cell, c->stream);
+    else launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     double *outs[6] = {x, y, z, px, py, pz};
@@ -669,11 +816,42 @@ int pic_moments(pic_ctx *c, double *out27) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!out27) return PIC_ERR_INVALID;
-    launch_moments(c->NC, c->p, c->sp, c->d_slot_sort, c->d_part, c->npart, c->d_red, c->stream);
+    if (c->fp32)
+        launch_moments32(c->g, c->NC, c->p32, c->sp, c->d_slot_sort, c->d_part, c->npart,
+                         c->d_red, c->stream);
+    else
+        launch_moments(c->NC, c->p, c->sp, c->d_slot_sort, c->d_part, c->npart, c->d_red,
+                       c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, 27 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     std::memcpy(out27, c->h_red, 27 * sizeof(double));
+    return PIC_SUCCESS;
+}
+
+int pic_gauss_residual(pic_ctx *c, double *res, double *max_drift, int set_reference) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    const Grid &g = c->g;
+    if (!c->d_rho) HIPCHK(hipMalloc(&c->d_rho, g.G * sizeof(long long)));
+    if (!c->d_gres) HIPCHK(hipMalloc(&c->d_gres, c->NC * sizeof(double)));
+    HIPCHK(hipMemsetAsync(c->d_rho, 0, g.G * sizeof(long long), c->stream));
+    if (c->fp32) {
+        launch_rho32(g, c->NC, c->p32, c->sp, c->d_rho, c->stream);
+        launch_gauss32(g, c->f32, c->d_rho, c->d_gres, c->stream);
+    } else {
+        launch_rho64(g, c->NC, c->p, c->sp, c->d_rho, c->stream);
+        launch_gauss64(g, c->f, c->d_rho, c->d_gres, c->stream);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<double> h(c->NC);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(h.data(), c->d_gres, c->NC * sizeof(double), hipMemcpyDeviceToHost));
+    if (res) std::memcpy(res, h.data(), c->NC * sizeof(double));
+    if (set_reference || c->gres_ref.size() != h.size()) c->gres_ref = h;
+    double d = 0.0;
+    for (long k = 0; k < c->NC; k++) d = std::max(d, std::fabs(h[k] - c->gres_ref[k]));
+    if (max_drift) *max_drift = d;
     return PIC_SUCCESS;
 }
 

@@ -216,3 +216,98 @@ int pic_generate_species(const pic_params *g, const pic_species_spec *s, pic_rng
 }
 
 }  // extern "C"
+
+/* ---------------------------------------------------------------- params */
+#include <cstdio>
+#include <cstring>
+
+extern "C" {
+
+int pic_write_param_file(const char *path, const pic_param_file *p) {
+    if (!path || !p) return PIC_ERR_INVALID;
+    FILE *f = fopen(path, "wt");
+    if (!f) return PIC_ERR_INVALID;
+    /* maxwell.cu:330-377: value then description, one per line */
+    fprintf(f, "%15.5e plasma electron temperature along X \n ", p->tex0);
+    fprintf(f, "%15.5e plasma electron temperature along Y \n ", p->tey0);
+    fprintf(f, "%15.5e plasma electron temperature along Z \n ", p->tez0);
+    fprintf(f, "%15.5e beam impulse\n ", p->rimp);
+    fprintf(f, "%15.5e beam velocity dispersion \n ", p->Tb);
+    fprintf(f, "%15.5e beam and plasma density ratio \n ", p->rbd);
+    fprintf(f, "%15.5e plasma density \n ", p->ni);
+    fprintf(f, "%15.5e external magnetic field (along X) \n ", p->B0);
+    fprintf(f, "%15.5e domain size X \n ", p->lx);
+    fprintf(f, "%15.5e domain size Y \n ", p->ly);
+    fprintf(f, "%15.5e domain size Z \n ", p->lz);
+    fprintf(f, "%15.5e plasma size Y \n ", p->py);
+    fprintf(f, "%15.5e plasma size Z \n ", p->pz);
+    fprintf(f, "%15.5e beam size X \n ", p->bx);
+    fprintf(f, "%15.5e beam size Y \n ", p->by);
+    fprintf(f, "%15.5e beam size Z \n ", p->bz);
+    fprintf(f, "%15d   average number of particles in cell \n ", p->lp);
+    fprintf(f, "%15d   number of mesh nodes along X \n ", p->nx);
+    fprintf(f, "%15d   number of mesh nodes along Y \n ", p->ny);
+    fprintf(f, "%15d   number of mesh nodes along Z \n ", p->nz);
+    fprintf(f, "%15.5e timestep \n ", p->tau);
+    fprintf(f, "%15d   1 if beam-plasma interaction, 0 if beam-beam \n", p->beam_plasma);
+    fprintf(f, "%15d   moment to start from saved\n ", p->start_from_file);
+    fprintf(f, "%15d   phase to start from save\n ", p->phase);
+    fprintf(f, "%15d   total steps \n ", p->ts);
+    fprintf(f, "%15d   number of steps between diagnostic files \n ", p->ms);
+    fclose(f);
+    return PIC_SUCCESS;
+}
+
+int pic_read_param_file(const char *path, pic_param_file *p) {
+    if (!path || !p) return PIC_ERR_INVALID;
+    FILE *f = fopen(path, "rt");
+    if (!f) return PIC_ERR_INVALID;
+    double v[26];
+    char line[512];
+    int n = 0;
+    while (n < 26 && fgets(line, sizeof(line), f)) {
+        char *endp = nullptr;
+        double d = strtod(line, &endp);
+        if (endp == line) continue;   /* blank or text-only line */
+        v[n++] = d;
+    }
+    fclose(f);
+    if (n < 26) return PIC_ERR_INVALID;
+    p->tex0 = v[0]; p->tey0 = v[1]; p->tez0 = v[2];
+    p->rimp = v[3]; p->Tb = v[4]; p->rbd = v[5]; p->ni = v[6]; p->B0 = v[7];
+    p->lx = v[8]; p->ly = v[9]; p->lz = v[10];
+    p->py = v[11]; p->pz = v[12];
+    p->bx = v[13]; p->by = v[14]; p->bz = v[15];
+    p->lp = (int)v[16];
+    p->nx = (int)v[17]; p->ny = (int)v[18]; p->nz = (int)v[19];
+    p->tau = v[20];
+    p->beam_plasma = (int)v[21]; p->start_from_file = (int)v[22]; p->phase = (int)v[23];
+    p->ts = (int)v[24]; p->ms = (int)v[25];
+    return PIC_SUCCESS;
+}
+
+int pic_species_from_param_file(const pic_param_file *pf, pic_params *grid,
+                                pic_species_spec species[3]) {
+    if (!pf || !grid || !species || pf->lp <= 0) return PIC_ERR_INVALID;
+    grid->nx = pf->nx; grid->ny = pf->ny; grid->nz = pf->nz;
+    grid->lx = pf->lx; grid->ly = pf->ly; grid->lz = pf->lz;
+    grid->tau = pf->tau;
+    grid->ni = pf->ni;
+    grid->n_per_cell = pf->lp;
+    grid->ion_q_m = 1.0 / 1836.0;
+    std::memset(species, 0, 3 * sizeof(pic_species_spec));
+    pic_species_spec &ion = species[0], &el = species[1], &bm = species[2];
+    ion.sort = PIC_ION; ion.ppc = pf->lp; ion.density = pf->ni + pf->rbd; ion.charge = 1.0;
+    ion.q_m = 1.0 / 1836.0;
+    el.sort = PIC_PLASMA_ELECTRON; el.ppc = 2 * pf->lp; el.density = pf->ni; el.charge = -1.0;
+    el.q_m = -1.0;
+    el.vth[0] = pf->tex0; el.vth[1] = pf->tey0; el.vth[2] = pf->tez0;
+    if (pf->beam_plasma == 1) el.drift[0] = -pf->rbd * pf->rimp;
+    bm.sort = PIC_BEAM_ELECTRON; bm.ppc = pf->lp; bm.density = pf->rbd; bm.charge = -1.0;
+    bm.q_m = -1.0;
+    bm.drift[0] = pf->rimp;
+    bm.vth[0] = bm.vth[1] = bm.vth[2] = pf->Tb * pf->rimp;
+    return PIC_SUCCESS;
+}
+
+}  // extern "C"

@@ -50,12 +50,40 @@ struct FieldPtrs {
     double *f[PIC_NFIELDS];
 };
 
+/* FP32 precision form: E, H, Q in FP32, J as int64 fixed point with
+ * resolution 2^-40 (exact, order-free accumulation) */
+#define PIC_J_SCALE 1099511627776.0           /* 2^40 */
+#define PIC_J_INV (1.0 / 1099511627776.0)
+
+struct FieldPtrs32 {
+    float *E[3], *H[3], *Q[3];
+    long long *J[3];
+};
+
 /* particle storage (FP64 plain form) */
 struct PartPtrs {
     double *p[6];            /* x, y, z, pu, pv, pw */
     unsigned char *sp;       /* species slot */
     int *count;              /* per bulk cell */
     int cap;
+};
+
+/* particle storage of the FP32 form: x, y, z are offsets in [0,1) inside the
+ * owning cell (cell units), px, py, pz the momenta */
+struct PartPtrs32 {
+    float *p[6];
+    unsigned char *sp;
+    int *count;
+    int cap;
+};
+
+struct QueuePtrs32 {
+    float *q[6];
+    unsigned char *sp;
+    int *dest;
+    int *slot;
+    int *count;
+    int qcap;
 };
 
 struct QueuePtrs {
@@ -87,6 +115,13 @@ void launch_fold_j(const Grid &g, FieldPtrs f, hipStream_t s);
 void launch_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks, double *out,
                    hipStream_t s);
 void launch_reset_status(DevStatus *st, hipStream_t s);
+void launch_h_half32(const Grid &g, FieldPtrs32 f, hipStream_t s);
+void launch_h2_32(const Grid &g, FieldPtrs32 f, hipStream_t s);
+void launch_e32(const Grid &g, FieldPtrs32 f, hipStream_t s);
+void launch_e_periodic32(const Grid &g, FieldPtrs32 f, hipStream_t s);
+void launch_fold_j32(const Grid &g, FieldPtrs32 f, hipStream_t s);
+void launch_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks, double *out,
+                     hipStream_t s);
 TileGeom make_tile_geom(int Nx, int Ny, int Nz);
 
 /* particle launchers (k_particles_f64.hip) */
@@ -103,5 +138,34 @@ void launch_gather_particles(long ncell, const long *offsets, PartPtrs p, double
                              unsigned char *dsp, int *dcell, hipStream_t s);
 void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort,
                     double *partials, int nblocks, double *out27, hipStream_t s);
+
+/* FP32 form (k_particles_f32.hip) */
+void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
+                             QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
+                             hipStream_t s);
+void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
+                       hipStream_t s);
+void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
+                           const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
+void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);
+void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtrs32 to,
+                           hipStream_t s);
+void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
+                               double *dst[6], unsigned char *dsp, int *dcell, hipStream_t s);
+void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,
+                      const int *slot_sort, double *partials, int nblocks, double *out27,
+                      hipStream_t s);
+
+/* Gauss-law residual div E - rho, FP64 (k_gauss.hip): rho by CIC on the
+ * cell-centre lattice into an int64 fixed-point array of G entries, then
+ * per bulk centre res = div E - rho; returns max |res - ref| if ref != 0 */
+void launch_rho64(const Grid &g, long ncell, PartPtrs p, SpeciesTable sp, long long *rho,
+                  hipStream_t s);
+void launch_rho32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp, long long *rho,
+                  hipStream_t s);
+void launch_gauss64(const Grid &g, FieldPtrs f, const long long *rho, double *res,
+                    hipStream_t s);
+void launch_gauss32(const Grid &g, FieldPtrs32 f, const long long *rho, double *res,
+                    hipStream_t s);
 
 #endif

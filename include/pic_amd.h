@@ -139,8 +139,8 @@ int pic_get_field(pic_ctx *ctx, int which, double *host);
  * after the stream completes; *energy (may be NULL) receives the
  * getElectricEnergy() value that Diagnose(nt) writes (gpu_plasma.h:621-686). */
 int pic_step(pic_ctx *ctx, int nt, double *energy);
-/* nsteps steps from nt0 with one synchronisation at the end; energies[i]
- * (may be NULL) receives the energy after step nt0+i. */
+/* nsteps consecutive pic_step calls from nt0; energies[i] (may be NULL)
+ * receives the energy after step nt0+i.  Stops at the first error. */
 int pic_run(pic_ctx *ctx, int nt0, int nsteps, double *energies);
 
 /* Stage entry points: the launch seam of Plasma::Step, in this order.
@@ -179,6 +179,14 @@ int pic_get_cell_counts(pic_ctx *ctx, int *counts);
 int pic_get_particles(pic_ctx *ctx, double *x, double *y, double *z, double *px,
                       double *py, double *pz, double *m, double *q_m, int *sort,
                       int *cell, long cap, long *n_out);
+/* Gauss-law residual div E - rho at every bulk cell centre, in FP64 (both
+ * forms).  rho is the cloud-in-cell charge on the cell-centre lattice that
+ * the charge-conserving deposit (cell.h:1511-1693) conserves; the Yee update
+ * then keeps div E - rho constant up to round-off.  res (may be NULL)
+ * receives nx*ny*nz values; the first call (or set_reference != 0) stores the
+ * reference, and *max_drift receives max |res - reference|.  No counterpart
+ * in the reference, which never fills its Rho arrays (cell.h:2085). */
+int pic_gauss_residual(pic_ctx *ctx, double *res, double *max_drift, int set_reference);
 /* per sort (3) x 9 moments, reduced in FP64: N, sum x, y, z, sum px, py, pz,
  * sum (m/q_m)(gamma-1), sum |px|+|py|+|pz| */
 int pic_moments(pic_ctx *ctx, double *out27);
@@ -223,6 +231,40 @@ int pic_generate_species(const pic_params *grid, const pic_species_spec *s,
                          int nthreads, double *x, double *y, double *z,
                          double *px, double *py, double *pz, long cap,
                          long *n_out, double *m_out);
+
+/* ------------------------------------------------------------------------
+ * Parameter file: the 000_params.dat format that writeParamsFile writes
+ * (maxwell.cu:330-377) and readParameterFile (params.h:11-20, declared but
+ * never defined in the reference) would read: one value per line, in order
+ * tex0 tey0 tez0 rimp Tb rbd ni B0 lx ly lz py pz bx by bz lp nx ny nz tau
+ * beam_plasma start_from_file phase ts ms.
+ * ---------------------------------------------------------------------- */
+typedef struct pic_param_file {
+    double tex0, tey0, tez0;   /* plasma electron thermal velocities */
+    double rimp, Tb;           /* beam velocity and relative spread */
+    double rbd, ni;            /* beam/plasma density ratio, plasma density */
+    double B0;                 /* external magnetic field along x */
+    double lx, ly, lz;         /* domain size */
+    double py, pz;             /* plasma size in y, z */
+    double bx, by, bz;         /* beam size */
+    int lp;                    /* average particles per cell */
+    int nx, ny, nz;            /* cells */
+    double tau;                /* time step */
+    int beam_plasma, start_from_file, phase;
+    int ts;                    /* total steps (run_control.h TOTAL_STEPS) */
+    int ms;                    /* steps between diagnostics */
+} pic_param_file;
+
+int pic_read_param_file(const char *path, pic_param_file *out);
+int pic_write_param_file(const char *path, const pic_param_file *in);
+/* The reference's species from a parameter file (maxwell.cu:379-555,
+ * 606-619): ions lp per cell of density ni + rbd (q/m = 1/1836, cold);
+ * plasma electrons 2 lp per cell of density ni with thermal velocities
+ * tex0..tez0 and, when beam_plasma is set, the return-current drift
+ * -rbd * rimp; beam electrons lp per cell of density rbd drifting at rimp
+ * along x with spread Tb * rimp.  Also fills the grid of *grid. */
+int pic_species_from_param_file(const pic_param_file *pf, pic_params *grid,
+                                pic_species_spec species[3]);
 
 #ifdef __cplusplus
 }

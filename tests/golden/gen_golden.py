@@ -1,7 +1,8 @@
 """Write the golden fixtures of tests/golden/ from the reference's own code.
 
 Runs oracle/_ref/ref_push (the reference's particle.h, Particle::Move) and
-oracle/_ref/ref_rng (the reference's maxwell.cxx, g05cae_/g05dde_), both
+oracle/_ref/ref_rng (the reference's maxwell.cxx: g05cae_/g05dde_, and
+writeParamsFile for params_ref.dat), both
 built by `make -C oracle ref` from the sources in the reference checkout,
 and stores their inputs and outputs as .npz data.  Run where that checkout
 exists:  python tests/golden/gen_golden.py
@@ -46,11 +47,18 @@ def rng_vectors(nu=2000, nn=2000, mean=0.3, sd=0.005):
     return dict(uniform=v[:nu], normal=v[nu:], mean=np.float64(mean), sd=np.float64(sd))
 
 
+def params_file():
+    """000_params.dat written by the reference's writeParamsFile."""
+    subprocess.run([os.path.join(REF_BIN, "ref_rng"), "params"], cwd=HERE, check=True)
+    os.replace(os.path.join(HERE, "000_params.dat"), os.path.join(HERE, "params_ref.dat"))
+
+
 def main():
     if not os.path.exists(os.path.join(REF_BIN, "ref_push")):
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference exists")
     np.savez_compressed(os.path.join(HERE, "boris_ref.npz"), **boris_vectors())
     np.savez_compressed(os.path.join(HERE, "rng_ref.npz"), **rng_vectors())
+    params_file()
     print("wrote", os.listdir(HERE))
 
 

@@ -128,3 +128,32 @@ def test_create_without_gpu_reports_no_device(picmod):
     p = picmod.default_params()
     ctx = C.c_void_p()
     assert lib.pic_create(C.byref(p), C.byref(ctx)) == 3
+
+
+def test_param_file_roundtrip_matches_reference_writer(picmod, tmp_path):
+    """params_ref.dat was written by the reference's writeParamsFile
+    (maxwell.cu:330-377); the reader must recover its values and the writer
+    must reproduce it byte for byte."""
+    ref = os.path.join(GOLDEN, "params_ref.dat")
+    pf = picmod.read_param_file(ref)
+    assert (pf.tex0, pf.tey0, pf.tez0, pf.rimp, pf.Tb, pf.rbd, pf.ni, pf.B0) == \
+        (1e-3, 2e-3, 3e-3, 0.2, 0.14, 2e-3, 1.0, 0.5)
+    assert (pf.lx, pf.ly, pf.lz, pf.py, pf.pz) == (1.1424, 0.05, 0.06, 0.04, 0.05)
+    assert (pf.lp, pf.nx, pf.ny, pf.nz, pf.tau) == (1000, 100, 4, 5, 1e-3)
+    assert (pf.beam_plasma, pf.start_from_file, pf.phase, pf.ts, pf.ms) == (1, 0, 0, 5, 2)
+    out = tmp_path / "000_params.dat"
+    picmod.write_param_file(str(out), pf)
+    assert out.read_bytes() == open(ref, "rb").read()
+
+
+def test_species_from_param_file(picmod):
+    pf = picmod.read_param_file(os.path.join(GOLDEN, "params_ref.dat"))
+    grid, (ion, el, beam) = picmod.species_from_param_file(pf)
+    assert (grid.nx, grid.ny, grid.nz, grid.tau) == (100, 4, 5, 1e-3)
+    # getMassCharge (maxwell.cu:606-619): weights ni/lp/2, (ni+rbd)/lp, rbd/lp
+    assert ion.ppc == 1000 and el.ppc == 2000 and beam.ppc == 1000
+    assert ion.charge * ion.density / ion.ppc == pytest.approx((1.0 + 2e-3) / 1000)
+    assert el.charge * el.density / el.ppc == pytest.approx(-1.0 / 1000 / 2)
+    assert beam.charge * beam.density / beam.ppc == pytest.approx(-2e-3 / 1000)
+    assert beam.drift[0] == 0.2 and beam.vth[0] == pytest.approx(0.14 * 0.2)
+    assert el.drift[0] == pytest.approx(-2e-3 * 0.2) and el.vth[2] == 3e-3
