@@ -38,7 +38,25 @@ class PicParams(C.Structure):
                 ("ni", C.c_double), ("n_per_cell", C.c_int),
                 ("ion_q_m", C.c_double), ("tau", C.c_double),
                 ("precision", C.c_int), ("departure_cap", C.c_int),
-                ("bucket_capacity", C.c_int), ("device", C.c_int)]
+                ("bucket_capacity", C.c_int), ("device", C.c_int),
+                ("differentiable", C.c_int), ("ghost_mode", C.c_int)]
+
+
+class PicDumpSort(C.Structure):
+    """pic_dump_sort: one sort of a mumu dump."""
+    _fields_ = [("n", C.c_long), ("q_m", C.c_double), ("m", C.c_double),
+                ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+                ("z", C.POINTER(C.c_double)), ("px", C.POINTER(C.c_double)),
+                ("py", C.POINTER(C.c_double)), ("pz", C.POINTER(C.c_double)),
+                ("cap", C.c_long)]
+
+
+class PicAdjointIO(C.Structure):
+    """pic_adjoint_io: per-id particle adjoints and ghosted field adjoints."""
+    _fields_ = [("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+                ("z", C.POINTER(C.c_double)), ("px", C.POINTER(C.c_double)),
+                ("py", C.POINTER(C.c_double)), ("pz", C.POINTER(C.c_double)),
+                ("field", C.POINTER(C.c_double) * 6)]
 
 
 class PicStepStatus(C.Structure):
@@ -88,6 +106,12 @@ EXPORTS = [
     # readback and timing
     "pic_profile", "pic_profile_read", "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
     "pic_get_particles", "pic_moments", "pic_gauss_residual",
+    # state dumps (mumu files) and control points
+    "pic_get_params", "pic_dump_file_name", "pic_dump_write", "pic_dump_read",
+    "pic_save_state_dump", "pic_load_state_dump", "pic_control_point",
+    # differentiable form
+    "pic_particle_id_count", "pic_get_particle_ids", "pic_forward_record", "pic_adjoint",
+    "pic_adjoint_timing",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
     "pic_generate_species",
@@ -150,6 +174,20 @@ def load_library(path=LIB_PATH):
                                         C.c_long, lp]),
         "pic_moments": (C.c_int, [vp, dp]),
         "pic_gauss_residual": (C.c_int, [vp, dp, dp, C.c_int]),
+        "pic_get_params": (C.c_int, [vp, C.POINTER(PicParams)]),
+        "pic_dump_file_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+        "pic_dump_write": (C.c_int, [C.c_char_p, C.c_long, C.POINTER(dp),
+                                     C.POINTER(PicDumpSort)]),
+        "pic_dump_read": (C.c_int, [C.c_char_p, C.c_long, C.POINTER(dp),
+                                    C.POINTER(PicDumpSort)]),
+        "pic_save_state_dump": (C.c_int, [vp, C.c_char_p]),
+        "pic_load_state_dump": (C.c_int, [vp, C.c_char_p, lp]),
+        "pic_control_point": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, dp, C.c_char_p]),
+        "pic_particle_id_count": (C.c_long, [vp]),
+        "pic_get_particle_ids": (C.c_int, [vp, ip, C.c_long, lp]),
+        "pic_forward_record": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+        "pic_adjoint": (C.c_int, [vp, dp, C.POINTER(PicAdjointIO), C.POINTER(PicAdjointIO)]),
+        "pic_adjoint_timing": (C.c_int, [vp, dp]),
         "pic_rng_seed": (None, [C.POINTER(PicRng), C.c_int, C.c_int, C.c_int]),
         "pic_rng_uniform": (C.c_double, [C.POINTER(PicRng)]),
         "pic_rng_normal": (C.c_double, [C.POINTER(PicRng), C.c_double, C.c_double]),
@@ -336,6 +374,100 @@ class Pic:
         _check(self.lib.pic_moments(self._ctx, _dptr(a)), "pic_moments")
         return a.reshape(3, 9)
 
+    # -- state dumps ---------------------------------------------------------
+    def params_out(self):
+        p = PicParams()
+        _check(self.lib.pic_get_params(self._ctx, C.byref(p)), "pic_get_params")
+        return p
+
+    def save_dump(self, path):
+        _check(self.lib.pic_save_state_dump(self._ctx, os.fsencode(path)),
+               "pic_save_state_dump")
+
+    def load_dump(self, path):
+        d = C.c_long(0)
+        _check(self.lib.pic_load_state_dump(self._ctx, os.fsencode(path), C.byref(d)),
+               "pic_load_state_dump")
+        return d.value
+
+    def control_point(self, path, nt=0, num=0, report=None):
+        """RMS differences (CheckArraySilent) of Ex..Qz and of the particles
+        against a dump; appends the control_points.dat line to `report`."""
+        out = np.zeros(13)
+        _check(self.lib.pic_control_point(self._ctx, os.fsencode(path), nt, num, _dptr(out),
+                                          os.fsencode(report) if report else None),
+               "pic_control_point")
+        return out
+
+    # -- differentiable form -------------------------------------------------
+    def id_count(self):
+        return int(self.lib.pic_particle_id_count(self._ctx))
+
+    def particle_ids(self):
+        """Particle ids in the order of particles()."""
+        n = self.particle_count()
+        a = np.empty(max(n, 1), dtype=np.int32)
+        nout = C.c_long(0)
+        _check(self.lib.pic_get_particle_ids(self._ctx, _iptr(a), len(a), C.byref(nout)),
+               "pic_get_particle_ids")
+        return a[:nout.value]
+
+    def forward_record(self, nsteps, ckpt_every):
+        """Run nsteps keeping checkpoints every ckpt_every steps; returns the
+        energy history W_1..W_nsteps."""
+        out = np.zeros(nsteps, dtype=np.float64)
+        _check(self.lib.pic_forward_record(self._ctx, nsteps, ckpt_every, _dptr(out)),
+               "pic_forward_record")
+        return out
+
+    def adjoint(self, weights=None, seed=None):
+        """Vector-Jacobian product of sum_t w_t W_t + <seed, s_T> with respect to
+        the initial state.  seed / result: dict with particle keys x, y, z, px,
+        py, pz (arrays over particle ids) and field keys Ex..Hz (ghosted)."""
+        keep = []
+
+        def io(d, alloc):
+            st = PicAdjointIO()
+            for k in ("x", "y", "z", "px", "py", "pz"):
+                a = d.get(k) if d else None
+                if a is None and alloc:
+                    a = np.zeros(max(self.id_count(), 1))
+                if a is not None:
+                    a = np.ascontiguousarray(a, dtype=np.float64)
+                    keep.append(a)
+                    if d is not None:
+                        d[k] = a
+                    setattr(st, k, _dptr(a))
+            for i, k in enumerate(FIELD_NAMES[:6]):
+                a = d.get(k) if d else None
+                if a is None and alloc:
+                    a = np.zeros(self.G)
+                if a is not None:
+                    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+                    keep.append(a)
+                    if d is not None:
+                        d[k] = a
+                    st.field[i] = _dptr(a)
+            return st
+        w = None
+        if weights is not None:
+            w = np.ascontiguousarray(weights, dtype=np.float64)
+            keep.append(w)
+        sd = io(dict(seed), False) if seed else None
+        out = {}
+        gr = io(out, True)
+        _check(self.lib.pic_adjoint(self._ctx, _dptr(w), C.byref(sd) if sd else None,
+                                    C.byref(gr)), "pic_adjoint")
+        n = self.id_count()
+        return {k: (v[:n] if k in ("x", "y", "z", "px", "py", "pz") else v)
+                for k, v in out.items()}
+
+    def adjoint_timing(self):
+        """(forward with checkpoints, recompute, adjoint steps) in ms."""
+        a = np.zeros(3)
+        _check(self.lib.pic_adjoint_timing(self._ctx, _dptr(a)), "pic_adjoint_timing")
+        return tuple(float(v) for v in a)
+
 
 # ---------------------------------------------------------------------------
 # Seeded beam-plasma workloads (units c = 1, w_pe = 1; cell size 0.1)
@@ -399,6 +531,59 @@ def species_from_param_file(pf):
     _check(load_library().pic_species_from_param_file(C.byref(pf), C.byref(grid), specs),
            "pic_species_from_param_file")
     return grid, list(specs)
+
+
+def dump_file_name(num, nt):
+    buf = C.create_string_buffer(64)
+    _check(load_library().pic_dump_file_name(num, nt, buf, 64), "pic_dump_file_name")
+    return buf.value.decode()
+
+
+def dump_write(path, fields, sorts):
+    """fields: 12 arrays (or None); sorts: 3 dicts with q_m, m, x..pz."""
+    G = len(next(f for f in fields if f is not None))
+    keep = []
+    fp = (C.POINTER(C.c_double) * 12)()
+    for k, f in enumerate(fields):
+        if f is not None:
+            a = np.ascontiguousarray(f, dtype=np.float64)
+            keep.append(a)
+            fp[k] = _dptr(a)
+    ds = (PicDumpSort * 3)()
+    for s, d in enumerate(sorts):
+        ds[s].n = len(d["x"])
+        ds[s].q_m, ds[s].m, ds[s].cap = d["q_m"], d["m"], len(d["x"])
+        for k in ("x", "y", "z", "px", "py", "pz"):
+            a = np.ascontiguousarray(d[k], dtype=np.float64)
+            keep.append(a)
+            setattr(ds[s], k, _dptr(a))
+    _check(load_library().pic_dump_write(os.fsencode(path), G, fp, ds), "pic_dump_write")
+
+
+def dump_read(path, G):
+    """(12 field arrays, 3 sort dicts) of a mumu dump."""
+    lib = load_library()
+    ds = (PicDumpSort * 3)()
+    rc = lib.pic_dump_read(os.fsencode(path), G, None, ds)
+    if rc not in (0, 6):
+        _check(rc, "pic_dump_read")
+    fields = [np.empty(G) for _ in range(12)]
+    fp = (C.POINTER(C.c_double) * 12)(*[_dptr(f) for f in fields])
+    sorts = []
+    for s in range(3):
+        n = ds[s].n
+        d = {k: np.empty(max(n, 1)) for k in ("x", "y", "z", "px", "py", "pz")}
+        for k, a in d.items():
+            setattr(ds[s], k, _dptr(a))
+        ds[s].cap = n
+        sorts.append(d)
+    _check(lib.pic_dump_read(os.fsencode(path), G, fp, ds), "pic_dump_read")
+    for s in range(3):
+        n = ds[s].n
+        for k in list(sorts[s]):
+            sorts[s][k] = sorts[s][k][:n]
+        sorts[s]["q_m"], sorts[s]["m"] = ds[s].q_m, ds[s].m
+    return fields, sorts
 
 
 def make_params(n, l, tau, precision=FP64, **kw):

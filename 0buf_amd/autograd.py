@@ -1,0 +1,151 @@
+"""torch.autograd.Function over the C ABI of the differentiable form.
+
+EnergyHistory.apply(run, x, y, z, px, py, pz, E, H) runs `run.nsteps` steps of
+the FP64 step on the GPU from the given initial state and returns the energy
+history W_1..W_T (W_t = sum over all ghosted nodes of Ex^2+Ey^2+Ez^2, the
+reference's Diagnose, gpu_plasma.h:621-686).  Its backward calls
+pic_adjoint: the checkpointed recompute and the hand-written adjoint
+kernels.  Particle tensors are in load order (concatenated species, the ids
+of the library); E and H are (3, G) ghosted arrays.
+
+The initial-state maps of the differentiable workloads (beam drift, the 8
+displacement amplitudes a_k sin(2 pi k x / Lx), the uniform Bx0) are plain
+torch code on top, so torch's own autograd carries the adjoint of s_0 to the
+parameters.  Tensors live on the CPU here: the GPU work happens inside the
+library, whose boundary takes host arrays.
+"""
+import importlib
+import math
+
+import numpy as np
+import torch
+
+pic = importlib.import_module("0buf_amd")
+
+PART_KEYS = ("x", "y", "z", "px", "py", "pz")
+FIELD_KEYS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+class Run:
+    """Grid, species (sort, m, q_m, count in load order), steps and checkpoint
+    interval of a differentiable run; `last` keeps the context of the latest
+    forward for its backward and timing."""
+
+    def __init__(self, params, species, nsteps, ckpt_every):
+        self.params = params
+        self.params.differentiable = 1
+        self.species = species
+        self.nsteps = nsteps
+        self.ckpt_every = ckpt_every
+        self.last = None
+
+    def make(self, x, y, z, px, py, pz, E, H):
+        g = pic.Pic(self.params)
+        off = 0
+        for (sort, m, q_m, n) in self.species:
+            sl = slice(off, off + n)
+            g.load_particles(sort, x[sl], y[sl], z[sl], px[sl], py[sl], pz[sl], m, q_m)
+            off += n
+        for i in range(3):
+            g.load_field(pic.EX + i, E[i])
+            g.load_field(pic.HX + i, H[i])
+        return g
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64, copy=False)
+
+
+class EnergyHistory(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, run, x, y, z, px, py, pz, E, H):
+        if run.last is not None:
+            run.last.close()
+        g = run.make(*[_np(t) for t in (x, y, z, px, py, pz)], _np(E), _np(H))
+        W = g.forward_record(run.nsteps, run.ckpt_every)
+        run.last = g
+        ctx.run = run
+        ctx.g = g
+        return torch.from_numpy(W)
+
+    @staticmethod
+    def backward(ctx, gW):
+        res = ctx.g.adjoint(weights=_np(gW))
+        parts = [torch.from_numpy(res[k].copy()) for k in PART_KEYS]
+        E = torch.from_numpy(np.stack([res[k] for k in FIELD_KEYS[:3]]))
+        H = torch.from_numpy(np.stack([res[k] for k in FIELD_KEYS[3:]]))
+        return (None, *parts, E, H)
+
+
+def energy_loss_window(W, first, last):
+    """sum of W_t for steps t = first..last (1-based, inclusive)"""
+    return W[first - 1:last].sum()
+
+
+# ---------------------------------------------------------------------------
+# Differentiable beam-plasma workloads
+# ---------------------------------------------------------------------------
+
+class BeamPlasma:
+    """Config 3/4 style run.  theta = (drift, a_1..a_8, bx0): the beam's drift
+    velocity along x, the amplitudes of the beam displacement
+    x -> x + sum_k a_k sin(2 pi k x / Lx), and a uniform external Hx = bx0
+    on every node.  Base state: the seeded generator's particles with the
+    beam drawn around drift0 and no displacement."""
+
+    def __init__(self, cfg, drift0=0.3, scale_n=None, nsteps=None, ckpt_every=20,
+                 stream_mode=None):
+        spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
+        params, parts = pic.generate_workload(spec_cfg, scale_n=scale_n,
+                                              stream_mode=stream_mode)
+        self.params = params
+        self.lx = params.lx
+        self.species = []
+        cols = [[] for _ in range(6)]
+        beam = []
+        for (sort, x, y, z, px, py, pz, m, q_m) in parts:
+            self.species.append((sort, m, q_m, len(x)))
+            for c, a in zip(cols, (x, y, z, px, py, pz)):
+                c.append(a)
+            beam.append(np.full(len(x), sort == pic.BEAM_ELECTRON))
+        base = [torch.from_numpy(np.concatenate(c)) for c in cols]
+        self.beam = torch.from_numpy(np.concatenate(beam))
+        self.x0, self.y0, self.z0 = base[0], base[1], base[2]
+        p = torch.stack(base[3:6])
+        u = p / torch.sqrt(1.0 + (p * p).sum(0))
+        u[0] = torch.where(self.beam, u[0] - drift0, u[0])
+        self.u_th = u          # thermal part of the beam velocity, full velocity otherwise
+        self.G = (params.nx + 2) * (params.ny + 2) * (params.nz + 2)
+        nsteps = nsteps or spec_cfg["steps"]
+        self.run = Run(params, self.species, nsteps, ckpt_every)
+
+    def initial_state(self, theta):
+        drift, amps = theta[0], theta[1:9]
+        bx0 = theta[9] if theta.numel() > 9 else theta.new_zeros(())
+        k = torch.arange(1, 9, dtype=theta.dtype)
+        disp = (amps[:, None] * torch.sin(2.0 * math.pi * k[:, None] * self.x0[None, :] /
+                                          self.lx)).sum(0)
+        x = torch.where(self.beam, self.x0 + disp, self.x0)
+        ux = torch.where(self.beam, self.u_th[0] + drift, self.u_th[0])
+        u = torch.stack([ux, self.u_th[1], self.u_th[2]])
+        p = u / torch.sqrt(1.0 - (u * u).sum(0))
+        E = torch.zeros(3, self.G, dtype=theta.dtype)
+        H = torch.stack([bx0 * torch.ones(self.G, dtype=theta.dtype),
+                         torch.zeros(self.G, dtype=theta.dtype),
+                         torch.zeros(self.G, dtype=theta.dtype)])
+        return x, self.y0, self.z0, p[0], p[1], p[2], E, H
+
+    def energy_history(self, theta):
+        return EnergyHistory.apply(self.run, *self.initial_state(theta))
+
+    def loss(self, theta, first=150, last=200):
+        return energy_loss_window(self.energy_history(theta), first, last)
+
+    def loss_value(self, theta, first=150, last=200):
+        """forward only (no checkpoints), for finite differences"""
+        with torch.no_grad():
+            st = self.initial_state(theta)
+        g = self.run.make(*[_np(t) for t in st[:6]], _np(st[6]), _np(st[7]))
+        W = g.run(1, self.run.nsteps)
+        g.close()
+        return float(W[first - 1:last].sum())

@@ -4,7 +4,7 @@ Usage: python 0buf_amd/build.py   (or __graft_entry__.build())
 
 Every HIP file is compiled with hipcc --offload-arch=gfx950; the host-only
 initializer with g++.  Objects go to 0buf_amd/_build/, the library to
-0buf_amd/libpic_amd.so.  No cmake/ninja: the dependency graph is five files.
+0buf_amd/libpic_amd.so.  No cmake/ninja: the dependency graph is eight files.
 """
 import os
 import subprocess
@@ -16,6 +16,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpic_amd.so")
+EXAMPLE = os.path.join(OUT, "plasma_main")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PIC_OFFLOAD_ARCH", "gfx950")
@@ -33,9 +34,12 @@ HIP_SOURCES = {
     "k_particles_f64.hip": ["-ffp-contract=off"],
     "k_particles_f32.hip": ["-ffp-contract=fast"],
     "k_gauss.hip": ["-ffp-contract=off"],
+    "k_adjoint.hip": ["-ffp-contract=off"],
     "pic_api.hip": ["-ffp-contract=off"],
+    "pic_adjoint.hip": ["-ffp-contract=off"],
 }
-CXX_SOURCES = {"pic_init.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"]}
+CXX_SOURCES = {"pic_init.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"],
+               "pic_io.cpp": ["-O2", "-fPIC", "-std=c++17"]}
 
 
 def _run(cmd):
@@ -67,6 +71,10 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, LIB)
+    # the reference's main.cu driver on the C++ facade (include/plasma.hpp)
+    _run(["g++", "-std=c++17", "-O2", "-I", os.path.join(ROOT, "include"),
+          os.path.join(HERE, "examples", "plasma_main.cpp"), "-o", EXAMPLE, "-L", HERE,
+          "-lpic_amd", "-Wl,-rpath,$ORIGIN/.."])
     return LIB
 
 

@@ -174,6 +174,23 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_z(Grid g, FV<T, JT> f)
     }
 }
 
+/* ghost_mode 1 only (no reference counterpart): Ex and Ez along y over the
+ * full ghosted extent of x and z, after the reference's copies */
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_e_periodic_y(Grid g, FV<T, JT> f) {
+    long t = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (t >= (long)(g.Nx + 2) * g.sy) return;
+    int i = (int)(t / g.sy), k = (int)(t % g.sy);
+    long base = (long)i * g.sx + k;
+    long n0 = base, n1 = base + g.sy, nN = base + (long)g.Ny * g.sy,
+         nN1 = base + (long)(g.Ny + 1) * g.sy;
+    for (int c = 0; c < 3; c += 2) {
+        T *E = f.E[c];
+        E[n0] = E[nN];
+        E[nN1] = E[n1];
+    }
+}
+
 /* SetPeriodicCurrentComponent (gpu_plasma.h:926-978) -> GPU_CurrentPeriodic /
  * periodicCurrentElement (kernels.cu:1124-1159), all three components for one
  * face direction `dir`; the three directions run as three launches in the
@@ -267,6 +284,9 @@ void e_periodic(const Grid &g, FV<T, JT> f, hipStream_t s) {
                        dim3(PIC_BLOCK), 0, s, g, f);
     hipLaunchKernelGGL((k_e_periodic_z<T, JT>), dim3(blocks_for((long)(g.Nx + 2) * (g.Ny + 2))),
                        dim3(PIC_BLOCK), 0, s, g, f);
+    if (g.ghost_mode == 1)
+        hipLaunchKernelGGL((k_e_periodic_y<T, JT>), dim3(blocks_for((long)(g.Nx + 2) * g.sy)),
+                           dim3(PIC_BLOCK), 0, s, g, f);
 }
 template <typename T, typename JT>
 void fold_j(const Grid &g, FV<T, JT> f, hipStream_t s) {

@@ -18,9 +18,18 @@
  */
 #include "pic_internal.h"
 
+/* copies of the LDS J box: lane l deposits into copy l % JCOPIES, which
+ * divides the same-address conflicts of the 64-bit LDS atomics (consecutive
+ * lanes hold particles of one cell) by JCOPIES; the copies are summed before
+ * the global flush (integer sums: the result does not depend on it) */
+#ifndef PIC_JCOPIES32
+#define PIC_JCOPIES32 4
+#endif
+
 namespace {
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
+constexpr int JCOPIES = PIC_JCOPIES32;
 
 struct PushArgs32 {
     Grid g;
@@ -113,13 +122,14 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ float sF[6 * NB];
-    __shared__ long long sJ[3 * NB];
+    __shared__ long long sJ[JCOPIES * 3 * NB];
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     extern __shared__ unsigned sMask[];
+    long long *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
     const Grid &g = a.g;
     const int tid = threadIdx.x;
@@ -156,7 +166,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
             val = Fsrc[fi][(long)gu * g.sx + (long)gv * g.sy + gw];
         sF[e] = val;
     }
-    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
+    for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
@@ -257,9 +267,9 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
             const float x2z = (m & 1) ? 0.5f : oz + duz * sr;
             const float *cj = a.cj[s];
             const float e1x = m ? x2x : o1x, e1y = m ? x2y : o1y, e1z = m ? x2z : o1z;
-            vb_seg<BY, BZ, NB>(sJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
+            vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
             if (m)
-                vb_seg<BY, BZ, NB>(sJ, ca, cb, cc, rx1, ry1, rz1, x2x, x2y, x2z, o1x, o1y, o1z,
+                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rx1, ry1, rz1, x2x, x2y, x2z, o1x, o1y, o1z,
                                    cj);
         }
 
@@ -322,7 +332,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
     }
 
     for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
-        const long long val = sJ[e];
+        long long val = sJ[e];
+        for (int r = 1; r < JCOPIES; r++) val += sJ[r * 3 * NB + e];
         if (val == 0) continue;
         int comp = e / NB, nn = e % NB;
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;

@@ -16,24 +16,22 @@
  * The arithmetic mirrors cell.h / particle.h in the reference's evaluation
  * order and is built with -ffp-contract=off.
  */
-#include "pic_internal.h"
+#include "k_push_math.h"
 
 #pragma clang fp contract(off)
 
 namespace {
 
-constexpr int TXC = 2, TYC = 4, TZC = 8;
+/* copies of the LDS J box, lane l depositing into copy l % JCOPIES: fewer
+ * same-address conflicts of the LDS atomics (consecutive lanes hold
+ * particles of one cell); the copies are summed in a fixed order before the
+ * global flush */
+#ifndef PIC_JCOPIES64
+#define PIC_JCOPIES64 2
+#endif
 
-/* Divisors that are constant over a run, with their correctly rounded
- * reciprocals: q = divc(x, d, r) returns the correctly rounded x/d in three
- * instructions (Markstein: q0 = x r, exact remainder x - q0 d by FMA, one
- * FMA correction), bit-identical to IEEE division. */
-struct DivConst {
-    double rhx, rhy, rhz;     /* 1/hx, 1/hy, 1/hz */
-    double r12;               /* 1/12 */
-    double Dx, Dy, Dz;        /* (tau hy) hz, (tau hx) hz, (tau hx) hy (cell.h:1578-1580) */
-    double rDx, rDy, rDz;
-};
+constexpr int TXC = 2, TYC = 4, TZC = 8;
+constexpr int JCOPIES = PIC_JCOPIES64;
 
 struct PushArgs {
     Grid g;
@@ -45,55 +43,28 @@ struct PushArgs {
     SpeciesTable sp;
     int dep_cap;
     DevStatus *st;
+    DiffArgs d;        /* differentiable form (DIFF instantiation only) */
+    int snap_on;
 };
 
-/* pow(v, -0.5) (particle.h:73,116) correctly rounded: r = 1/sqrt(v) refined
- * by one Newton step whose residual 1 - v r^2 is formed exactly with FMAs. */
-__device__ __forceinline__ double pow_m05(double v) {
-    double s = sqrt(v);
-    double r = 1.0 / s;
-    double rr = r * r;
-    double rr_lo = fma(r, r, -rr);
-    double e = fma(-v, rr, 1.0);
-    e = fma(-v, rr_lo, e);
-    return fma(0.5 * r, e, r);
+/* one node contribution to the LDS J box: FP64 atomic add, or in the
+ * differentiable form the contribution v 2^48 rounded to an integer with the
+ * 1.5 2^52 shifter (exact for |v| < 8, flagged otherwise) and added as int64 */
+template <bool DIFF>
+__device__ __forceinline__ void jadd(double *sJ, int n, double v, unsigned &bad) {
+    if constexpr (DIFF) {
+        const double t = fma(v, PIC_J_SCALE64, 6755399441055744.0);
+        const long long q = __double_as_longlong(t) - 0x4338000000000000LL;
+        if (!(fabs(v) < 8.0)) bad = 1;
+        atomicAdd(reinterpret_cast<unsigned long long *>(sJ) + n, (unsigned long long)q);
+    } else {
+        atomicAdd(&sJ[n], v);
+    }
 }
-
-__device__ __forceinline__ double divc(double x, double d, double r) {
-    double q0 = x * r;
-    double e = fma(-q0, d, x);
-    return fma(e, r, q0);
-}
-
-/* cell.h:1427-1478 Reflect: (x>xm)(x-xm) + (x<0)(xm+x) + (0<x<xm) x.  The
- * select form returns the same bits in every case, including exact 0 and xm
- * (both map to +0) and the signed zeros of the products. */
-__device__ __forceinline__ double reflect1(double x, double xm) {
-    return x > xm ? x - xm : (x < 0.0 ? xm + x : ((x > 0 && x < xm) ? x : 0.0));
-}
-
-template <int BY, int BZ>
-__device__ __forceinline__ double interp(const double *F, int u, int v, int w, double sx,
-                                         double sx1, double sy, double sy1, double sz,
-                                         double sz1) {
-    /* cell.h:1054-1153 Interpolate3D: t2 + t1 */
-    const double *p = F + (u * BY + v) * BZ + w;
-    double t_ilk = p[0], t_ilk1 = p[1], t_il1k = p[BZ], t_il1k1 = p[BZ + 1];
-    double t_i1lk = p[BY * BZ], t_i1lk1 = p[BY * BZ + 1];
-    double t_i1l1k = p[BY * BZ + BZ], t_i1l1k1 = p[BY * BZ + BZ + 1];
-    double t1 = sx * (sy * (sz * t_ilk + sz1 * t_ilk1) + sy1 * (sz * t_il1k + sz1 * t_il1k1));
-    double t2 = sx1 * (sy * (sz * t_i1lk + sz1 * t_i1lk1) + sy1 * (sz * t_i1l1k + sz1 * t_i1l1k1));
-    return t2 + t1;
-}
-
-struct Geo {
-    double hx, hy, hz, x0, y0, z0, tau;
-    int ca, cb, cc;   /* owner cell inside the tile */
-};
 
 /* cell.h:1511-1693 pqr: 12 node contributions of one straight segment,
  * accumulated into the LDS J box at local node (owner + tile index - 2). */
-template <int BY, int BZ, int NB>
+template <int BY, int BZ, int NB, bool DIFF>
 __device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, const DivConst &dc, int ix,
                                         int iy, int iz, double xx, double xy, double xz,
                                         double x1x, double x1y, double x1z, double a,
@@ -116,22 +87,18 @@ __device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, const DivConst
     const int u = c.ca + ix - 2, v = c.cb + iy - 2, w = c.cc + iz - 2;
     const int n = (u * BY + v) * BZ + w;
     double *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
-    atomicAdd(&Jx[n], su * (dy1 * dz1 + s1));
-    atomicAdd(&Jx[n + 1], su * (dy1 * dz - s1));
-    atomicAdd(&Jx[n + BZ], su * (dy * dz1 - s1));
-    atomicAdd(&Jx[n + BZ + 1], su * (dy * dz + s1));
-    atomicAdd(&Jy[n], sv * (dx1 * dz1 + s2));
-    atomicAdd(&Jy[n + 1], sv * (dx1 * dz - s2));
-    atomicAdd(&Jy[n + BY * BZ], sv * (dx * dz1 - s2));
-    atomicAdd(&Jy[n + BY * BZ + 1], sv * (dx * dz + s2));
-    atomicAdd(&Jz[n], sw * (dx1 * dy1 + s3));
-    atomicAdd(&Jz[n + BZ], sw * (dx1 * dy - s3));
-    atomicAdd(&Jz[n + BY * BZ], sw * (dx * dy1 - s3));
-    atomicAdd(&Jz[n + BY * BZ + BZ], sw * (dx * dy + s3));
-}
-
-__device__ __forceinline__ double transit_average(double h, int i1, int i2, double x0) {
-    return (h * (((double)i1 + (double)i2) * 0.5 - 2.0) + x0);
+    jadd<DIFF>(Jx, n, su * (dy1 * dz1 + s1), bad);
+    jadd<DIFF>(Jx, n + 1, su * (dy1 * dz - s1), bad);
+    jadd<DIFF>(Jx, n + BZ, su * (dy * dz1 - s1), bad);
+    jadd<DIFF>(Jx, n + BZ + 1, su * (dy * dz + s1), bad);
+    jadd<DIFF>(Jy, n, sv * (dx1 * dz1 + s2), bad);
+    jadd<DIFF>(Jy, n + 1, sv * (dx1 * dz - s2), bad);
+    jadd<DIFF>(Jy, n + BY * BZ, sv * (dx * dz1 - s2), bad);
+    jadd<DIFF>(Jy, n + BY * BZ + 1, sv * (dx * dz + s2), bad);
+    jadd<DIFF>(Jz, n, sw * (dx1 * dy1 + s3), bad);
+    jadd<DIFF>(Jz, n + BZ, sw * (dx1 * dy - s3), bad);
+    jadd<DIFF>(Jz, n + BY * BZ, sw * (dx * dy1 - s3), bad);
+    jadd<DIFF>(Jz, n + BY * BZ + BZ, sw * (dx * dy + s3), bad);
 }
 
 /* smallest pos in [from,to) whose bit (base+pos) equals `want`, else `to` */
@@ -152,12 +119,12 @@ __device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from
     return to;
 }
 
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ double sF[6 * NB];
-    __shared__ double sJ[3 * NB];
+    __shared__ double sJ[JCOPIES * 3 * NB];
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
@@ -166,9 +133,11 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
     __shared__ unsigned sFlags;
     __shared__ unsigned long long sDrop, sDup;
     extern __shared__ unsigned sMask[];
+    double *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
     const Grid &g = a.g;
     const int tid = threadIdx.x;
+    const bool snap = DIFF && a.snap_on;
 
     /* XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin,
      * so each XCD gets a contiguous run of neighbouring tiles and their
@@ -190,8 +159,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
         int cnt = 0;
         if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
-            cnt = a.p.count[((long)gi * g.Ny + gl) * g.Nz + gk];
+            const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+            cnt = a.p.count[cell];
             cnt = cnt < cap ? cnt : cap;
+            if (snap) a.d.snap.count[cell] = cnt;
         }
         sCnt[c] = cnt;
         sLeave[c] = 0;
@@ -206,7 +177,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
             val = a.f.f[PIC_EX + fi][(long)gu * g.sx + (long)gv * g.sy + gw];
         sF[e] = val;
     }
-    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) sJ[e] = 0.0;
+    for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0.0;
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
@@ -248,120 +219,64 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         G.y0 = (double)gl * hy;
         G.z0 = (double)gk * hz;
 
-        double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
+        const double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
         double pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
         const unsigned char s = a.p.sp[pidx];
+        const int pid = DIFF ? a.p.id[pidx] : 0;
         const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
+        if (snap) {
+            a.d.snap.p[0][pidx] = x;
+            a.d.snap.p[1][pidx] = y;
+            a.d.snap.p[2][pidx] = z;
+            a.d.snap.p[3][pidx] = pu;
+            a.d.snap.p[4][pidx] = pv;
+            a.d.snap.p[5][pidx] = pw;
+            a.d.snap.sp[pidx] = s;
+            a.d.snap.id[pidx] = pid;
+        }
 
-        /* GetField (cell.h:1257-1290): tile indices from (x - x0)/h
-         * (getCellNumber, getCellNumberCenter: cell.h:527-549), weights from
-         * the global x/h (s1..s6_interpolate: cell.h:1193-1236) */
         const double frx = divc(x - G.x0, hx, a.dc.rhx);
         const double fry = divc(y - G.y0, hy, a.dc.rhy);
         const double frz = divc(z - G.z0, hz, a.dc.rhz);
         double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
         if (!(x < 0 || y < 0 || z < 0)) {
-            const double xh = divc(x, hx, a.dc.rhx), yh = divc(y, hy, a.dc.rhy),
-                         zh = divc(z, hz, a.dc.rhz);
-            int ix = (int)(frx + 1.0) + 1, i1x = (int)((frx + 1.0) + 1.5);
-            double s1 = (int)(xh + 1.0) - xh, s2 = (int)(xh + 1.5) - 0.5 - xh;
-            int iy = (int)(fry + 1.0) + 1, i1y = (int)((fry + 1.0) + 1.5);
-            double s3 = (int)(yh + 1.0) - yh, s4 = (int)(yh + 1.5) - 0.5 - yh;
-            int iz = (int)(frz + 1.0) + 1, i1z = (int)((frz + 1.0) + 1.5);
-            double s5 = (int)(zh + 1.0) - zh, s6 = (int)(zh + 1.5) - 0.5 - zh;
-            double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4, s51 = 1. - s5,
-                   s61 = 1. - s6;
-            if (ix < 2 || ix > 3 || i1x < 2 || i1x > 3 || iy < 2 || iy > 3 || i1y < 2 ||
-                i1y > 3 || iz < 2 || iz > 3 || i1z < 2 || i1z > 3) {
+            const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
+            const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
+            const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
+                         s51 = 1. - s5, s61 = 1. - s6;
+            if (!W.ok) {
                 bad = 1;
             } else {
                 const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
-                Ex = interp<BY, BZ>(sF + 0 * NB, ua + ix, va + i1y, wa + i1z, s1, s11, s4, s41, s6, s61);
-                Ey = interp<BY, BZ>(sF + 1 * NB, ua + i1x, va + iy, wa + i1z, s2, s21, s3, s31, s6, s61);
-                Ez = interp<BY, BZ>(sF + 2 * NB, ua + i1x, va + i1y, wa + iz, s2, s21, s4, s41, s5, s51);
-                Hx = interp<BY, BZ>(sF + 3 * NB, ua + i1x, va + iy, wa + iz, s2, s21, s3, s31, s5, s51);
-                Hy = interp<BY, BZ>(sF + 4 * NB, ua + ix, va + i1y, wa + iz, s1, s11, s4, s41, s5, s51);
-                Hz = interp<BY, BZ>(sF + 5 * NB, ua + ix, va + iy, wa + i1z, s1, s11, s3, s31, s6, s61);
+                Ex = interp<BY, BZ>(sF + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, s1, s11, s4, s41, s6, s61);
+                Ey = interp<BY, BZ>(sF + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, s2, s21, s3, s31, s6, s61);
+                Ez = interp<BY, BZ>(sF + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, s2, s21, s4, s41, s5, s51);
+                Hx = interp<BY, BZ>(sF + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, s2, s21, s3, s31, s5, s51);
+                Hy = interp<BY, BZ>(sF + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, s1, s11, s4, s41, s5, s51);
+                Hz = interp<BY, BZ>(sF + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, s1, s11, s3, s31, s6, s61);
             }
         }
 
         /* Particle::Move (particle.h:160-202) */
-        const double tau1 = q_m * tau * 0.5;
-        pu += tau1 * Ex;
-        pv += tau1 * Ey;
-        pw += tau1 * Ez;
-        double ps = tau1 * pow_m05((pu * pu + pv * pv + pw * pw) * 1. + 1.0);
-        double bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
-        double su = pu + pv * bz - pw * by;
-        double sv = pv + pw * bx - pu * bz;
-        double sw = pw + pu * by - pv * bx;
-        double q1 = bx * bx, q2 = by * by, q3 = bz * bz;
-        double q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-        double sden = q1 + 1. + q2 + q3;
-        const double rden = 1.0 / sden;   /* one reciprocal for the three quotients */
-        double pu1 = divc((q1 + 1.) * su + (q4 + bz) * sv + (q6 - by) * sw, sden, rden);
-        double pv1 = divc((q4 - bz) * su + (q2 + 1.) * sv + (q5 + bx) * sw, sden, rden);
-        double pw1 = divc((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.) * sw, sden, rden);
-        pu = pu1 + Ex * tau1;
-        pv = pv1 + Ey * tau1;
-        pw = pw1 + Ez * tau1;
-        ps = pow_m05(((pu * pu + pv * pv + pw * pw) + 1.0));
-        const double x1 = x + tau * (ps * pu);
-        const double y1 = y + tau * (ps * pv);
-        const double z1 = z + tau * (ps * pw);
+        const Boris B = boris_move(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, q_m * tau * 0.5);
+        pu = B.p1[0];
+        pv = B.p1[1];
+        pw = B.p1[2];
+        const double x1 = x + tau * (B.g2 * pu);
+        const double y1 = y + tau * (B.g2 * pv);
+        const double z1 = z + tau * (B.g2 * pw);
 
-        /* CurrentToMesh (cell.h:1306-1422).  Tile indices of both ends
-         * (getCellNumberCenterCurrent, cell.h:560-566), crossing code
-         * m = 4|di| + 2|dl| + |dk|; each case's relay point is formed with
-         * cheap selects and the one division of the case is done once. */
+        /* CurrentToMesh (cell.h:1306-1422): one code path for every lane;
+         * without a crossing the first segment is the whole path (x2 = x1)
+         * and the second is skipped */
         {
-            const double t1x = divc(x1 - G.x0, hx, a.dc.rhx);
-            const double t1y = divc(y1 - G.y0, hy, a.dc.rhy);
-            const double t1z = divc(z1 - G.z0, hz, a.dc.rhz);
-            const int i1x = (int)((frx + 1.5) + 1), i1y = (int)((fry + 1.5) + 1),
-                      i1z = (int)((frz + 1.5) + 1);
-            const int i2x = (int)((t1x + 1.5) + 1), i2y = (int)((t1y + 1.5) + 1),
-                      i2z = (int)((t1z + 1.5) + 1);
-            int m = 4 * abs(i2x - i1x) + 2 * abs(i2y - i1y) + abs(i2z - i1z);
-            if (m > 7) m = 0;
-            const double dxs = x1 - x, dys = y1 - y, dzs = z1 - z;
-            /* relay coordinates on the crossed dual-cell faces
-             * (getCellTransitAverage, cell.h:600-602) */
-            const double fx = transit_average(hx, i1x, i2x, G.x0);
-            const double fy = transit_average(hy, i1y, i2y, G.y0);
-            const double fz = transit_average(hz, i1z, i2z, G.z0);
-            double num = 0.0, den = 1.0;
-            switch (m) {
-            case 1: num = fz - z; den = dzs; break;
-            case 2: num = fy - y; den = dys; break;
-            case 4: num = fx - x; den = dxs; break;
-            case 3:
-                num = (fz - z) * dzs + (fy - y) * dys;
-                den = (z1 - z) * (z1 - z) + (y1 - y) * (y1 - y);
-                break;
-            case 5:
-                num = (fz - z) * dzs + (fx - x) * dxs;
-                den = (z1 - z) * (z1 - z) + (x1 - x) * (x1 - x);
-                break;
-            case 6:
-                num = (fy - y) * dys + (fx - x) * dxs;
-                den = (y1 - y) * (y1 - y) + (x1 - x) * (x1 - x);
-                break;
-            default: break;
-            }
-            const double sr = num / den;
-            /* crossed axes take the face value, the others the point of the
-             * straight path at parameter sr (getRatioBasedX, cell.h:617) */
-            const double x2x = (m & 4) ? fx : (x + dxs * sr);
-            const double x2y = (m & 2) ? fy : (y + dys * sr);
-            const double x2z = (m & 1) ? fz : (z + dzs * sr);
-            /* one code path for every lane: without a crossing the first
-             * segment is the whole path (x2 = x1) and the second is skipped */
-            const double e1x = m ? x2x : x1, e1y = m ? x2y : y1, e1z = m ? x2z : z1;
-            pqr_lds<BY, BZ, NB>(sJ, G, a.dc, i1x, i1y, i1z, x, y, z, e1x, e1y, e1z, mass, bad,
-                                true);
-            pqr_lds<BY, BZ, NB>(sJ, G, a.dc, i2x, i2y, i2z, x2x, x2y, x2z, x1, y1, z1, mass, bad,
-                                m != 0);
+            const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
+            const int m = R.m;
+            const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
+            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y, e1z,
+                                      mass, bad, true);
+            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i2x, R.i2y, R.i2z, R.x2x, R.x2y, R.x2z, x1,
+                                      y1, z1, mass, bad, m != 0);
         }
 
         /* x <- x1, Reflect (cell.h:1410-1478) */
@@ -431,6 +346,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
                             a.q.q[4][e] = pv;
                             a.q.q[5][e] = pw;
                             a.q.sp[e] = s;
+                            if (DIFF) a.q.id[e] = pid;
                             a.q.dest[e] = (int)(((long)rx[ax] * g.Ny + ry[ay]) * g.Nz + rz[az]);
                         }
             }
@@ -455,6 +371,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
                 const long from = cell * cap + sidx, to = cell * cap + h;
                 for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
                 a.p.sp[to] = a.p.sp[from];
+                if (DIFF) a.p.id[to] = a.p.id[from];
                 h = next_bit(sMask, base, h + 1, np, true);
                 sidx = next_bit(sMask, base, sidx + 1, n, false);
             }
@@ -465,13 +382,23 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
 
     /* tile J box -> global J (GPU_WriteAllCurrents, kernels.cu:83-112) */
     for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
-        double val = sJ[e];
-        if (val == 0.0) continue;
         int comp = e / NB, nn = e % NB;
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
         int gu = I0 + u, gv = L0 + v, gw = K0 + w;
         if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
-        atomicAdd(&a.f.f[PIC_JX + comp][(long)gu * g.sx + (long)gv * g.sy + gw], val);
+        const long gidx = (long)gu * g.sx + (long)gv * g.sy + gw;
+        if constexpr (DIFF) {
+            const unsigned long long *u = reinterpret_cast<const unsigned long long *>(sJ);
+            unsigned long long val = u[e];
+            for (int r = 1; r < JCOPIES; r++) val += u[r * 3 * NB + e];
+            if (val == 0ull) continue;
+            atomicAdd(reinterpret_cast<unsigned long long *>(a.d.jfix[comp]) + gidx, val);
+        } else {
+            double val = sJ[e];
+            for (int r = 1; r < JCOPIES; r++) val += sJ[r * 3 * NB + e];
+            if (val == 0.0) continue;
+            atomicAdd(&a.f.f[PIC_JX + comp][gidx], val);
+        }
     }
 
     if (maxout > 0) {
@@ -492,6 +419,18 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
     }
 }
 
+/* differentiable form: fixed-point J -> the FP64 J arrays (exact for
+ * |J| < 2^5), and the fixed-point arrays cleared for the next step */
+__global__ __launch_bounds__(PIC_BLOCK) void k_jfix_to_f64(long G, FieldPtrs f, DiffArgs d) {
+    const long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (n >= G) return;
+    for (int c = 0; c < 3; c++) {
+        const long long v = d.jfix[c][n];
+        f.f[PIC_JX + c][n] = (double)v * (1.0 / PIC_J_SCALE64);
+        d.jfix[c][n] = 0;
+    }
+}
+
 /* arrivals: each queued leaver claims the next slot of its destination
  * bucket (the Insert of GPU_ArrangeFlights, kernels.cu:357-360) */
 __global__ __launch_bounds__(PIC_BLOCK) void k_arrive(PartPtrs p, QueuePtrs q, DevStatus *st) {
@@ -508,6 +447,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive(PartPtrs p, QueuePtrs q, D
             const long to = (long)d * p.cap + slot;
             for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
             p.sp[to] = q.sp[e];
+            if (p.id) p.id[to] = q.id[e];
         } else {
             over = true;
         }
@@ -549,6 +489,34 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_repack(long ncell, PartPtrs from,
     const long dst = c * to.cap + j;
     for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
     to.sp[dst] = from.sp[e];
+    if (to.id) to.id[dst] = from.id[e];
+}
+
+/* bucket copy with any capacities (checkpoint save and restore): occupied
+ * slots, ids and counts */
+__global__ __launch_bounds__(PIC_BLOCK) void k_copy_parts(long ncell, PartPtrs from, PartPtrs to) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * from.cap) return;
+    const long c = e / from.cap;
+    const int j = (int)(e % from.cap);
+    int n = from.count[c];
+    if (n > from.cap) n = from.cap;
+    if (j == 0) to.count[c] = n;
+    if (j >= n) return;
+    const long dst = c * to.cap + j;
+    for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
+    to.sp[dst] = from.sp[e];
+    if (to.id) to.id[dst] = from.id[e];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_scatter_ids(const long *off, long ncell,
+                                                          const int *src, PartPtrs p) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    if (j >= off[c + 1] - off[c]) return;
+    p.id[e] = src[off[c] + j];
 }
 
 __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals(QueuePtrs q, int oldcap, PartPtrs to) {
@@ -562,6 +530,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals(QueuePtrs q, int old
         const long dst = (long)q.dest[e] * to.cap + slot;
         for (int k = 0; k < 6; k++) to.p[k][dst] = q.q[k][e];
         to.sp[dst] = q.sp[e];
+        if (to.id) to.id[dst] = q.id[e];
     }
 }
 
@@ -569,7 +538,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles(long ncell, cons
                                                                PartPtrs p, double *d0, double *d1,
                                                                double *d2, double *d3, double *d4,
                                                                double *d5, unsigned char *dsp,
-                                                               int *dcell) {
+                                                               int *dcell, int *did) {
     const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (e >= ncell * p.cap) return;
     const long c = e / p.cap;
@@ -584,6 +553,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles(long ncell, cons
     d5[dst] = p.p[5][e];
     dsp[dst] = p.sp[e];
     dcell[dst] = (int)c;
+    if (did) did[dst] = p.id ? p.id[e] : -1;
 }
 
 /* FP64 moments per sort: N, sum x,y,z, sum p, sum (m/q_m)(gamma-1), sum |p| */
@@ -642,19 +612,10 @@ size_t push_lds_bytes(const TileGeom &t, int cap) {
 
 void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             hipStream_t s) {
+                             const DiffArgs *diff, hipStream_t s) {
     PushArgs a;
     a.g = g;
-    a.dc.rhx = 1.0 / g.hx;
-    a.dc.rhy = 1.0 / g.hy;
-    a.dc.rhz = 1.0 / g.hz;
-    a.dc.r12 = 1.0 / 12.0;
-    a.dc.Dx = g.tau * g.hy * g.hz;
-    a.dc.Dy = g.tau * g.hx * g.hz;
-    a.dc.Dz = g.tau * g.hx * g.hy;
-    a.dc.rDx = 1.0 / a.dc.Dx;
-    a.dc.rDy = 1.0 / a.dc.Dy;
-    a.dc.rDz = 1.0 / a.dc.Dz;
+    a.dc = make_divconst(g);
     a.t = t;
     a.f = f;
     a.p = p;
@@ -662,8 +623,18 @@ void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, Part
     a.sp = sp;
     a.dep_cap = dep_cap;
     a.st = st;
-    hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK),
-                       push_lds_bytes(t, p.cap), s, a);
+    a.d = diff ? *diff : DiffArgs{};
+    a.snap_on = diff && diff->snap.p[0] != nullptr;
+    if (diff)
+        hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+    else
+        hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+}
+
+void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s) {
+    hipLaunchKernelGGL(k_jfix_to_f64, dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g.G, f, d);
 }
 
 void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s) {
@@ -687,10 +658,21 @@ void launch_fix_arrivals(const TileGeom &t, QueuePtrs q, int oldcap, PartPtrs to
 }
 
 void launch_gather_particles(long ncell, const long *offsets, PartPtrs p, double *dst[6],
-                             unsigned char *dsp, int *dcell, hipStream_t s) {
+                             unsigned char *dsp, int *dcell, int *did, hipStream_t s) {
     hipLaunchKernelGGL(k_gather_particles, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
                        ncell, offsets, p, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dsp,
-                       dcell);
+                       dcell, did);
+}
+
+void launch_scatter_ids(const long *offsets, long ncell, const int *src, PartPtrs p,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_ids, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       offsets, ncell, src, p);
+}
+
+void launch_copy_parts(long ncell, PartPtrs from, PartPtrs to, hipStream_t s) {
+    hipLaunchKernelGGL(k_copy_parts, dim3(blocks_for(ncell * from.cap)), dim3(PIC_BLOCK), 0, s,
+                       ncell, from, to);
 }
 
 void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort,

@@ -1,0 +1,377 @@
+/*
+ * pic_adjoint.hip -- host side of the differentiable form (the reference has
+ * no counterpart): forward run with checkpoints, segment recompute with
+ * per-step records, and the reverse sweep through the adjoint kernels of
+ * k_adjoint.hip.
+ *
+ * Memory (one GPU): a checkpoint is the bucketed particle state (six FP64
+ * arrays, species, id, counts) plus E and H; checkpoints are kept every K
+ * steps and at the end of the run.  During the reverse sweep one segment of
+ * K steps is recomputed from its checkpoint and each step's pre-push
+ * particle state (written by the push kernel itself) and E^n, H^{n+1/2} are
+ * recorded; the segment is then reversed step by step.  The J deposit of
+ * this form is order-free (int64 fixed point), so the recompute is bitwise
+ * the first forward run.
+ */
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "pic_ctx.h"
+
+namespace picx {
+void enq_h_half(pic_ctx *c);
+void enq_push(pic_ctx *c);
+void enq_arrive(pic_ctx *c);
+void enq_fold(pic_ctx *c);
+void enq_h2(pic_ctx *c);
+void enq_e(pic_ctx *c);
+void enq_e_periodic(pic_ctx *c);
+}  // namespace picx
+
+using namespace picx;
+
+struct FieldSet {
+    double *f[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   /* E, H */
+};
+
+struct AdjointStore {
+    int nsteps = 0, K = 0;
+    bool recorded = false;
+    long long anomalies = 0;          /* dropped + duplicated leavers in the forward run */
+    std::vector<PartPtrs> ck;         /* checkpoints at 0, K, 2K, ... and at nsteps */
+    std::vector<FieldSet> ckf;
+    std::vector<PartPtrs> rec;        /* per-step particle records of one segment */
+    std::vector<FieldSet> recf;       /* E^n, H^{n+1/2} of each recorded step */
+    double *Eend[3] = {nullptr, nullptr, nullptr};
+    AdjLam lam{};
+    LamParts lp{};
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    double ms[3] = {0, 0, 0};
+};
+
+namespace {
+
+void free_fields(FieldSet &f) {
+    for (int k = 0; k < 6; k++)
+        if (f.f[k]) (void)hipFree(f.f[k]);
+    f = FieldSet{};
+}
+
+int alloc_fields(FieldSet &f, long G) {
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&f.f[k], G * sizeof(double)));
+    return PIC_SUCCESS;
+}
+
+void free_lam(AdjointStore *s) {
+    for (int k = 0; k < 3; k++) {
+        if (s->lam.E[k]) (void)hipFree(s->lam.E[k]);
+        if (s->lam.H[k]) (void)hipFree(s->lam.H[k]);
+        if (s->lam.J[k]) (void)hipFree(s->lam.J[k]);
+        if (s->lam.Q[k]) (void)hipFree(s->lam.Q[k]);
+        if (s->Eend[k]) (void)hipFree(s->Eend[k]);
+        s->Eend[k] = nullptr;
+    }
+    for (int k = 0; k < 6; k++)
+        if (s->lp.v[k]) (void)hipFree(s->lp.v[k]);
+    s->lam = AdjLam{};
+    s->lp = LamParts{};
+}
+
+int save_state(pic_ctx *c, PartPtrs &dst, FieldSet &f) {
+    if (dst.p[0] && dst.cap != c->p.cap) free_parts(dst);
+    if (!dst.p[0]) {
+        int rc = alloc_parts(dst, c->NC, c->p.cap, true, true);
+        if (rc) return rc;
+    }
+    if (!f.f[0]) {
+        int rc = alloc_fields(f, c->g.G);
+        if (rc) return rc;
+    }
+    launch_copy_parts(c->NC, c->p, dst, c->stream);
+    const size_t bytes = c->g.G * sizeof(double);
+    for (int k = 0; k < 6; k++)
+        HIPCHK(hipMemcpyAsync(f.f[k], c->f.f[PIC_EX + k], bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+int restore_state(pic_ctx *c, const PartPtrs &src, const FieldSet &f) {
+    if (src.cap > c->p.cap) return PIC_ERR_STATE;
+    launch_copy_parts(c->NC, src, c->p, c->stream);
+    const size_t bytes = c->g.G * sizeof(double);
+    for (int k = 0; k < 6; k++)
+        HIPCHK(hipMemcpyAsync(c->f.f[PIC_EX + k], f.f[k], bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    HIPCHK(hipGetLastError());
+    return PIC_SUCCESS;
+}
+
+/* one forward step that records the pre-push particle state into `snap`
+ * (through the push kernel) and E^n, H^{n+1/2} into `f` */
+int recorded_step(pic_ctx *c, PartPtrs &snap, FieldSet &f) {
+    const size_t bytes = c->g.G * sizeof(double);
+    launch_reset_status(c->d_st, c->stream);
+    enq_h_half(c);
+    for (int k = 0; k < 6; k++)
+        HIPCHK(hipMemcpyAsync(f.f[k], c->f.f[PIC_EX + k], bytes, hipMemcpyDeviceToDevice,
+                              c->stream));
+    c->rec_snap = snap;
+    enq_push(c);
+    c->rec_snap = PartPtrs{};
+    enq_arrive(c);
+    enq_fold(c);
+    enq_h2(c);
+    enq_e(c);
+    enq_e_periodic(c);
+    HIPCHK(hipGetLastError());
+    const int cap = c->p.cap;
+    int rc = finish_step(c);
+    if (rc) return rc;
+    /* the recompute sees the forward's own counts, which the live capacity
+     * (the largest of the run) already holds */
+    if (c->p.cap != cap) return PIC_ERR_STATE;
+    return PIC_SUCCESS;
+}
+
+int zero_async(void *p, size_t bytes, hipStream_t s) {
+    HIPCHK(hipMemsetAsync(p, 0, bytes, s));
+    return PIC_SUCCESS;
+}
+
+}  // namespace
+
+namespace picx {
+void free_adjoint(pic_ctx *c) {
+    AdjointStore *s = c->adj;
+    if (!s) return;
+    for (auto &p : s->ck) free_parts(p);
+    for (auto &f : s->ckf) free_fields(f);
+    for (auto &p : s->rec) free_parts(p);
+    for (auto &f : s->recf) free_fields(f);
+    free_lam(s);
+    for (int k = 0; k < 3; k++)
+        if (s->ev[k]) (void)hipEventDestroy(s->ev[k]);
+    delete s;
+    c->adj = nullptr;
+}
+}  // namespace picx
+
+extern "C" {
+
+long pic_particle_id_count(pic_ctx *c) { return c ? c->n_ids : -1; }
+
+int pic_get_particle_ids(pic_ctx *c, int *id, long cap, long *n_out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!id) return PIC_ERR_INVALID;
+    if (c->fp32 || !c->p.id) {
+        const long n = pic_particle_count(c);
+        if (n_out) *n_out = n;
+        if (n > cap) return PIC_ERR_INVALID;
+        for (long j = 0; j < n; j++) id[j] = -1;
+        return PIC_SUCCESS;
+    }
+    HIPCHK(hipSetDevice(c->dev));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const long NC = c->NC;
+    std::vector<int> cnt(NC);
+    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<long> off(NC + 1, 0);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    const long P = off[NC];
+    if (n_out) *n_out = P;
+    if (P > cap) return PIC_ERR_INVALID;
+    const size_t Pn = (size_t)std::max(P, 1L);
+    long *d_off = nullptr;
+    double *d[6] = {nullptr};
+    unsigned char *d_sp = nullptr;
+    int *d_cell = nullptr, *d_id = nullptr;
+    HIPCHK(hipMalloc(&d_off, (NC + 1) * sizeof(long)));
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&d[k], Pn * sizeof(double)));
+    HIPCHK(hipMalloc(&d_sp, Pn));
+    HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
+    HIPCHK(hipMalloc(&d_id, Pn * sizeof(int)));
+    HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
+    launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, d_id, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (P) HIPCHK(hipMemcpy(id, d_id, P * sizeof(int), hipMemcpyDeviceToHost));
+    (void)hipFree(d_off);
+    for (int k = 0; k < 6; k++) (void)hipFree(d[k]);
+    (void)hipFree(d_sp);
+    (void)hipFree(d_cell);
+    (void)hipFree(d_id);
+    return PIC_SUCCESS;
+}
+
+int pic_forward_record(pic_ctx *c, int nsteps, int ckpt_every, double *energy) {
+    if (!c || nsteps < 1 || ckpt_every < 1) return PIC_ERR_INVALID;
+    if (!c->diff) return PIC_ERR_STATE;
+    int rc = check_ready(c);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(c->dev));
+    free_adjoint(c);
+    AdjointStore *s = new AdjointStore();
+    c->adj = s;
+    s->nsteps = nsteps;
+    s->K = ckpt_every;
+    for (int k = 0; k < 3; k++) HIPCHK(hipEventCreate(&s->ev[k]));
+    const int nck = (nsteps - 1) / ckpt_every + 1;
+    s->ck.resize(nck + 1);
+    s->ckf.resize(nck + 1);
+    HIPCHK(hipEventRecord(s->ev[0], c->stream));
+    for (int t = 0; t < nsteps; t++) {
+        if (t % ckpt_every == 0) {
+            rc = save_state(c, s->ck[t / ckpt_every], s->ckf[t / ckpt_every]);
+            if (rc) return rc;
+        }
+        rc = enq_step(c);
+        if (rc) return rc;
+        rc = finish_step(c);
+        if (rc) return rc;
+        s->anomalies += c->last.dropped + c->last.duplicated;
+        if (energy) energy[t] = c->h_red[0];
+    }
+    rc = save_state(c, s->ck[nck], s->ckf[nck]);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(s->ev[1], c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev[0], s->ev[1]));
+    s->ms[0] = ms;
+    s->ms[1] = s->ms[2] = 0.0;
+    s->recorded = true;
+    return PIC_SUCCESS;
+}
+
+int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
+                pic_adjoint_io *grad) {
+    if (!c) return PIC_ERR_INVALID;
+    AdjointStore *s = c->adj;
+    if (!c->diff || !s || !s->recorded) return PIC_ERR_STATE;
+    /* a leaver taken by two cells (or by none) has no single adjoint path */
+    if (s->anomalies) return PIC_ERR_STATE;
+    HIPCHK(hipSetDevice(c->dev));
+    const Grid &g = c->g;
+    const long G = g.G;
+    const long NI = std::max(c->n_ids, 1L);
+    hipStream_t st = c->stream;
+    int rc;
+
+    free_lam(s);
+    for (int k = 0; k < 3; k++) {
+        HIPCHK(hipMalloc(&s->lam.E[k], G * sizeof(double)));
+        HIPCHK(hipMalloc(&s->lam.H[k], G * sizeof(double)));
+        HIPCHK(hipMalloc(&s->lam.J[k], G * sizeof(double)));
+        HIPCHK(hipMalloc(&s->lam.Q[k], G * sizeof(double)));
+        HIPCHK(hipMalloc(&s->Eend[k], G * sizeof(double)));
+        if ((rc = zero_async(s->lam.E[k], G * sizeof(double), st))) return rc;
+        if ((rc = zero_async(s->lam.H[k], G * sizeof(double), st))) return rc;
+    }
+    for (int k = 0; k < 6; k++) {
+        HIPCHK(hipMalloc(&s->lp.v[k], NI * sizeof(double)));
+        if ((rc = zero_async(s->lp.v[k], NI * sizeof(double), st))) return rc;
+    }
+    s->lp.n = c->n_ids;
+    if (seed) {
+        const double *ps[6] = {seed->x, seed->y, seed->z, seed->px, seed->py, seed->pz};
+        for (int k = 0; k < 6; k++)
+            if (ps[k] && c->n_ids)
+                HIPCHK(hipMemcpyAsync(s->lp.v[k], ps[k], c->n_ids * sizeof(double),
+                                      hipMemcpyHostToDevice, st));
+        for (int k = 0; k < 6; k++)
+            if (seed->field[k])
+                HIPCHK(hipMemcpyAsync(k < 3 ? s->lam.E[k] : s->lam.H[k - 3], seed->field[k],
+                                      G * sizeof(double), hipMemcpyHostToDevice, st));
+    }
+    const int K = s->K;
+    if ((int)s->rec.size() != K) {
+        s->rec.resize(K);
+        s->recf.resize(K);
+    }
+    for (int r = 0; r < K; r++) {
+        if (s->rec[r].p[0] && s->rec[r].cap != c->p.cap) free_parts(s->rec[r]);
+        if (!s->rec[r].p[0] && (rc = alloc_parts(s->rec[r], c->NC, c->p.cap, true, true)))
+            return rc;
+        if (!s->recf[r].f[0] && (rc = alloc_fields(s->recf[r], G))) return rc;
+    }
+
+    s->ms[1] = s->ms[2] = 0.0;
+    const int nseg = (s->nsteps - 1) / K + 1;
+    for (int seg = nseg - 1; seg >= 0; seg--) {
+        const int t0 = seg * K, t1 = std::min(t0 + K, s->nsteps);
+        HIPCHK(hipEventRecord(s->ev[0], st));
+        if ((rc = restore_state(c, s->ck[seg], s->ckf[seg]))) return rc;
+        for (int t = t0 + 1; t <= t1; t++)
+            if ((rc = recorded_step(c, s->rec[t - t0 - 1], s->recf[t - t0 - 1]))) return rc;
+        for (int k = 0; k < 3; k++)
+            HIPCHK(hipMemcpyAsync(s->Eend[k], c->f.f[PIC_EX + k], G * sizeof(double),
+                                  hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipEventRecord(s->ev[1], st));
+        launch_reset_status(c->d_st, st);
+        /* the recomputed segment must end on the stored state, bit for bit:
+         * fields and per-cell counts */
+        for (int k = 0; k < 6; k++)
+            launch_mismatch(c->f.f[PIC_EX + k], s->ckf[seg + 1].f[k], G, 8, c->d_st, st);
+        launch_mismatch(c->p.count, s->ck[seg + 1].count, c->NC, 4, c->d_st, st);
+        for (int t = t1; t > t0; t--) {
+            const int r = t - t0 - 1;
+            const double *Et[3];
+            for (int k = 0; k < 3; k++) Et[k] = (t == t1) ? s->Eend[k] : s->recf[r + 1].f[k];
+            if (weights && weights[t - 1] != 0.0)
+                launch_adj_seed_energy(g, Et, weights[t - 1], s->lam, st);
+            launch_adj_e_periodic(g, s->lam, st);
+            launch_adj_curl_e(g, s->lam, st);
+            launch_adj_fold_j(g, s->lam, st);
+            AdjFwd fwd;
+            for (int k = 0; k < 3; k++) {
+                fwd.E[k] = s->recf[r].f[k];
+                fwd.H[k] = s->recf[r].f[3 + k];
+            }
+            launch_adj_particles_f64(g, c->t, fwd, s->rec[r], c->sp, s->lam, s->lp, c->d_st, st);
+            launch_adj_curl_h(g, s->lam, st);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(s->ev[2], st));
+        HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (c->h_st->flags & PIC_F_RANGE) return PIC_ERR_CELL_RANGE;
+        if (c->h_st->flags & PIC_F_RECOMPUTE) {
+            snprintf(g_hip_err, sizeof(g_hip_err),
+                     "recomputed segment %d differs from its checkpoint", seg);
+            return PIC_ERR_STATE;
+        }
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, s->ev[0], s->ev[1]));
+        HIPCHK(hipEventElapsedTime(&b, s->ev[1], s->ev[2]));
+        s->ms[1] += a;
+        s->ms[2] += b;
+    }
+    if (grad) {
+        double *pg[6] = {grad->x, grad->y, grad->z, grad->px, grad->py, grad->pz};
+        for (int k = 0; k < 6; k++)
+            if (pg[k] && c->n_ids)
+                HIPCHK(hipMemcpyAsync(pg[k], s->lp.v[k], c->n_ids * sizeof(double),
+                                      hipMemcpyDeviceToHost, st));
+        for (int k = 0; k < 6; k++)
+            if (grad->field[k])
+                HIPCHK(hipMemcpyAsync(grad->field[k], k < 3 ? s->lam.E[k] : s->lam.H[k - 3],
+                                      G * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    /* leave the context at s_nsteps */
+    if ((rc = restore_state(c, s->ck.back(), s->ckf.back()))) return rc;
+    HIPCHK(hipStreamSynchronize(st));
+    return PIC_SUCCESS;
+}
+
+int pic_adjoint_timing(pic_ctx *c, double *out3) {
+    if (!c || !out3) return PIC_ERR_INVALID;
+    if (!c->adj) return PIC_ERR_STATE;
+    for (int k = 0; k < 3; k++) out3[k] = c->adj->ms[k];
+    return PIC_SUCCESS;
+}
+
+}  // extern "C"

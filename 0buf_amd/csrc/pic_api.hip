@@ -16,11 +16,9 @@
 #include <type_traits>
 #include <vector>
 
-#include "pic_internal.h"
+#include "pic_ctx.h"
 
-TileGeom make_tile_geom(int Nx, int Ny, int Nz);
-
-namespace {
+namespace picx {
 
 thread_local char g_hip_err[256] = "";
 
@@ -29,59 +27,6 @@ int record_hip(hipError_t e, const char *where) {
     return e == hipErrorOutOfMemory ? PIC_ERR_NOMEM : PIC_ERR_HIP;
 }
 
-#define HIPCHK(x)                                          \
-    do {                                                   \
-        hipError_t e_ = (x);                               \
-        if (e_ != hipSuccess) return record_hip(e_, #x);   \
-    } while (0)
-
-struct Staged {
-    std::vector<double> v[6];
-    std::vector<unsigned char> sp;
-    std::vector<int> cell;
-};
-
-}  // namespace
-
-struct pic_ctx {
-    pic_params prm;
-    Grid g;
-    TileGeom t;
-    long NC = 0;
-    int dev = 0;
-    hipStream_t stream = nullptr;
-    bool fp32 = false;
-    FieldPtrs f{};          /* FP64 plain form */
-    PartPtrs p{};
-    QueuePtrs q{};
-    FieldPtrs32 f32{};      /* FP32 precision form */
-    PartPtrs32 p32{};
-    QueuePtrs32 q32{};
-    long long *d_rho = nullptr;   /* Gauss residual scratch */
-    double *d_gres = nullptr;
-    std::vector<double> gres_ref;
-    SpeciesTable sp{};
-    int slot_sort[PIC_MAX_SLOTS] = {0};
-    int nslots = 0;
-    int *d_slot_sort = nullptr;
-    DevStatus *d_st = nullptr;
-    DevStatus *h_st = nullptr;
-    double *d_part = nullptr;
-    int npart = 1024;
-    double *d_red = nullptr;
-    double *h_red = nullptr;
-    Staged staged;
-    bool committed = false;
-    int regrows = 0;
-    pic_step_status last{};
-    /* stage timing */
-    bool prof = false;
-    hipEvent_t ev[PIC_NSTAGES + 1] = {};
-    double prof_ms[PIC_NSTAGES] = {0};
-    long prof_n[PIC_NSTAGES] = {0};
-};
-
-namespace {
 
 template <class P>
 void free_parts(P &p) {
@@ -89,11 +34,14 @@ void free_parts(P &p) {
         if (p.p[k]) (void)hipFree(p.p[k]);
     if (p.sp) (void)hipFree(p.sp);
     if (p.count) (void)hipFree(p.count);
+    if constexpr (std::is_same<P, PartPtrs>::value)
+        if (p.id) (void)hipFree(p.id);
     p = P{};
 }
 
+/* buckets of ncell x cap slots; ids only for the differentiable form */
 template <class P>
-int alloc_parts(P &p, long ncell, int cap, bool with_count) {
+int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(p.p[0])>>;
     p = P{};
     p.cap = cap;
@@ -101,8 +49,13 @@ int alloc_parts(P &p, long ncell, int cap, bool with_count) {
     for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&p.p[k], n * sizeof(T)));
     HIPCHK(hipMalloc(&p.sp, n));
     if (with_count) HIPCHK(hipMalloc(&p.count, ncell * sizeof(int)));
+    if constexpr (std::is_same<P, PartPtrs>::value)
+        if (with_id) HIPCHK(hipMalloc(&p.id, n * sizeof(int)));
     return PIC_SUCCESS;
 }
+
+template int alloc_parts<PartPtrs>(PartPtrs &, long, int, bool, bool);
+template void free_parts<PartPtrs>(PartPtrs &);
 
 template <class Q>
 void free_queue(Q &q) {
@@ -112,11 +65,13 @@ void free_queue(Q &q) {
     if (q.dest) (void)hipFree(q.dest);
     if (q.slot) (void)hipFree(q.slot);
     if (q.count) (void)hipFree(q.count);
+    if constexpr (std::is_same<Q, QueuePtrs>::value)
+        if (q.id) (void)hipFree(q.id);
     q = Q{};
 }
 
 template <class Q>
-int alloc_queue(Q &q, const TileGeom &t, int qcap) {
+int alloc_queue(Q &q, const TileGeom &t, int qcap, bool with_id) {
     using T = std::remove_pointer_t<std::remove_reference_t<decltype(q.q[0])>>;
     q = Q{};
     q.qcap = qcap;
@@ -127,6 +82,8 @@ int alloc_queue(Q &q, const TileGeom &t, int qcap) {
     HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
     HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
     HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
+    if constexpr (std::is_same<Q, QueuePtrs>::value)
+        if (with_id) HIPCHK(hipMalloc(&q.id, n * sizeof(int)));
     return PIC_SUCCESS;
 }
 
@@ -150,6 +107,7 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
     const long P_ = (long)c->staged.cell.size();
     std::vector<T> sorted[6];
     std::vector<unsigned char> ssp(P_);
+    std::vector<int> sid(c->diff ? P_ : 0);
     for (int k = 0; k < 6; k++) sorted[k].resize(P_);
     {
         const Grid &g = c->g;
@@ -173,11 +131,12 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
                 for (int k = 0; k < 6; k++) sorted[k][d] = (T)c->staged.v[k][j];
             }
             ssp[d] = c->staged.sp[j];
+            if (c->diff) sid[d] = c->staged.id[j];
         }
     }
-    int rc = alloc_parts(parts, NC, cap, true);
+    int rc = alloc_parts(parts, NC, cap, true, c->diff);
     if (rc) return rc;
-    rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap));
+    rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
     long *d_off = nullptr;
     T *d_src[6] = {nullptr};
@@ -197,8 +156,18 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
         launch_scatter_load32(d_off, NC, src, d_sp, parts, c->stream);
     else
         launch_scatter_load(d_off, NC, src, d_sp, parts, c->stream);
+    int *d_id = nullptr;
+    if constexpr (std::is_same<P, PartPtrs>::value) {
+        if (c->diff) {
+            HIPCHK(hipMalloc(&d_id, Pn * sizeof(int)));
+            if (P_ > 0)
+                HIPCHK(hipMemcpy(d_id, sid.data(), P_ * sizeof(int), hipMemcpyHostToDevice));
+            launch_scatter_ids(d_off, NC, d_id, parts, c->stream);
+        }
+    }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (d_id) (void)hipFree(d_id);
     (void)hipFree(d_off);
     for (int k = 0; k < 6; k++) (void)hipFree(d_src[k]);
     (void)hipFree(d_sp);
@@ -238,7 +207,7 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     const int oldcap = parts.cap;
     const int cap = std::max(auto_capacity(maxc), oldcap + 8);
     P np;
-    int rc = alloc_parts(np, NC, cap, false);
+    int rc = alloc_parts(np, NC, cap, false, c->diff);
     if (rc) return rc;
     np.count = parts.count;
     if constexpr (std::is_same<P, PartPtrs32>::value) {
@@ -254,7 +223,7 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     free_parts(parts);
     parts = np;
     Q nq;
-    rc = alloc_queue(nq, c->t, queue_capacity(c->t, cap));
+    rc = alloc_queue(nq, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
     free_queue(queue);
     queue = nq;
@@ -278,12 +247,20 @@ void enq_h_half(pic_ctx *c) {
     else launch_h_half(c->g, c->f, c->stream);
 }
 void enq_push(pic_ctx *c) {
-    if (c->fp32)
+    if (c->fp32) {
         launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->q32, c->sp, c->prm.departure_cap,
                                 c->d_st, c->stream);
-    else
+    } else if (c->diff) {
+        DiffArgs d{};
+        for (int k = 0; k < 3; k++) d.jfix[k] = c->d_jfix[k];
+        d.snap = c->rec_snap;
         launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap,
-                                c->d_st, c->stream);
+                                c->d_st, &d, c->stream);
+        launch_jfix_to_f64(c->g, c->f, d, c->stream);
+    } else {
+        launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap,
+                                c->d_st, nullptr, c->stream);
+    }
 }
 void enq_arrive(pic_ctx *c) {
     if (c->fp32) launch_arrive_f32(c->t, c->p32, c->q32, c->d_st, c->stream);
@@ -331,6 +308,32 @@ void accumulate_profile(pic_ctx *c) {
     }
 }
 
+/* one whole step enqueued on the context stream (Plasma::Step,
+ * gpu_plasma.h:605-620); the energy lands in h_red[0] after the sync */
+int enq_step(pic_ctx *c) {
+    launch_reset_status(c->d_st, c->stream);
+    mark(c, PIC_STAGE_H_HALF);
+    enq_h_half(c);
+    mark(c, PIC_STAGE_PUSH_DEPOSIT);
+    enq_push(c);
+    mark(c, PIC_STAGE_MIGRATE);
+    enq_arrive(c);
+    mark(c, PIC_STAGE_FOLD_J);
+    enq_fold(c);
+    mark(c, PIC_STAGE_H2);
+    enq_h2(c);
+    mark(c, PIC_STAGE_E);
+    enq_e(c);
+    mark(c, PIC_STAGE_E_PERIODIC);
+    enq_e_periodic(c);
+    mark(c, PIC_STAGE_ENERGY);
+    enq_energy(c);
+    mark(c, PIC_NSTAGES);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    return PIC_SUCCESS;
+}
+
 int finish_step(pic_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -351,7 +354,9 @@ int finish_step(pic_ctx *c) {
     return PIC_SUCCESS;
 }
 
-}  // namespace
+}  // namespace picx
+
+using namespace picx;
 
 extern "C" {
 
@@ -415,6 +420,9 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return PIC_ERR_INVALID;
     if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
     if (prm->precision != PIC_FP64 && prm->precision != PIC_FP32) return PIC_ERR_INVALID;
+    /* the differentiable form runs on the FP64 step */
+    if (prm->differentiable && prm->precision != PIC_FP64) return PIC_ERR_INVALID;
+    if (prm->ghost_mode != 0 && prm->ghost_mode != 1) return PIC_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
     if (prm->device < 0 || prm->device >= ndev) return PIC_ERR_INVALID;
@@ -433,6 +441,7 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     g.hy = prm->ly / (double)prm->ny;
     g.hz = prm->lz / (double)prm->nz;
     g.tau = prm->tau;
+    g.ghost_mode = prm->ghost_mode;
     c->NC = (long)g.Nx * g.Ny * g.Nz;
     c->t = make_tile_geom(g.Nx, g.Ny, g.Nz);
     int rc = PIC_SUCCESS;
@@ -461,6 +470,13 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
         e = hipMemset(*ptr, 0, g.G * esz);
         if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
     }
+    c->diff = prm->differentiable != 0;
+    for (int k = 0; c->diff && k < 3; k++) {
+        if ((e = hipMalloc(&c->d_jfix[k], g.G * sizeof(long long))) != hipSuccess)
+            return fail(record_hip(e, "hipMalloc jfix"));
+        if ((e = hipMemset(c->d_jfix[k], 0, g.G * sizeof(long long))) != hipSuccess)
+            return fail(record_hip(e, "hipMemset jfix"));
+    }
     if ((e = hipMalloc(&c->d_st, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
     if ((e = hipMemset(c->d_st, 0, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
     if ((e = hipHostMalloc(&c->h_st, sizeof(DevStatus))) != hipSuccess) return fail(record_hip(e, "status"));
@@ -487,6 +503,9 @@ int pic_destroy(pic_ctx *c) {
         if (c->f32.Q[k]) (void)hipFree(c->f32.Q[k]);
         if (c->f32.J[k]) (void)hipFree(c->f32.J[k]);
     }
+    free_adjoint(c);
+    for (int k = 0; k < 3; k++)
+        if (c->d_jfix[k]) (void)hipFree(c->d_jfix[k]);
     free_parts(c->p);
     free_queue(c->q);
     free_parts(c->p32);
@@ -503,6 +522,12 @@ int pic_destroy(pic_ctx *c) {
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+    return PIC_SUCCESS;
+}
+
+int pic_get_params(pic_ctx *c, pic_params *out) {
+    if (!c || !out) return PIC_ERR_INVALID;
+    *out = c->prm;
     return PIC_SUCCESS;
 }
 
@@ -529,6 +554,7 @@ int pic_load_particles(pic_ctx *c, int sort, const double *x, const double *y, c
     for (int k = 0; k < 6; k++) S.v[k].reserve(S.v[k].size() + n);
     S.sp.reserve(S.sp.size() + n);
     S.cell.reserve(S.cell.size() + n);
+    if (c->diff) S.id.reserve(S.id.size() + n);
     for (long j = 0; j < n; j++) {
         /* getPointCell (cell.h:593-598): (int)((x - 0.0)/h) per axis */
         int i = (int)((x[j] - 0.0) / g.hx);
@@ -553,7 +579,9 @@ int pic_load_particles(pic_ctx *c, int sort, const double *x, const double *y, c
         S.v[5].push_back(pz[j]);
         S.sp.push_back((unsigned char)slot);
         S.cell.push_back((int)(((long)i * g.Ny + l) * g.Nz + k));
+        if (c->diff) S.id.push_back((int)(c->n_ids + j));
     }
+    c->n_ids += n;
     if (n_dropped) *n_dropped = dropped;
     return PIC_SUCCESS;
 }
@@ -675,26 +703,8 @@ int pic_step(pic_ctx *c, int nt, double *energy) {
     int rc = check_ready(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->dev));
-    launch_reset_status(c->d_st, c->stream);
-    mark(c, PIC_STAGE_H_HALF);
-    enq_h_half(c);
-    mark(c, PIC_STAGE_PUSH_DEPOSIT);
-    enq_push(c);
-    mark(c, PIC_STAGE_MIGRATE);
-    enq_arrive(c);
-    mark(c, PIC_STAGE_FOLD_J);
-    enq_fold(c);
-    mark(c, PIC_STAGE_H2);
-    enq_h2(c);
-    mark(c, PIC_STAGE_E);
-    enq_e(c);
-    mark(c, PIC_STAGE_E_PERIODIC);
-    enq_e_periodic(c);
-    mark(c, PIC_STAGE_ENERGY);
-    enq_energy(c);
-    mark(c, PIC_NSTAGES);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    rc = enq_step(c);
+    if (rc) return rc;
     rc = finish_step(c);
     accumulate_profile(c);
     if (energy) *energy = c->h_red[0];
@@ -790,7 +800,7 @@ int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, d
     HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
     HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
     if (c->fp32) launch_gather_particles32(c->g, NC, d_off, c->p32, d, d_sp, d_cell, c->stream);
-    else launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, c->stream);
+    else launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, nullptr, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     double *outs[6] = {x, y, z, px, py, pz};

@@ -1,0 +1,86 @@
+/*
+ * pic_ctx.h -- the object behind the opaque pic_ctx handle, shared by the
+ * host files pic_api.hip (plain and precision forms) and pic_adjoint.hip
+ * (differentiable form).  Not part of the ABI.
+ */
+#ifndef PIC_CTX_H
+#define PIC_CTX_H
+
+#include <vector>
+
+#include "pic_internal.h"
+
+#define HIPCHK(x)                                          \
+    do {                                                   \
+        hipError_t e_ = (x);                               \
+        if (e_ != hipSuccess) return picx::record_hip(e_, #x);   \
+    } while (0)
+
+struct Staged {
+    std::vector<double> v[6];
+    std::vector<unsigned char> sp;
+    std::vector<int> cell;
+    std::vector<int> id;     /* differentiable form: load-order id */
+};
+
+struct AdjointStore;   /* pic_adjoint.hip */
+
+struct pic_ctx {
+    pic_params prm;
+    Grid g;
+    TileGeom t;
+    long NC = 0;
+    int dev = 0;
+    hipStream_t stream = nullptr;
+    bool fp32 = false;
+    FieldPtrs f{};          /* FP64 plain form */
+    PartPtrs p{};
+    QueuePtrs q{};
+    FieldPtrs32 f32{};      /* FP32 precision form */
+    PartPtrs32 p32{};
+    QueuePtrs32 q32{};
+    long long *d_rho = nullptr;   /* Gauss residual scratch */
+    double *d_gres = nullptr;
+    std::vector<double> gres_ref;
+    /* differentiable form: particle ids, fixed-point current, adjoint state */
+    bool diff = false;
+    long long *d_jfix[3] = {nullptr, nullptr, nullptr};
+    long n_ids = 0;
+    AdjointStore *adj = nullptr;
+    PartPtrs rec_snap{};      /* record target of the next push (adjoint recompute) */
+    SpeciesTable sp{};
+    int slot_sort[PIC_MAX_SLOTS] = {0};
+    int nslots = 0;
+    int *d_slot_sort = nullptr;
+    DevStatus *d_st = nullptr;
+    DevStatus *h_st = nullptr;
+    double *d_part = nullptr;
+    int npart = 1024;
+    double *d_red = nullptr;
+    double *h_red = nullptr;
+    Staged staged;
+    bool committed = false;
+    int regrows = 0;
+    pic_step_status last{};
+    /* stage timing */
+    bool prof = false;
+    hipEvent_t ev[PIC_NSTAGES + 1] = {};
+    double prof_ms[PIC_NSTAGES] = {0};
+    long prof_n[PIC_NSTAGES] = {0};
+};
+
+namespace picx {
+extern thread_local char g_hip_err[256];
+int record_hip(hipError_t e, const char *where);
+int commit(pic_ctx *c);
+int check_ready(pic_ctx *c);
+int enq_step(pic_ctx *c);
+int finish_step(pic_ctx *c);
+template <class P>
+int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id);
+template <class P>
+void free_parts(P &p);
+void free_adjoint(pic_ctx *c);   /* pic_adjoint.hip */
+}  // namespace picx
+
+#endif

@@ -27,7 +27,8 @@ enum {
     PIC_F_TOO_MANY = 1,
     PIC_F_QUEUE_OVERFLOW = 2,
     PIC_F_BUCKET_OVERFLOW = 4,
-    PIC_F_RANGE = 8
+    PIC_F_RANGE = 8,
+    PIC_F_RECOMPUTE = 16     /* differentiable form: recompute differs from the first run */
 };
 
 struct DevStatus {
@@ -44,6 +45,7 @@ struct Grid {
     int sy, sx;          /* ghosted strides: sy = Nz+2, sx = (Ny+2)(Nz+2) */
     long G;              /* ghosted nodes */
     double Lx, Ly, Lz, hx, hy, hz, tau;
+    int ghost_mode;      /* pic_params.ghost_mode */
 };
 
 struct FieldPtrs {
@@ -66,6 +68,7 @@ struct PartPtrs {
     unsigned char *sp;       /* species slot */
     int *count;              /* per bulk cell */
     int cap;
+    int *id;                 /* differentiable form: load-order particle id, else null */
 };
 
 /* particle storage of the FP32 form: x, y, z are offsets in [0,1) inside the
@@ -93,6 +96,7 @@ struct QueuePtrs {
     int *slot;               /* slot claimed by the arrival kernel */
     int *count;              /* per tile */
     int qcap;                /* entries per tile */
+    int *id;                 /* differentiable form only */
 };
 
 struct TileGeom {
@@ -124,18 +128,32 @@ void launch_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks
                      hipStream_t s);
 TileGeom make_tile_geom(int Nx, int Ny, int Nz);
 
+/* Differentiable form of the FP64 step: the push deposits J as int64 fixed
+ * point with resolution 2^-48 (order-free, so a recomputed step is bitwise
+ * equal to the first run), converted to the FP64 J arrays before the fold;
+ * when snap.p[0] is set the push also writes the pre-step particle state into
+ * the record buckets `snap` (same capacity as the live buckets). */
+#define PIC_J_SCALE64 281474976710656.0       /* 2^48 */
+struct DiffArgs {
+    long long *jfix[3];
+    PartPtrs snap;
+};
+
 /* particle launchers (k_particles_f64.hip) */
 size_t push_lds_bytes(const TileGeom &t, int cap);
 void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             hipStream_t s);
+                             const DiffArgs *diff, hipStream_t s);
+void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s);
 void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s);
 void launch_scatter_load(const long *offsets, long ncell, const double *src[6],
                          const unsigned char *srcsp, PartPtrs p, hipStream_t s);
 void launch_repack(long ncell, PartPtrs from, PartPtrs to, hipStream_t s);
 void launch_fix_arrivals(const TileGeom &t, QueuePtrs q, int oldcap, PartPtrs to, hipStream_t s);
 void launch_gather_particles(long ncell, const long *offsets, PartPtrs p, double *dst[6],
-                             unsigned char *dsp, int *dcell, hipStream_t s);
+                             unsigned char *dsp, int *dcell, int *did, hipStream_t s);
+void launch_scatter_ids(const long *offsets, long ncell, const int *src, PartPtrs p,
+                        hipStream_t s);
 void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort,
                     double *partials, int nblocks, double *out27, hipStream_t s);
 
@@ -167,5 +185,32 @@ void launch_gauss64(const Grid &g, FieldPtrs f, const long long *rho, double *re
                     hipStream_t s);
 void launch_gauss32(const Grid &g, FieldPtrs32 f, const long long *rho, double *res,
                     hipStream_t s);
+
+/* Differentiable form, adjoint stages (k_adjoint.hip).  Adjoint arrays
+ * have the field layout (G entries); the particle adjoints are indexed by
+ * particle id. */
+struct AdjLam {
+    double *E[3], *H[3], *J[3], *Q[3];
+};
+struct AdjFwd {             /* forward fields the particle stage read */
+    const double *E[3];     /* E^n */
+    const double *H[3];     /* H^{n+1/2} = H^n + Q */
+};
+struct LamParts {
+    double *v[6];           /* d/dx, d/dy, d/dz, d/dpu, d/dpv, d/dpw */
+    long n;
+};
+void launch_adj_seed_energy(const Grid &g, const double *const E[3], double w, AdjLam l,
+                            hipStream_t s);
+void launch_adj_e_periodic(const Grid &g, AdjLam l, hipStream_t s);
+void launch_adj_curl_e(const Grid &g, AdjLam l, hipStream_t s);
+void launch_adj_fold_j(const Grid &g, AdjLam l, hipStream_t s);
+void launch_adj_particles_f64(const Grid &g, const TileGeom &t, AdjFwd fwd, PartPtrs snap,
+                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
+                              hipStream_t s);
+void launch_adj_curl_h(const Grid &g, AdjLam l, hipStream_t s);
+void launch_copy_parts(long ncell, PartPtrs from, PartPtrs to, hipStream_t s);
+void launch_mismatch(const void *a, const void *b, long n, int elem_bytes, DevStatus *st,
+                     hipStream_t s);
 
 #endif

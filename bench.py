@@ -13,6 +13,11 @@ HBM before the timed region.  Prints ONE JSON line:
                  the stream it runs on, against the 8 TB/s HBM peak
   cpu_baseline   the FP64 CPU restatement (oracle/) timed on this host on a
                  bounded slab of the same workload
+  fp32           the FP32 precision form on the same particles: pushes/s,
+                 speed-up, energy-history error against the FP64 run, roofline
+  adjoint        config 3 gradient (differentiable form): wall time of
+                 forward-with-checkpoints + recompute + adjoint sweep over a
+                 plain forward run
 """
 import argparse
 import importlib
@@ -69,6 +74,74 @@ def cpu_baseline(pic, cfg, slab, steps=2, threads=1):
                       f"{dt:.1f} s, single thread"}
 
 
+def run_form(pic, params, parts, bx0, precision, steps, warmup):
+    """Load the workload into a context of the given precision, run warmup
+    untimed steps then `steps` timed steps (torch.cuda.synchronize on both
+    sides); per-stage mean launch times from HIP events on the context's
+    stream."""
+    import torch
+    params.precision = precision
+    g = pic.Pic(params)
+    for (sort, x, y, z, px, py, pz, m, qm) in parts:
+        g.load_particles(sort, x, y, z, px, py, pz, m, qm)
+    if bx0:
+        g.load_field(pic.HX, np.full(g.G, bx0))
+    P = g.particle_count()
+    torch.cuda.synchronize()
+    energies = []
+    for nt in range(1, warmup + 1):
+        energies.append(g.step(nt))
+    g.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for nt in range(warmup + 1, warmup + steps + 1):
+        energies.append(g.step(nt))
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    res = {"P": P, "ncell": g.ncell, "dt": dt, "prof": g.profile_read(), "status": g.status(),
+           "P_end": g.particle_count(), "energies": np.array(energies)}
+    g.close()
+    return res
+
+
+def push_roofline(res, width):
+    prof = res["prof"]
+    push_ms = prof["push_deposit"][0] / max(prof["push_deposit"][1], 1)
+    alg = algorithmic_bytes_push(res["P"], res["ncell"], width)
+    achieved = alg / (push_ms * 1e-3) / 1e9
+    return push_ms, alg, achieved
+
+
+def adjoint_ratio(pic):
+    """Config 3 (64x16x16 cells, 32 ppc, 200 steps, checkpoints every 20):
+    wall time of a gradient (forward with checkpoints + recompute + adjoint
+    sweep) against a plain forward run of the same form."""
+    import torch
+    ag = importlib.import_module("0buf_amd.autograd")
+    bp = ag.BeamPlasma(3)
+    theta = torch.tensor([0.3] + [1e-3] * 8, dtype=torch.float64, requires_grad=True)
+    bp.loss_value(theta.detach())                    # warm-up
+    t0 = time.perf_counter()
+    bp.loss_value(theta.detach())
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    L = bp.loss(theta)
+    t1 = time.perf_counter()
+    L.backward()
+    t2 = time.perf_counter()
+    dev = bp.run.last.adjoint_timing()
+    bp.run.last.close()
+    return {"workload": "config3: beam-plasma 64x16x16 cells, 32 ppc (ions 8, electrons 16, "
+                        "beam 8), 200 steps, FP64 differentiable form, checkpoint every 20 "
+                        "steps, loss = sum of field energy over steps 150..200, parameters "
+                        "= beam drift + 8 displacement amplitudes",
+            "forward_s": t_fwd, "gradient_s": t2 - t0, "ratio": (t2 - t0) / t_fwd,
+            "forward_record_s": t1 - t0, "backward_s": t2 - t1,
+            "device_ms": {"forward_with_checkpoints": dev[0], "recompute": dev[1],
+                          "adjoint_steps": dev[2]},
+            "grad_norm": float(theta.grad.norm())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,14 +149,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-slab", type=int, default=8)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run a plain elementwise kernel (dst = src + 1, FP32) that reads "
+                         "and writes BYTES each, 5 times: the known-size reference for the "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.add(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": n * 4, "bytes_written": n * 4,
+                          "launches": 5}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the step would shard as x-slabs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     pic = importlib.import_module("0buf_amd")
     cfg = pic.beam_plasma_config(args.config)
     ppc = tuple(s.ppc for s in cfg["species"])
@@ -91,37 +180,20 @@ def main():
     t0 = time.perf_counter()
     params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
     log(f"generated {sum(len(p[1]) for p in parts)} particles in {time.perf_counter() - t0:.1f} s")
-    g = pic.Pic(params)
-    for (sort, x, y, z, px, py, pz, m, qm) in parts:
-        g.load_particles(sort, x, y, z, px, py, pz, m, qm)
-    del parts
-    if cfg["bx0"]:
-        g.load_field(pic.HX, np.full(g.G, cfg["bx0"]))
-    P = g.particle_count()
-    ncell = g.ncell
-    log(f"loaded {P} particles into {ncell} cells")
-    torch.cuda.synchronize()
-    for nt in range(1, args.warmup + 1):
-        g.step(nt)
-    log(f"warm-up done ({args.warmup} steps)")
-    g.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e = 0.0
-    for nt in range(args.warmup + 1, args.warmup + args.steps + 1):
-        e = g.step(nt)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    prof = g.profile_read()
-    st = g.status()
-    P_end = g.particle_count()
-    log(f"timed {args.steps} steps in {dt:.3f} s; energy {e:.6e}; particles {P_end}")
 
-    ms_step = dt * 1e3 / args.steps
-    pushes = P * args.steps / dt
-    push_ms = prof["push_deposit"][0] / max(prof["push_deposit"][1], 1)
-    alg = algorithmic_bytes_push(P, ncell, 8)
-    achieved = alg / (push_ms * 1e-3) / 1e9
+    r64 = run_form(pic, params, parts, cfg["bx0"], pic.FP64, args.steps, args.warmup)
+    log(f"FP64: {r64['P']} particles, {args.steps} steps in {r64['dt']:.3f} s")
+    r32 = None
+    if not args.no_fp32:
+        r32 = run_form(pic, params, parts, cfg["bx0"], pic.FP32, args.steps, args.warmup)
+        log(f"FP32: {args.steps} steps in {r32['dt']:.3f} s")
+    del parts
+
+    P, ncell = r64["P"], r64["ncell"]
+    ms_step = r64["dt"] * 1e3 / args.steps
+    pushes = P * args.steps / r64["dt"]
+    push_ms, alg, achieved = push_roofline(r64, 8)
+    prof = r64["prof"]
     yee_ms = sum(prof[k][0] for k in ("h_half", "h2", "e", "e_periodic", "fold_j")) / args.steps
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()}
 
@@ -133,6 +205,31 @@ def main():
         except Exception:
             traffic = None
 
+    fp32 = None
+    if r32 is not None:
+        p32_ms, alg32, ach32 = push_roofline(r32, 4)
+        rel = np.abs(r32["energies"] - r64["energies"]) / np.abs(r64["energies"])
+        yee32 = sum(r32["prof"][k][0] for k in ("h_half", "h2", "e", "e_periodic",
+                                                "fold_j")) / args.steps
+        fp32 = {"value": P * args.steps / r32["dt"], "unit": "particle pushes/s",
+                "ms_per_step": r32["dt"] * 1e3 / args.steps,
+                "speedup_vs_fp64": r64["dt"] / r32["dt"],
+                "energy_history_max_rel_err_vs_fp64": float(rel.max()),
+                "energy_history_steps_compared": int(len(rel)),
+                "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
+                             "kernel": "k_push_deposit32", "algorithmic_bytes_per_launch": alg32,
+                             "mean_launch_ms": p32_ms},
+                "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in r32["prof"].items()},
+                "yee_cell_updates_per_s": ncell / (yee32 * 1e-3) if yee32 > 0 else None,
+                "status": r32["status"]}
+
+    adj = None
+    if not args.no_adjoint:
+        log("timing the differentiable form (config 3 gradient)")
+        adj = adjoint_ratio(pic)
+        log(f"adjoint/forward ratio {adj['ratio']:.2f}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a slab of the workload")
@@ -140,7 +237,8 @@ def main():
         log(f"cpu baseline {cpu['value']:.3e} pushes/s")
 
     out = {
-        "metric": "particle pushes/s/GPU (FP64 plain form, config 1)",
+        "metric": "particle pushes/s/GPU (FP64 plain form, config 1; FP32 form and "
+                  "adjoint/forward ratio alongside)",
         "value": pushes,
         "unit": "particle pushes/s",
         "n_gpus": 1,
@@ -159,12 +257,13 @@ def main():
                      "kernel": "k_push_deposit (gather+Boris+VB deposit+departures)",
                      "algorithmic_bytes_per_launch": alg, "mean_launch_ms": push_ms},
         "cpu_baseline": cpu,
+        "fp32": fp32,
+        "adjoint": adj,
         "stage_ms": stage_ms,
         "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,
-        "particles_end": P_end,
-        "status": st,
+        "particles_end": r64["P_end"],
+        "status": r64["status"],
     }
-    g.close()
     print(json.dumps(out))
 
 

@@ -98,6 +98,17 @@ typedef struct pic_params {
                                 <= 0 disables the check */
     int bucket_capacity;     /* particle slots per cell; 0 = automatic */
     int device;              /* HIP device ordinal */
+    int differentiable;      /* 1: differentiable form (FP64 only): particles carry
+                                their load-order id, J is accumulated in int64
+                                fixed point (2^-48, order-free, so a recomputed
+                                step is bitwise the first run) and
+                                pic_forward_record / pic_adjoint are enabled */
+    int ghost_mode;          /* 0: the reference's periodic E ghosts (only Ex along z,
+                                Ey along x and z, Ez along x: gpu_plasma.h:486-505);
+                                1: additionally Ex and Ez along y, so every ghost
+                                node a particle interpolates from is a periodic
+                                image and the field seen across the y faces is
+                                continuous */
 } pic_params;
 
 typedef struct pic_ctx pic_ctx;
@@ -190,6 +201,81 @@ int pic_gauss_residual(pic_ctx *ctx, double *res, double *max_drift, int set_ref
 /* per sort (3) x 9 moments, reduced in FP64: N, sum x, y, z, sum px, py, pz,
  * sum (m/q_m)(gamma-1), sum |px|+|py|+|pz| */
 int pic_moments(pic_ctx *ctx, double *out27);
+
+/* the parameters the context was created with */
+int pic_get_params(pic_ctx *ctx, pic_params *out);
+
+/* ------------------------------------------------------------------------
+ * State dumps in the reference's format (mumu<num:3><nt:8>.dat, Fortran
+ * unformatted records; readControlPoint gpu_plasma.h:1088-1125,
+ * load_data.cu:35-343): 12 field records Ex..Qz of (nx+2)(ny+2)(nz+2)
+ * doubles, then per sort ION, PLASMA_ELECTRON, BEAM_ELECTRON a record
+ * (count, q_m, m) and six records x, y, z, px, py, pz.
+ * ---------------------------------------------------------------------- */
+typedef struct pic_dump_sort {
+    long n;                  /* particles (read: filled in) */
+    double q_m, m;
+    double *x, *y, *z, *px, *py, *pz;
+    long cap;                /* read: capacity of the six arrays */
+} pic_dump_sort;
+
+/* "mumu%03d%08d.dat" (gpu_plasma.h:1101) */
+int pic_dump_file_name(int num, int nt, char *buf, size_t len);
+/* host only; a null field pointer writes zeros / skips on read.  pic_dump_read
+ * returns PIC_ERR_NOMEM (counts filled in) when a sort exceeds its cap. */
+int pic_dump_write(const char *path, long G, const double *const fields[12],
+                   const pic_dump_sort sorts[3]);
+int pic_dump_read(const char *path, long G, double *const fields[12], pic_dump_sort sorts[3]);
+/* a context's fields and particles to a dump / a dump into an uncommitted
+ * context (LoadParticleData, load_data.cu:320-343) */
+int pic_save_state_dump(pic_ctx *ctx, const char *path);
+int pic_load_state_dump(pic_ctx *ctx, const char *path, long *n_dropped);
+/* checkControlPoint (gpu_plasma.h:1176-1276): out13[0..11] the RMS
+ * difference (CheckArraySilent, service_functions.cu:108-126) of Ex..Qz
+ * against the dump, out13[12] the RMS difference of the particles (each sort
+ * ordered by position; HUGE_VAL if a sort's count differs); report_path, if
+ * set, gets the control_points.dat line appended */
+int pic_control_point(pic_ctx *ctx, const char *path, int nt, int num, double *out13,
+                      const char *report_path);
+
+/* ------------------------------------------------------------------------
+ * Differentiable form.  No reference counterpart: the reference has no
+ * adjoint.  The state after step t is s_t = (E, H, particles); a run of
+ * nsteps from s_0 gives the energy history W_t = sum over all ghosted nodes
+ * of Ex^2 + Ey^2 + Ez^2 (the reference's Diagnose, gpu_plasma.h:621-686).
+ * pic_adjoint returns the vector-Jacobian product
+ *     d/ds_0 [ sum_t w_t W_t + <seed, s_nsteps> ]
+ * through hand-written adjoint kernels; the cell index of a particle (every
+ * floor, crossing code and periodic wrap) is held fixed.
+ * ---------------------------------------------------------------------- */
+
+/* host arrays for particle adjoints (indexed by particle id: the position in
+ * load order over all pic_load_particles calls, dropped particles included)
+ * and field adjoints (ghosted layout, G entries each); any pointer may be
+ * null */
+typedef struct pic_adjoint_io {
+    double *x, *y, *z, *px, *py, *pz;
+    double *field[6];        /* Ex, Ey, Ez, Hx, Hy, Hz */
+} pic_adjoint_io;
+
+/* number of particle ids handed out so far (particles offered to
+ * pic_load_particles, dropped ones included) */
+long pic_particle_id_count(pic_ctx *ctx);
+/* ids in the order pic_get_particles returns particles (-1 without ids) */
+int pic_get_particle_ids(pic_ctx *ctx, int *id, long cap, long *n_out);
+/* run nsteps from the current state, keeping checkpoints of the state every
+ * ckpt_every steps in device memory; energy[t-1] = W_t */
+int pic_forward_record(pic_ctx *ctx, int nsteps, int ckpt_every, double *energy);
+/* after pic_forward_record: recompute each checkpoint segment (recording the
+ * per-step particle and field state) and run the adjoint step by step back
+ * to s_0.  weights: nsteps doubles (w_t = weights[t-1]) or null; seed: the
+ * adjoint of s_nsteps or null; grad receives the adjoint of s_0.  The
+ * context is left at s_nsteps. */
+int pic_adjoint(pic_ctx *ctx, const double *weights, const pic_adjoint_io *seed,
+                pic_adjoint_io *grad);
+/* device time of the last pic_forward_record and pic_adjoint calls (ms):
+ * out[0] forward with checkpoints, out[1] recompute, out[2] adjoint steps */
+int pic_adjoint_timing(pic_ctx *ctx, double *out3);
 
 /* ------------------------------------------------------------------------
  * Seeded initial conditions (host only; the step never calls these).

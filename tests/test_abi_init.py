@@ -55,11 +55,37 @@ def test_archapi_constants_match_reference(picmod):
         assert re.search(rf"#define {name}\s+{val}\b", text)
 
 
-def test_params_struct_layout(picmod):
+def _c_layout(struct, fields, tmp_path):
+    """sizeof and offsetof of a header struct, from the C compiler."""
+    src = tmp_path / "layout.c"
+    body = "".join(f'printf("%zu\\n", offsetof({struct}, {f}));' for f in fields)
+    src.write_text(f"#include <stdio.h>\n#include <stddef.h>\n#include \"{HEADER}\"\n"
+                   f"int main(void){{printf(\"%zu\\n\", sizeof({struct}));{body}return 0;}}\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    vals = [int(v) for v in subprocess.run([str(exe)], stdout=subprocess.PIPE, text=True,
+                                           check=True).stdout.split()]
+    return vals[0], vals[1:]
+
+
+@pytest.mark.parametrize("struct,cls", [("pic_params", "PicParams"),
+                                        ("pic_step_status", "PicStepStatus"),
+                                        ("pic_adjoint_io", "PicAdjointIO"),
+                                        ("pic_species_spec", "PicSpeciesSpec"),
+                                        ("pic_param_file", "PicParamFile"),
+                                        ("pic_dump_sort", "PicDumpSort")])
+def test_struct_layout_matches_header(picmod, tmp_path, struct, cls):
+    ct = getattr(picmod, cls)
+    fields = [f for f, _ in ct._fields_]
+    size, offs = _c_layout(struct, fields, tmp_path)
+    assert C.sizeof(ct) == size
+    assert [getattr(ct, f).offset for f in fields] == offs
+
+
+def test_params_defaults(picmod):
     p = picmod.default_params()
     assert (p.nx, p.ny, p.nz, p.departure_cap) == (32, 8, 8, 50)
-    assert p.tau == 0.05 and p.precision == picmod.FP64
-    assert C.sizeof(picmod.PicParams) == 88
+    assert p.tau == 0.05 and p.precision == picmod.FP64 and p.differentiable == 0
 
 
 def test_host_error_strings(picmod):
