@@ -114,7 +114,7 @@ EXPORTS = [
     "pic_adjoint_timing",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
-    "pic_generate_species",
+    "pic_generate_species", "pic_generate_device",
     # parameter file (000_params.dat)
     "pic_read_param_file", "pic_write_param_file", "pic_species_from_param_file",
 ]
@@ -196,6 +196,8 @@ def load_library(path=LIB_PATH):
                                            C.POINTER(PicRng), C.c_int, C.c_ulonglong,
                                            C.c_int, dp, dp, dp, dp, dp, dp, C.c_long, lp,
                                            dp]),
+        "pic_generate_device": (C.c_int, [vp, C.POINTER(PicSpeciesSpec), C.c_int,
+                                          C.c_ulonglong, lp]),
         "pic_read_param_file": (C.c_int, [C.c_char_p, C.POINTER(PicParamFile)]),
         "pic_write_param_file": (C.c_int, [C.c_char_p, C.POINTER(PicParamFile)]),
         "pic_species_from_param_file": (C.c_int, [C.POINTER(PicParamFile), C.POINTER(PicParams),
@@ -373,6 +375,15 @@ class Pic:
         a = np.empty(27, dtype=np.float64)
         _check(self.lib.pic_moments(self._ctx, _dptr(a)), "pic_moments")
         return a.reshape(3, 9)
+
+    def generate_device(self, specs, seed=12345):
+        """Seeded species drawn on the GPU straight into the buckets (the
+        stream_mode-1 draws of generate_species)."""
+        arr = (PicSpeciesSpec * len(specs))(*specs)
+        n = C.c_long(0)
+        _check(self.lib.pic_generate_device(self._ctx, arr, len(specs), seed, C.byref(n)),
+               "pic_generate_device")
+        return n.value
 
     # -- state dumps ---------------------------------------------------------
     def params_out(self):

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "pic_ctx.h"
+#include "pic_draw.h"
 
 namespace picx {
 
@@ -583,6 +584,99 @@ int pic_load_particles(pic_ctx *c, int sort, const double *x, const double *y, c
     }
     c->n_ids += n;
     if (n_dropped) *n_dropped = dropped;
+    return PIC_SUCCESS;
+}
+
+int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
+                        unsigned long long seed, long *n_out) {
+    if (!c || !specs || nspec < 1 || nspec > PIC_MAX_SLOTS) return PIC_ERR_INVALID;
+    if (c->committed || !c->staged.cell.empty()) return PIC_ERR_STATE;
+    const pic_params &gp = c->prm;
+    const Grid &g = c->g;
+    const long NC = c->NC, NLK = (long)g.Ny * g.Nz;
+    std::vector<std::vector<int>> cnt(nspec, std::vector<int>(NLK));
+    std::vector<std::vector<long>> off(nspec, std::vector<long>(NLK + 1, 0));
+    std::vector<int> tot(NLK, 0);
+    for (int s = 0; s < nspec; s++) {
+        const pic_species_spec &sp = specs[s];
+        if (sp.sort < 0 || sp.sort > 2 || sp.q_m == 0.0) return PIC_ERR_INVALID;
+        for (int q = 0; q < 8; q++)
+            if (sp.perturb[q] != 0.0) return PIC_ERR_INVALID;   /* moves particles between cells */
+        pic_species_cell_counts(&gp, &sp, cnt[s].data());
+        for (long j = 0; j < NLK; j++) {
+            off[s][j + 1] = off[s][j] + cnt[s][j];
+            tot[j] += cnt[s][j];
+        }
+    }
+    int maxc = 0;
+    for (long j = 0; j < NLK; j++) maxc = std::max(maxc, tot[j]);
+    int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
+    if (maxc > cap) cap = auto_capacity(maxc);
+    HIPCHK(hipSetDevice(c->dev));
+    int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, false)
+                     : alloc_parts(c->p, NC, cap, true, c->diff);
+    if (rc) return rc;
+    rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), false)
+                 : alloc_queue(c->q, c->t, queue_capacity(c->t, cap), c->diff);
+    if (rc) return rc;
+    HIPCHK(hipMemsetAsync(count_ptr(c), 0, NC * sizeof(int), c->stream));
+    int *d_cnt = nullptr, *d_start = nullptr, *d_fail = nullptr;
+    long *d_off = nullptr;
+    unsigned long long *d_bad = nullptr;
+    HIPCHK(hipMalloc(&d_cnt, NLK * sizeof(int)));
+    HIPCHK(hipMalloc(&d_off, (NLK + 1) * sizeof(long)));
+    HIPCHK(hipMalloc(&d_start, NC * sizeof(int)));
+    HIPCHK(hipMalloc(&d_bad, sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&d_fail, sizeof(int)));
+    HIPCHK(hipMemsetAsync(d_bad, 0, sizeof(unsigned long long), c->stream));
+    HIPCHK(hipMemsetAsync(d_fail, 0, sizeof(int), c->stream));
+    long total = 0;
+    for (int s = 0; s < nspec; s++) {
+        const pic_species_spec &sp = specs[s];
+        /* getMassCharge (maxwell.cu:606-619): weight = q n / ppc */
+        const double m = sp.charge * sp.density / (double)(sp.ppc > 0 ? sp.ppc : 1);
+        int slot = -1;
+        for (int q = 0; q < c->nslots; q++)
+            if (c->slot_sort[q] == sp.sort && c->sp.m[q] == m && c->sp.q_m[q] == sp.q_m) slot = q;
+        if (slot < 0) {
+            if (c->nslots >= PIC_MAX_SLOTS) return PIC_ERR_INVALID;
+            slot = c->nslots++;
+            c->slot_sort[slot] = sp.sort;
+            c->sp.m[slot] = m;
+            c->sp.q_m[slot] = sp.q_m;
+        }
+        HIPCHK(hipMemcpyAsync(d_cnt, cnt[s].data(), NLK * sizeof(int), hipMemcpyHostToDevice,
+                              c->stream));
+        HIPCHK(hipMemcpyAsync(d_off, off[s].data(), (NLK + 1) * sizeof(long),
+                              hipMemcpyHostToDevice, c->stream));
+        const long plane = off[s][NLK];
+        launch_generate_species(g, sp, d_cnt, d_off, plane, seed, (unsigned char)slot,
+                                c->n_ids + total, c->fp32 ? nullptr : &c->p,
+                                c->fp32 ? &c->p32 : nullptr, d_start, d_bad, d_fail, c->stream);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));   /* the host count vectors are reused */
+        total += plane * g.Nx;
+    }
+    HIPCHK(hipMemcpy(c->d_slot_sort, c->slot_sort, sizeof(c->slot_sort), hipMemcpyHostToDevice));
+    unsigned long long bad = 0;
+    int fail = 0;
+    HIPCHK(hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&fail, d_fail, sizeof(fail), hipMemcpyDeviceToHost));
+    (void)hipFree(d_cnt);
+    (void)hipFree(d_off);
+    (void)hipFree(d_start);
+    (void)hipFree(d_bad);
+    (void)hipFree(d_fail);
+    c->n_ids += total;
+    c->committed = true;
+    c->last.bucket_capacity = cap;
+    if (n_out) *n_out = total - (long)bad;
+    if (bad || fail) {
+        snprintf(g_hip_err, sizeof(g_hip_err),
+                 "device initializer: %llu particles outside their cell, generator fail %d", bad,
+                 fail);
+        return PIC_ERR_STATE;
+    }
     return PIC_SUCCESS;
 }
 

@@ -11,6 +11,7 @@
  * weight convention of getMassCharge (maxwell.cu:606-619): m = q n / ppc.
  */
 #include "pic_amd.h"
+#include "pic_draw.h"
 
 #include <cmath>
 #include <cstdint>
@@ -29,79 +30,14 @@ void pic_rng_seed(pic_rng *r, int ix, int iy, int iz) {
 /* maxwell.cu:147-179: three multiplicative congruential generators
  * (171 mod 30269, 172 mod 30307, 170 mod 30323) by Schrage's method; the
  * uniform is the fractional part of the sum of the three ratios. */
-double pic_rng_uniform(pic_rng *r) {
-    r->ix = (r->ix - r->ix / 177 * 177) * 171 - (r->ix / 177 << 1);
-    r->iy = (r->iy - r->iy / 176 * 176) * 172 - (r->iy / 176 << 1);
-    r->iz = (r->iz - r->iz / 178 * 178) * 170 - (r->iz / 178 << 1);
-    if (r->ix < 0) r->ix += 30269;
-    if (r->iy < 0) r->iy += 30307;
-    if (r->iz < 0) r->iz += 30323;
-    double ai = (double)r->ix / 30269. + (double)r->iy / 30307. + (double)r->iz / 30323.;
-    int ii = (int)ai;
-    return ai - ii;
-}
+double pic_rng_uniform(pic_rng *r) { return wh_uniform(r); }
 
-/* maxwell.cu:77-144: d[n] is the standard-normal quantile whose two-sided
- * tail mass is 2^-n (d[1] = 0.6745 = Phi^-1(0.75)).  The interval
- * [d[n-1], d[n]) is chosen from the carried uniform store1 by repeated
- * doubling, the deviate inside it by rejection, and the sign by the
- * leftover uniform, which is carried into store1 for the next call. */
-double pic_rng_normal(pic_rng *r, double a, double b) {
-    static const double d[41] = {
-        0., .674489750196082, 1.150349380376008, 1.534120544352546, 1.862731867421652,
-        2.153874694061456, 2.417559016236505, 2.66006746861746, 2.885634912426757,
-        3.097269078198785, 3.297193345691964, 3.487104104114431, 3.668329285121323,
-        3.841930685501911, 4.008772594168585, 4.169569323349106, 4.324919040826046,
-        4.475328424654204, 4.621231001499247, 4.763001034267814, 4.900964207963193,
-        5.035405969463927, 5.166578119728753, 5.294704084854598, 5.419983174916868,
-        5.54259405780294, 5.662697617459439, 5.780439324478935, 5.895951216739571,
-        6.009353565530745, 6.120756285971941, 6.230260137989044, 6.33795775455379,
-        6.443934526538564, 6.548269367831731, 6.651035379893011, 6.752300431407015,
-        6.852127665896068, 6.95057594791675, 7.047700256664409, 7.14355203435219};
-    const double half = .5, one = 1.;
-    double u = r->store1;
-    int n;
-    for (n = 1; n <= 39; ++n) {
-        if (u > half) break;
-        u += u;
-    }
-    if (n > 39) n = 40;
-    double t = d[n - 1];
-    u = pic_rng_uniform(r);
-    double w, v;
-    for (;;) {
-        w = (d[n] - t) * u;
-        v = w * (w * half + t);
-        for (;;) {
-            u = pic_rng_uniform(r);
-            if (v <= u) goto accept;
-            v = pic_rng_uniform(r);
-            if (u > v) continue;
-            break;
-        }
-        u = (v - u) / (one - u);
-    }
-accept:
-    u = (u - v) / (one - v);
-    if (u > half) {
-        r->store1 = u + u - one;
-        return a - b * (w + t);
-    }
-    r->store1 = u + u;
-    return a + b * (w + t);
-}
+/* maxwell.cu:77-144 (table-driven normal with carried state store1) */
+double pic_rng_normal(pic_rng *r, double a, double b) { return wh_normal(r, a, b, nullptr); }
 
 }  // extern "C"
 
 namespace {
-
-/* splitmix64 finaliser: maps (seed, cell) to per-cell generator seeds */
-inline uint64_t mix64(uint64_t z) {
-    z += 0x9e3779b97f4a7c15ULL;
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-    return z ^ (z >> 31);
-}
 
 /* transverse Gaussian occupancy of cell (l,k): peak count at the axis */
 inline int cell_count(const pic_params *g, const pic_species_spec *s, int l, int k) {
@@ -112,35 +48,12 @@ inline int cell_count(const pic_params *g, const pic_species_spec *s, int l, int
     return (int)std::floor(s->ppc * f + 0.5);
 }
 
-/* one particle: x,y,z uniform in the cell, then ux,uy,uz normal around the
- * drift; momentum u/sqrt(1-u^2) (maxwell.cu:521-531); x displaced by
- * sum_k a_k sin(2 pi k x / Lx) and wrapped into [0, Lx). */
-inline void draw_particle(const pic_params *g, const pic_species_spec *s, pic_rng *r, int i,
-                          int l, int k, double hx, double hy, double hz, double *o) {
-    double x = ((double)i + pic_rng_uniform(r)) * hx;
-    double y = ((double)l + pic_rng_uniform(r)) * hy;
-    double z = ((double)k + pic_rng_uniform(r)) * hz;
-    double ux = pic_rng_normal(r, s->drift[0], s->vth[0]);
-    double uy = pic_rng_normal(r, s->drift[1], s->vth[1]);
-    double uz = pic_rng_normal(r, s->drift[2], s->vth[2]);
-    double d = 0.0;
-    for (int q = 0; q < 8; q++)
-        if (s->perturb[q] != 0.0) d += s->perturb[q] * std::sin(2.0 * 3.14159265358979323846 * (q + 1) * x / g->lx);
-    if (d != 0.0) {
-        x += d;
-        if (x >= g->lx) x -= g->lx;
-        if (x < 0.0) x += g->lx;
-    }
-    double vb0 = std::sqrt(1.0 - ux * ux - uy * uy - uz * uz);
-    o[0] = x;
-    o[1] = y;
-    o[2] = z;
-    o[3] = ux / vb0;
-    o[4] = uy / vb0;
-    o[5] = uz / vb0;
-}
-
 }  // namespace
+
+void pic_species_cell_counts(const pic_params *g, const pic_species_spec *s, int *counts_lk) {
+    for (int l = 0; l < g->ny; l++)
+        for (int k = 0; k < g->nz; k++) counts_lk[(size_t)l * g->nz + k] = cell_count(g, s, l, k);
+}
 
 extern "C" {
 
@@ -174,7 +87,7 @@ int pic_generate_species(const pic_params *g, const pic_species_spec *s, pic_rng
         int cnt = cell_count(g, s, l, k);
         double o[6];
         for (int j = 0; j < cnt; j++) {
-            draw_particle(g, s, r, i, l, k, hx, hy, hz, o);
+            draw_particle(g->lx, s, r, i, l, k, hx, hy, hz, o, nullptr);
             x[base + j] = o[0]; y[base + j] = o[1]; z[base + j] = o[2];
             px[base + j] = o[3]; py[base + j] = o[4]; pz[base + j] = o[5];
         }
@@ -195,10 +108,8 @@ int pic_generate_species(const pic_params *g, const pic_species_spec *s, pic_rng
     int nt = nthreads < 1 ? 1 : nthreads;
     auto worker = [&](int t) {
         for (long c = t; c < ncell; c += nt) {
-            uint64_t h = mix64(seed ^ mix64((uint64_t)c * 0x100000001b3ULL + (uint64_t)s->sort));
             pic_rng r;
-            pic_rng_seed(&r, 1 + (int)(h % 30268u), 1 + (int)((h >> 20) % 30306u),
-                         1 + (int)((h >> 40) % 30322u));
+            cell_stream(&r, seed, c, s->sort);
             int i = (int)(c / ((long)g->ny * g->nz));
             int l = (int)((c / g->nz) % g->ny);
             int k = (int)(c % g->nz);

@@ -174,6 +174,14 @@ void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,
                       const int *slot_sort, double *partials, int nblocks, double *out27,
                       hipStream_t s);
 
+/* device-side seeded initializer (k_init.hip): one species into the
+ * buckets (p64 or p32), appended behind what the cells already hold */
+void launch_generate_species(const Grid &g, const pic_species_spec &spec, const int *cnt_lk,
+                             const long *off_lk, long plane, unsigned long long seed,
+                             unsigned char slot, long id_base, PartPtrs *p64, PartPtrs32 *p32,
+                             int *start_of_cell, unsigned long long *misplaced, int *fail,
+                             hipStream_t s);
+
 /* Gauss-law residual div E - rho, FP64 (k_gauss.hip): rho by CIC on the
  * cell-centre lattice into an int64 fixed-point array of G entries, then
  * per bulk centre res = div E - rho; returns max |res - ref| if ref != 0 */

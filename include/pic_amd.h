@@ -318,6 +318,17 @@ int pic_generate_species(const pic_params *grid, const pic_species_spec *s,
                          double *px, double *py, double *pz, long cap,
                          long *n_out, double *m_out);
 
+/* Device-side seeded initializer: the stream_mode-1 draws of
+ * pic_generate_species for nspec species, run on the GPU (one lane per cell)
+ * and written straight into the context's cell buckets, equal bit for bit to
+ * pic_generate_species(stream_mode 1, seed) of each species followed by
+ * pic_load_particles in species order (ids included).  The context must
+ * hold no particles yet; species with displacement amplitudes are refused
+ * (a displaced particle can leave its cell).  A particle that the load-time
+ * predicates would place elsewhere fails the call with PIC_ERR_STATE. */
+int pic_generate_device(pic_ctx *ctx, const pic_species_spec *specs, int nspec,
+                        unsigned long long seed, long *n_out);
+
 /* ------------------------------------------------------------------------
  * Parameter file: the 000_params.dat format that writeParamsFile writes
  * (maxwell.cu:330-377) and readParameterFile (params.h:11-20, declared but

@@ -1,0 +1,169 @@
+"""Differentiable form on the MI355X: hand-written adjoint kernels behind
+pic_forward_record / pic_adjoint and the torch.autograd.Function harness.
+
+Pins (no reference counterpart exists: the reference has no adjoint):
+  * transposes: with no particles the step is linear in (E, H), so
+    <seed, F(v)> = <F^T seed, v> to round-off, and with energy weights the
+    loss is a quadratic form, so <grad, v> = 2 loss (Euler);
+  * vector-Jacobian products with particles against central finite
+    differences of the same forward, over 1, 4 and 16 steps, random seeds on
+    every state component (fields, positions, momenta);
+  * the checkpointed recompute is bitwise the first forward run (checked by
+    the library on every segment) and leaves the context at s_T;
+  * the config-3 loss (field energy summed over steps 150..200) against
+    central finite differences with step 1e-5 in each of the 9 parameters
+    (beam drift, 8 displacement amplitudes): within 1e-4 relative.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import load_workload
+
+pytestmark = pytest.mark.gpu
+
+PK = ("x", "y", "z", "px", "py", "pz")
+FK = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def make_ctx(picmod, n, parts, fields, ghost):
+    L = tuple(0.1 * v for v in n)
+    g = picmod.Pic(picmod.make_params(n, L, 0.05, differentiable=1, ghost_mode=ghost))
+    for (sort, x, y, z, px, py, pz, m, qm) in parts:
+        g.load_particles(sort, x, y, z, px, py, pz, m, qm)
+    for i, a in fields.items():
+        g.load_field(i, a)
+    return g
+
+
+def phi_of(g, seed, weights, W):
+    phi = float(np.dot(weights, W)) if weights is not None else 0.0
+    for i, k in enumerate(FK):
+        if k in seed:
+            phi += float(np.dot(seed[k], g.get_field(i)))
+    if any(k in seed for k in PK):
+        pr, ids = g.particles(), g.particle_ids()
+        for k in PK:
+            if k in seed:
+                phi += float(np.dot(seed[k][ids], pr[k]))
+    return phi
+
+
+def rand_parts(rng, n, N, pscale):
+    L = [0.1 * v for v in n]
+    out = []
+    for sort, qm, m in ((1, -1.0, -0.02), (0, 1 / 1836., 0.02), (2, -1.0, -0.005)):
+        x = [rng.uniform(0.001, L[d] - 0.001, N) for d in range(3)]
+        p = [rng.normal(0, pscale, N) for _ in range(3)]
+        out.append([sort, x[0], x[1], x[2], p[0], p[1], p[2], m, qm])
+    return out
+
+
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_field_transpose_exact(picmod, ghost):
+    rng = np.random.default_rng(11)
+    n = (8, 6, 5)
+    G = 10 * 8 * 7
+    v = {i: rng.normal(size=G) for i in range(6)}
+    seed = {k: rng.normal(size=G) for k in FK}
+    g = make_ctx(picmod, n, [], v, ghost)
+    W = g.forward_record(3, 2)
+    phi = phi_of(g, seed, None, W)
+    gr = g.adjoint(seed=seed)
+    dot = sum(float(np.dot(gr[FK[i]], v[i])) for i in range(6))
+    assert abs(phi - dot) <= 1e-12 * abs(phi)
+    g.close()
+    w = rng.uniform(0.5, 1.5, 3)
+    g = make_ctx(picmod, n, [], v, ghost)
+    W = g.forward_record(3, 2)
+    gr = g.adjoint(weights=w)
+    dot = sum(float(np.dot(gr[FK[i]], v[i])) for i in range(6))
+    assert abs(2 * np.dot(w, W) - dot) <= 1e-12 * abs(dot)
+    g.close()
+
+
+@pytest.mark.parametrize("T", [1, 4, 16])
+@pytest.mark.parametrize("ghost", [1, 0])
+def test_vjp_against_central_differences(picmod, ghost, T):
+    rng = np.random.default_rng(100 + T)
+    n = (8, 8, 8)
+    G = 10 * 10 * 10
+    parts = rand_parts(rng, n, 3000, 0.3)
+    f0 = {i: 0.05 * rng.normal(size=G) for i in range(6)}
+    g = make_ctx(picmod, n, parts, f0, ghost)
+    nid = g.id_count()
+    seed = {k: 0.1 * rng.normal(size=G) for k in FK}
+    for k in PK:
+        seed[k] = rng.normal(size=nid)
+    w = rng.uniform(0.5, 1.5, T)
+    g.forward_record(T, 3)
+    gr = g.adjoint(weights=w, seed=seed)
+    g.close()
+    dirs = {k: rng.normal(size=G) for k in FK}
+    for k in PK:
+        dirs[k] = rng.normal(size=nid) * (1e-2 if k in "xyz" else 1.0)
+    dot = sum(float(np.dot(gr[k], dirs[k])) for k in dirs)
+
+    def phi_at(eps):
+        pp, off = [], 0
+        for (sort, x, y, z, px, py, pz, m, qm) in parts:
+            sl = slice(off, off + len(x))
+            arr = [a + eps * dirs[k][sl] for a, k in zip((x, y, z, px, py, pz), PK)]
+            pp.append([sort] + arr + [m, qm])
+            off += len(x)
+        ff = {i: f0[i] + eps * dirs[FK[i]] for i in range(6)}
+        h = make_ctx(picmod, n, pp, ff, ghost)
+        Wh = h.run(1, T)
+        ph = phi_of(h, seed, w, Wh)
+        h.close()
+        return ph
+    eps = 1e-6
+    fd = (phi_at(eps) - phi_at(-eps)) / (2 * eps)
+    print(f"ghost {ghost} T {T}: adjoint {dot:.12e} central difference {fd:.12e}")
+    assert abs(dot - fd) <= 1e-6 * abs(fd)
+
+
+def test_recompute_is_bitwise_and_restores(picmod):
+    params, parts = picmod.generate_workload(0)
+    prm = picmod.make_params((32, 8, 8), (3.2, 0.8, 0.8), 0.05, differentiable=1)
+    a = picmod.Pic(prm)
+    load_workload(a, parts)
+    Wa = a.run(1, 25)
+    b = picmod.Pic(prm)
+    load_workload(b, parts)
+    Wb = b.forward_record(25, 10)
+    assert np.array_equal(Wa, Wb)
+    b.adjoint(weights=np.ones(25))     # raises if a recomputed segment differs
+    for i in range(6):
+        assert np.array_equal(a.get_field(i), b.get_field(i))
+    assert np.array_equal(a.cell_counts(), b.cell_counts())
+    assert np.array_equal(a.run(26, 5), b.run(26, 5))
+    # the fixed-point deposit of this form against the plain FP64 form
+    c = picmod.Pic(picmod.make_params((32, 8, 8), (3.2, 0.8, 0.8), 0.05))
+    load_workload(c, parts)
+    Wc = c.run(1, 25)
+    assert np.max(np.abs(Wa - Wc) / np.abs(Wc)) < 1e-10
+    for h in (a, b, c):
+        h.close()
+
+
+@pytest.mark.parametrize("ghost", [1])
+def test_config3_gradient_against_finite_differences(picmod, ghost):
+    torch = pytest.importorskip("torch")
+    ag = importlib.import_module("0buf_amd.autograd")
+    bp = ag.BeamPlasma(3)
+    bp.run.params.ghost_mode = ghost
+    theta = torch.tensor([0.3] + [1e-3] * 8, dtype=torch.float64, requires_grad=True)
+    L = bp.loss(theta)
+    L.backward()
+    grad = theta.grad.numpy().copy()
+    bp.run.last.close()
+    d = 1e-5
+    th = theta.detach().numpy()
+    for i in range(9):
+        e = np.zeros(9)
+        e[i] = d
+        fd = (bp.loss_value(torch.tensor(th + e)) - bp.loss_value(torch.tensor(th - e))) / (2 * d)
+        print(f"parameter {i}: adjoint {grad[i]:.10e} central difference {fd:.10e}")
+        assert abs(grad[i] - fd) <= 1e-4 * abs(fd)
