@@ -74,16 +74,20 @@ def cpu_baseline(pic, cfg, slab, steps=2, threads=1):
                       f"{dt:.1f} s, single thread"}
 
 
-def run_form(pic, params, parts, bx0, precision, steps, warmup):
-    """Load the workload into a context of the given precision, run warmup
+def run_form(pic, params, cfg, parts, bx0, precision, steps, warmup, seed=12345):
+    """Load the workload into a context of the given precision (drawn on the
+    GPU by the device initializer, or the host-generated `parts`), run warmup
     untimed steps then `steps` timed steps (torch.cuda.synchronize on both
     sides); per-stage mean launch times from HIP events on the context's
     stream."""
     import torch
     params.precision = precision
     g = pic.Pic(params)
-    for (sort, x, y, z, px, py, pz, m, qm) in parts:
-        g.load_particles(sort, x, y, z, px, py, pz, m, qm)
+    if parts is None:
+        g.generate_device(cfg["species"], seed=seed)
+    else:
+        for (sort, x, y, z, px, py, pz, m, qm) in parts:
+            g.load_particles(sort, x, y, z, px, py, pz, m, qm)
     if bx0:
         g.load_field(pic.HX, np.full(g.G, bx0))
     P = g.particle_count()
@@ -152,6 +156,9 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--cpu-slab", type=int, default=8)
+    ap.add_argument("--host-init", action="store_true",
+                    help="generate the particles on the host and upload them instead of "
+                         "drawing them on the GPU (same particles, bit for bit)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run a plain elementwise kernel (dst = src + 1, FP32) that reads "
                          "and writes BYTES each, 5 times: the known-size reference for the "
@@ -176,16 +183,21 @@ def main():
     pic = importlib.import_module("0buf_amd")
     cfg = pic.beam_plasma_config(args.config)
     ppc = tuple(s.ppc for s in cfg["species"])
-    log(f"generating {workload_name(cfg, args.config, ppc)}")
-    t0 = time.perf_counter()
-    params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
-    log(f"generated {sum(len(p[1]) for p in parts)} particles in {time.perf_counter() - t0:.1f} s")
+    log(f"workload {workload_name(cfg, args.config, ppc)}")
+    parts = None
+    n = cfg["n"]
+    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"])
+    if args.host_init:
+        t0 = time.perf_counter()
+        params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
+        log(f"generated {sum(len(p[1]) for p in parts)} particles on the host in "
+            f"{time.perf_counter() - t0:.1f} s")
 
-    r64 = run_form(pic, params, parts, cfg["bx0"], pic.FP64, args.steps, args.warmup)
+    r64 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP64, args.steps, args.warmup)
     log(f"FP64: {r64['P']} particles, {args.steps} steps in {r64['dt']:.3f} s")
     r32 = None
     if not args.no_fp32:
-        r32 = run_form(pic, params, parts, cfg["bx0"], pic.FP32, args.steps, args.warmup)
+        r32 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP32, args.steps, args.warmup)
         log(f"FP32: {args.steps} steps in {r32['dt']:.3f} s")
     del parts
 
@@ -249,7 +261,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic seeded beam-plasma (Wichmann-Hill per-cell streams)",
+        "data": "synthetic seeded beam-plasma (Wichmann-Hill per-cell streams, seed 12345, "
+                "drawn on the GPU by the device initializer)",
         "config": {"workload": workload_name(cfg, args.config, ppc), "particles": P,
                    "cells": ncell, "precision": "fp64 plain form"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -39,7 +39,12 @@ def test_device_init_matches_host(picmod, precision):
     assert np.array_equal(a.cell_counts(), b.cell_counts())
     same_particles(a, b)
     for nt in range(1, 6):
-        assert a.step(nt) == b.step(nt)
+        ea, eb = a.step(nt), b.step(nt)
+        if precision == picmod.FP32:
+            assert ea == eb           # integer current: order-free, bitwise
+        else:
+            # FP64 atomics sum J in a run-dependent order: last-bit differences
+            assert ea == pytest.approx(eb, rel=1e-12)
     a.close()
     b.close()
 
@@ -51,6 +56,8 @@ def test_device_init_gaussian_beam_and_ids(picmod):
     same_particles(a, b)
     assert np.array_equal(a.particle_ids(), b.particle_ids())
     assert a.id_count() == b.id_count()
+    for nt in range(1, 4):
+        assert a.step(nt) == b.step(nt)
     a.close()
     b.close()
 
