@@ -105,7 +105,7 @@ EXPORTS = [
     "pic_sync",
     # readback and timing
     "pic_profile", "pic_profile_read", "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
-    "pic_get_particles", "pic_moments", "pic_gauss_residual",
+    "pic_get_particles", "pic_moments", "pic_gauss_residual", "pic_cell_moments",
     # state dumps (mumu files) and control points
     "pic_get_params", "pic_dump_file_name", "pic_dump_write", "pic_dump_read",
     "pic_save_state_dump", "pic_load_state_dump", "pic_control_point",
@@ -183,6 +183,7 @@ def load_library(path=LIB_PATH):
         "pic_save_state_dump": (C.c_int, [vp, C.c_char_p]),
         "pic_load_state_dump": (C.c_int, [vp, C.c_char_p, lp]),
         "pic_control_point": (C.c_int, [vp, C.c_char_p, C.c_int, C.c_int, dp, C.c_char_p]),
+        "pic_cell_moments": (C.c_int, [vp, C.c_int, C.c_int, dp]),
         "pic_particle_id_count": (C.c_long, [vp]),
         "pic_get_particle_ids": (C.c_int, [vp, ip, C.c_long, lp]),
         "pic_forward_record": (C.c_int, [vp, C.c_int, C.c_int, dp]),
@@ -384,6 +385,16 @@ class Pic:
         _check(self.lib.pic_generate_device(self._ctx, arr, len(specs), seed, C.byref(n)),
                "pic_generate_device")
         return n.value
+
+    MOMENT_OPS = {"density": 0, "kinetic_energy": 1, "velocity": 2, "current": 3}
+
+    def cell_moments(self, sort, op):
+        """Per-cell FP64 moment of one sort: density / kinetic_energy (ncell),
+        velocity / current (3, ncell)."""
+        k = self.MOMENT_OPS[op] if isinstance(op, str) else op
+        a = np.empty((3 if k >= 2 else 1) * self.ncell)
+        _check(self.lib.pic_cell_moments(self._ctx, sort, k, _dptr(a)), "pic_cell_moments")
+        return a.reshape(3, self.ncell) if k >= 2 else a
 
     # -- state dumps ---------------------------------------------------------
     def params_out(self):

@@ -94,10 +94,15 @@ int queue_capacity(const TileGeom &t, int cap) {
     return (int)std::max(64L, c);
 }
 
+/* slots per bucket: 1.5 x the fullest cell + 8, rounded up to 32 so that every
+ * bucket starts on a 128-byte line in the FP32 arrays (and on a line pair
+ * in the FP64 ones): a cell's particles then occupy whole lines and a
+ * wavefront's loads and in-place stores touch no partial line at a bucket
+ * start */
 int auto_capacity(int maxc) {
     int cap = (int)(maxc * 1.5) + 8;
-    cap = (cap + 7) / 8 * 8;
-    return std::max(cap, 16);
+    cap = (cap + 31) / 32 * 32;
+    return std::max(cap, 32);
 }
 
 /* upload staged particles into cell buckets (cell-sorted, stable) */
@@ -206,7 +211,7 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     int maxc = 0;
     for (long k = 0; k < NC; k++) maxc = std::max(maxc, cnt[k]);
     const int oldcap = parts.cap;
-    const int cap = std::max(auto_capacity(maxc), oldcap + 8);
+    const int cap = std::max(auto_capacity(maxc), (oldcap + 8 + 31) / 32 * 32);
     P np;
     int rc = alloc_parts(np, NC, cap, false, c->diff);
     if (rc) return rc;
@@ -931,6 +936,25 @@ int pic_moments(pic_ctx *c, double *out27) {
     HIPCHK(hipStreamSynchronize(c->stream));
     std::memcpy(out27, c->h_red, 27 * sizeof(double));
     return PIC_SUCCESS;
+}
+
+int pic_cell_moments(pic_ctx *c, int sort, int op, double *out) {
+    int rc = check_ready(c);
+    if (rc) return rc;
+    if (!out || sort < 0 || sort > 2 || op < PIC_DENSITY || op > PIC_CURRENT) return PIC_ERR_INVALID;
+    const long n = (op >= PIC_VELOCITY ? 3 : 1) * c->NC;
+    double *d = nullptr;
+    HIPCHK(hipMalloc(&d, n * sizeof(double)));
+    if (c->fp32)
+        launch_cell_moments32(c->NC, c->p32, c->sp, c->d_slot_sort, sort, op, d, c->stream);
+    else
+        launch_cell_moments64(c->NC, c->p, c->sp, c->d_slot_sort, sort, op, d, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    rc = PIC_SUCCESS;
+    if (hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = PIC_ERR_HIP;
+    (void)hipFree(d);
+    return rc;
 }
 
 int pic_gauss_residual(pic_ctx *c, double *res, double *max_drift, int set_reference) {

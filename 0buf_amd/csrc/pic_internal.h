@@ -174,6 +174,13 @@ void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,
                       const int *slot_sort, double *partials, int nblocks, double *out27,
                       hipStream_t s);
 
+/* per-cell moments of one sort (k_gauss.hip): op 0 density, 1 kinetic
+ * energy, 2 mean velocity (3 x ncell), 3 current (3 x ncell) */
+void launch_cell_moments64(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sort, int sort,
+                           int op, double *out, hipStream_t s);
+void launch_cell_moments32(long ncell, PartPtrs32 p, SpeciesTable sp, const int *slot_sort,
+                           int sort, int op, double *out, hipStream_t s);
+
 /* device-side seeded initializer (k_init.hip): one species into the
  * buckets (p64 or p32), appended behind what the cells already hold */
 void launch_generate_species(const Grid &g, const pic_species_spec &spec, const int *cnt_lk,

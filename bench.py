@@ -180,6 +180,11 @@ def main():
               "(the step would shard as x-slabs; see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
+    # torch carries its own HIP runtime beside the system one the library links;
+    # it initialises first, before any library call touches the device
+    import torch
+    torch.cuda.init()
+    torch.cuda.synchronize()
     pic = importlib.import_module("0buf_amd")
     cfg = pic.beam_plasma_config(args.config)
     ppc = tuple(s.ppc for s in cfg["species"])

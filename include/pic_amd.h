@@ -205,6 +205,16 @@ int pic_moments(pic_ctx *ctx, double *out27);
 /* the parameters the context was created with */
 int pic_get_params(pic_ctx *ctx, pic_params *out);
 
+/* Per-cell moments of one sort, reduced in FP64 over each cell's particles:
+ * the reference's diagnostics_operations (particle_types.h:15-16, declared
+ * and never implemented).  out: nx*ny*nz values for PIC_DENSITY (sum of the
+ * charge weights m) and PIC_KINETIC_ENERGY (sum of (m/q_m)(gamma-1), with
+ * gamma-1 formed as p.p/(gamma+1)),
+ * 3*nx*ny*nz (x block, y block, z block) for PIC_VELOCITY (mean of p/gamma,
+ * 0 in empty cells) and PIC_CURRENT (sum of m p/gamma). */
+enum pic_moment_op { PIC_DENSITY = 0, PIC_KINETIC_ENERGY = 1, PIC_VELOCITY = 2, PIC_CURRENT = 3 };
+int pic_cell_moments(pic_ctx *ctx, int sort, int op, double *out);
+
 /* ------------------------------------------------------------------------
  * State dumps in the reference's format (mumu<num:3><nt:8>.dat, Fortran
  * unformatted records; readControlPoint gpu_plasma.h:1088-1125,

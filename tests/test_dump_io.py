@@ -160,6 +160,8 @@ def test_gpu_dump_round_trip_and_oracle_step(picmod, orc, tmp_path):
     report = str(tmp_path / "control_points.dat")
     cp = b.control_point(path, nt=15, num=0, report=report)
     assert cp[:6].max() > 0
+    for k in range(12):                    # CheckArraySilent as the oracle restates it
+        assert cp[k] == pytest.approx(orc.rms(b.get_field(k), fields[k]), rel=1e-14, abs=0)
     line = open(report).read()
     assert line.startswith("nt    15 num   0 Ex ") and line.rstrip().split()[-2] == "particles"
     a.close()
