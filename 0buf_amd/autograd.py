@@ -91,13 +91,19 @@ class BeamPlasma:
     velocity along x, the amplitudes of the beam displacement
     x -> x + sum_k a_k sin(2 pi k x / Lx), and a uniform external Hx = bx0
     on every node.  Base state: the seeded generator's particles with the
-    beam drawn around drift0 and no displacement."""
+    beam drawn around drift0 and no displacement.
+
+    ghost_mode 1 (every E ghost node periodic) is the default here: with the
+    reference's ghosts (0) the field a particle sees jumps when it crosses a
+    y face, so the discrete loss is discontinuous in theta and central
+    differences at steps above ~1e-7 pick up the jumps."""
 
     def __init__(self, cfg, drift0=0.3, scale_n=None, nsteps=None, ckpt_every=20,
-                 stream_mode=None):
+                 stream_mode=None, ghost_mode=1):
         spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
         params, parts = pic.generate_workload(spec_cfg, scale_n=scale_n,
                                               stream_mode=stream_mode)
+        params.ghost_mode = ghost_mode
         self.params = params
         self.lx = params.lx
         self.species = []
@@ -149,3 +155,54 @@ class BeamPlasma:
         W = g.run(1, self.run.nsteps)
         g.close()
         return float(W[first - 1:last].sum())
+
+
+def fd_check(cfg=3, deltas=(1e-5,), ghost_mode=1, theta=None, first=150, last=200):
+    """Adjoint gradient of the windowed energy loss against central finite
+    differences of the same forward, for each step in `deltas`.  Returns a
+    dict: theta, adjoint gradient, {delta: finite differences}, loss."""
+    bp = BeamPlasma(cfg, ghost_mode=ghost_mode)
+    if theta is None:
+        theta = [0.3] + [1e-3] * 8
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    L = bp.loss(th, first, last)
+    L.backward()
+    grad = th.grad.numpy().copy()
+    bp.run.last.close()
+    base = th.detach().numpy()
+    fds = {}
+    for d in deltas:
+        fd = np.zeros(len(base))
+        for i in range(len(base)):
+            e = np.zeros(len(base))
+            e[i] = d
+            fd[i] = (bp.loss_value(torch.tensor(base + e), first, last) -
+                     bp.loss_value(torch.tensor(base - e), first, last)) / (2 * d)
+        fds[d] = fd
+    return {"theta": base, "grad": grad, "fd": fds, "loss": float(L)}
+
+
+def _main():
+    import argparse
+    ap = argparse.ArgumentParser(description="adjoint vs central finite differences")
+    ap.add_argument("--config", type=int, default=3)
+    ap.add_argument("--ghost-mode", type=int, default=1)
+    ap.add_argument("--deltas", default="1e-5")
+    args = ap.parse_args()
+    deltas = [float(v) for v in args.deltas.split(",")]
+    r = fd_check(args.config, deltas, args.ghost_mode)
+    names = ["drift"] + [f"a_{k}" for k in range(1, 9)]
+    print(f"config {args.config}, ghost_mode {args.ghost_mode}, loss {r['loss']:.15e}")
+    print("| parameter | adjoint | " + " | ".join(f"FD {d:.0e} (rel err)" for d in deltas) + " |")
+    print("|---|---|" + "---|" * len(deltas))
+    for i, nm in enumerate(names):
+        cells = [f"{r['fd'][d][i]:.9e} ({abs(r['grad'][i] - r['fd'][d][i]) / abs(r['fd'][d][i]):.1e})"
+                 for d in deltas]
+        print(f"| {nm} | {r['grad'][i]:.9e} | " + " | ".join(cells) + " |")
+    row = [f"{np.linalg.norm(r['grad'] - r['fd'][d]) / np.linalg.norm(r['fd'][d]):.2e}"
+           for d in deltas]
+    print("| normwise | | " + " | ".join(row) + " |")
+
+
+if __name__ == "__main__":
+    _main()

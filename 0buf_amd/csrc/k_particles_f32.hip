@@ -117,8 +117,9 @@ __device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from
     return to;
 }
 
+/* 8 waves per SIMD (at most 64 VGPRs, no spills) */
 template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
+__global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ float sF[6 * NB];
@@ -128,13 +129,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sLeave[C];
     __shared__ int sQ;
     __shared__ unsigned sFlags;
-    /* per-round lists: deferred second segments and leavers */
-    __shared__ float sDv[6][PIC_BLOCK];
-    __shared__ int sDi[PIC_BLOCK];
-    __shared__ float sLv[6][PIC_BLOCK];
-    __shared__ int sLd[PIC_BLOCK];
-    __shared__ unsigned char sLs[PIC_BLOCK];
-    __shared__ int sNd[2], sNl[2];
     extern __shared__ unsigned sMask[];
     long long *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
@@ -177,8 +171,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
-        sNd[0] = sNd[1] = 0;
-        sNl[0] = sNl[1] = 0;
     }
     __syncthreads();
     if (tid == 0) {
@@ -194,163 +186,127 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit32(PushArgs32 a) {
     for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
     __syncthreads();
 
-    /* rounds of one particle per lane; second deposit segments and leavers
-     * go to per-round LDS lists processed by consecutive lanes after the
-     * round (counters alternate by round parity) */
-    const int rounds = (T + PIC_BLOCK - 1) / PIC_BLOCK;
-    for (int rnd = 0; rnd < rounds; rnd++) {
-        const int fidx = rnd * PIC_BLOCK + tid;
-        const int par = rnd & 1;
-        if (fidx < T) {
-            int lo = 0, hi = C;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (sOff[mid] <= fidx) lo = mid; else hi = mid;
-            }
-            const int c = lo;
-            const int j = fidx - sOff[c];
-            const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-            const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-            const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
-
-            const float ox = a.p.p[0][pidx], oy = a.p.p[1][pidx], oz = a.p.p[2][pidx];
-            float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
-            const unsigned char s = a.p.sp[pidx];
-
-            /* gather: node-type axes interpolate between faces i, i+1 with
-             * weights (1-o, o); centre-type axes between the two nearest
-             * centres (rd = 1 when o >= 1/2; rd is also the start's dual cell) */
-            const int rdx = ox >= 0.5f, rdy = oy >= 0.5f, rdz = oz >= 0.5f;
-            const float cx0 = rdx ? 1.5f - ox : 0.5f - ox, cx1 = 1.0f - cx0;
-            const float cy0 = rdy ? 1.5f - oy : 0.5f - oy, cy1 = 1.0f - cy0;
-            const float cz0 = rdz ? 1.5f - oz : 0.5f - oz, cz1 = 1.0f - cz0;
-            const float nx0 = 1.0f - ox, ny0 = 1.0f - oy, nz0 = 1.0f - oz;
-            const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
-            const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
-            const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
-            const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
-            const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
-            const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
-
-            /* Boris push, Cayley form (particle.h:160-202) */
-            const float tau1 = a.qm[s] * a.tau * 0.5f;
-            pu += tau1 * Ex;
-            pv += tau1 * Ey;
-            pw += tau1 * Ez;
-            const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
-            const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
-            const float su = pu + pv * bz - pw * by;
-            const float sv = pv + pw * bx - pu * bz;
-            const float sw = pw + pu * by - pv * bx;
-            const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
-            const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-            const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
-            pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
-            pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
-            pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
-            const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
-            const float dux = a.tdh[0] * gi2 * pu, duy = a.tdh[1] * gi2 * pv,
-                        duz = a.tdh[2] * gi2 * pw;
-            const float o1x = ox + dux, o1y = oy + duy, o1z = oz + duz;
-
-            /* deposit: dual cell of each end (floor(o + 1/2)), crossing code m
-             * and relay on the crossed dual faces (o = 1/2), as CurrentToMesh */
-            {
-                const int rx1 = o1x >= 0.5f, ry1 = o1y >= 0.5f, rz1 = o1z >= 0.5f;
-                const int m = 4 * (rdx != rx1) + 2 * (rdy != ry1) + (rdz != rz1);
-                float num = 0.0f, den = 1.0f;
-                switch (m) {
-                case 1: num = 0.5f - oz; den = duz; break;
-                case 2: num = 0.5f - oy; den = duy; break;
-                case 4: num = 0.5f - ox; den = dux; break;
-                case 3:
-                    num = (0.5f - oz) * duz * a.hh[2] + (0.5f - oy) * duy * a.hh[1];
-                    den = duz * duz * a.hh[2] + duy * duy * a.hh[1];
-                    break;
-                case 5:
-                    num = (0.5f - oz) * duz * a.hh[2] + (0.5f - ox) * dux * a.hh[0];
-                    den = duz * duz * a.hh[2] + dux * dux * a.hh[0];
-                    break;
-                case 6:
-                    num = (0.5f - oy) * duy * a.hh[1] + (0.5f - ox) * dux * a.hh[0];
-                    den = duy * duy * a.hh[1] + dux * dux * a.hh[0];
-                    break;
-                default: break;
-                }
-                const float sr = num / den;
-                const float x2x = (m & 4) ? 0.5f : ox + dux * sr;
-                const float x2y = (m & 2) ? 0.5f : oy + duy * sr;
-                const float x2z = (m & 1) ? 0.5f : oz + duz * sr;
-                const float *cj = a.cj[s];
-                const float e1x = m ? x2x : o1x, e1y = m ? x2y : o1y, e1z = m ? x2z : o1z;
-                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
-                if (m) {
-                    const int k = atomicAdd(&sNd[par], 1);
-                    sDv[0][k] = x2x;
-                    sDv[1][k] = x2y;
-                    sDv[2][k] = x2z;
-                    sDv[3][k] = o1x;
-                    sDv[4][k] = o1y;
-                    sDv[5][k] = o1z;
-                    sDi[k] = c | (rx1 << 8) | (ry1 << 9) | (rz1 << 10) | ((int)s << 16);
-                }
-            }
-
-            /* migration by the cell-local offset */
-            int shx = (int)floorf(o1x), shy = (int)floorf(o1y), shz = (int)floorf(o1z);
-            float nxo = o1x - (float)shx, nyo = o1y - (float)shy, nzo = o1z - (float)shz;
-            if (nxo >= 1.0f) { nxo -= 1.0f; shx += 1; }
-            if (nyo >= 1.0f) { nyo -= 1.0f; shy += 1; }
-            if (nzo >= 1.0f) { nzo -= 1.0f; shz += 1; }
-            if ((shx | shy | shz) == 0) {
-                a.p.p[0][pidx] = nxo;
-                a.p.p[1][pidx] = nyo;
-                a.p.p[2][pidx] = nzo;
-                a.p.p[3][pidx] = pu;
-                a.p.p[4][pidx] = pv;
-                a.p.p[5][pidx] = pw;
-            } else {
-                atomicAdd(&sLeave[c], 1);
-                atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
-                int rx = gi + shx, ry = gl + shy, rz = gk + shz;
-                if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
-                if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
-                if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
-                const int k = atomicAdd(&sNl[par], 1);
-                sLv[0][k] = nxo;
-                sLv[1][k] = nyo;
-                sLv[2][k] = nzo;
-                sLv[3][k] = pu;
-                sLv[4][k] = pv;
-                sLv[5][k] = pw;
-                sLd[k] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
-                sLs[k] = s;
-            }
+    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+        int lo = 0, hi = C;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
         }
-        __syncthreads();
-        const int nd = sNd[par], nl = sNl[par];
-        if (tid == 0) {
-            sNd[par ^ 1] = 0;
-            sNl[par ^ 1] = 0;
+        const int c = lo;
+        const int j = fidx - sOff[c];
+        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+
+        const float ox = a.p.p[0][pidx], oy = a.p.p[1][pidx], oz = a.p.p[2][pidx];
+        float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
+        const unsigned char s = a.p.sp[pidx];
+
+        /* gather: node-type axes interpolate between faces i, i+1 with
+         * weights (1-o, o); centre-type axes between the two nearest centres
+         * (rd = 1 when o >= 1/2; rd is also the start's dual cell) */
+        const int rdx = ox >= 0.5f, rdy = oy >= 0.5f, rdz = oz >= 0.5f;
+        const float cx0 = rdx ? 1.5f - ox : 0.5f - ox, cx1 = 1.0f - cx0;
+        const float cy0 = rdy ? 1.5f - oy : 0.5f - oy, cy1 = 1.0f - cy0;
+        const float cz0 = rdz ? 1.5f - oz : 0.5f - oz, cz1 = 1.0f - cz0;
+        const float nx0 = 1.0f - ox, ny0 = 1.0f - oy, nz0 = 1.0f - oz;
+        const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
+        const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
+        const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
+        const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
+        const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
+        const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
+
+        /* Boris push, Cayley form (particle.h:160-202) */
+        const float tau1 = a.qm[s] * a.tau * 0.5f;
+        pu += tau1 * Ex;
+        pv += tau1 * Ey;
+        pw += tau1 * Ez;
+        const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+        const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
+        const float su = pu + pv * bz - pw * by;
+        const float sv = pv + pw * bx - pu * bz;
+        const float sw = pw + pu * by - pv * bx;
+        const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
+        const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
+        const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
+        pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
+        pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
+        pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
+        const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+        const float dux = a.tdh[0] * gi2 * pu, duy = a.tdh[1] * gi2 * pv,
+                    duz = a.tdh[2] * gi2 * pw;
+        const float o1x = ox + dux, o1y = oy + duy, o1z = oz + duz;
+
+        /* deposit: dual cell of each end (floor(o + 1/2)), crossing code m
+         * and relay on the crossed dual faces (o = 1/2), as CurrentToMesh */
+        {
+            const int rx1 = o1x >= 0.5f, ry1 = o1y >= 0.5f, rz1 = o1z >= 0.5f;
+            const int m = 4 * (rdx != rx1) + 2 * (rdy != ry1) + (rdz != rz1);
+            float num = 0.0f, den = 1.0f;
+            switch (m) {
+            case 1: num = 0.5f - oz; den = duz; break;
+            case 2: num = 0.5f - oy; den = duy; break;
+            case 4: num = 0.5f - ox; den = dux; break;
+            case 3:
+                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - oy) * duy * a.hh[1];
+                den = duz * duz * a.hh[2] + duy * duy * a.hh[1];
+                break;
+            case 5:
+                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - ox) * dux * a.hh[0];
+                den = duz * duz * a.hh[2] + dux * dux * a.hh[0];
+                break;
+            case 6:
+                num = (0.5f - oy) * duy * a.hh[1] + (0.5f - ox) * dux * a.hh[0];
+                den = duy * duy * a.hh[1] + dux * dux * a.hh[0];
+                break;
+            default: break;
+            }
+            const float sr = num / den;
+            const float x2x = (m & 4) ? 0.5f : ox + dux * sr;
+            const float x2y = (m & 2) ? 0.5f : oy + duy * sr;
+            const float x2z = (m & 1) ? 0.5f : oz + duz * sr;
+            const float *cj = a.cj[s];
+            const float e1x = m ? x2x : o1x, e1y = m ? x2y : o1y, e1z = m ? x2z : o1z;
+            vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
+            if (m)
+                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rx1, ry1, rz1, x2x, x2y, x2z, o1x, o1y, o1z,
+                                   cj);
         }
-        if (tid < nd) {
-            const int pk = sDi[tid];
-            const int c = pk & 255;
-            const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-            vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, (pk >> 8) & 1, (pk >> 9) & 1, (pk >> 10) & 1,
-                               sDv[0][tid], sDv[1][tid], sDv[2][tid], sDv[3][tid], sDv[4][tid],
-                               sDv[5][tid], a.cj[(pk >> 16) & 255]);
-        }
-        if (tid < nl) {
+
+        /* migration by the cell-local offset */
+        int shx = (int)floorf(o1x), shy = (int)floorf(o1y), shz = (int)floorf(o1z);
+        float nxo = o1x - (float)shx, nyo = o1y - (float)shy, nzo = o1z - (float)shz;
+        if (nxo >= 1.0f) { nxo -= 1.0f; shx += 1; }
+        if (nyo >= 1.0f) { nyo -= 1.0f; shy += 1; }
+        if (nzo >= 1.0f) { nzo -= 1.0f; shz += 1; }
+        if ((shx | shy | shz) == 0) {
+            a.p.p[0][pidx] = nxo;
+            a.p.p[1][pidx] = nyo;
+            a.p.p[2][pidx] = nzo;
+            a.p.p[3][pidx] = pu;
+            a.p.p[4][pidx] = pv;
+            a.p.p[5][pidx] = pw;
+        } else {
+            atomicAdd(&sLeave[c], 1);
+            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
+            int rx = gi + shx, ry = gl + shy, rz = gk + shz;
+            if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
+            if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
+            if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
             const int slot = atomicAdd(&sQ, 1);
             if (slot < a.q.qcap) {
                 const long e = tile * (long)a.q.qcap + slot;
-                for (int q = 0; q < 6; q++) a.q.q[q][e] = sLv[q][tid];
-                a.q.sp[e] = sLs[tid];
-                a.q.dest[e] = sLd[tid];
+                a.q.q[0][e] = nxo;
+                a.q.q[1][e] = nyo;
+                a.q.q[2][e] = nzo;
+                a.q.q[3][e] = pu;
+                a.q.q[4][e] = pv;
+                a.q.q[5][e] = pw;
+                a.q.sp[e] = s;
+                a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
             }
         }
-        __syncthreads();
     }
     __syncthreads();
 

@@ -119,8 +119,10 @@ __device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from
     return to;
 }
 
+/* 4 waves per SIMD (at most 128 VGPRs): the FP64 chains of the push are
+ * latency-bound at 3 */
 template <int TX, int TY, int TZ, bool DIFF>
-__global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
+__global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ double sF[6 * NB];
@@ -132,14 +134,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     __shared__ unsigned long long sDrop, sDup;
-    /* per-round lists: deferred second segments and leavers */
-    __shared__ double sDv[6][PIC_BLOCK];
-    __shared__ int sDi[PIC_BLOCK];
-    __shared__ unsigned char sDs[PIC_BLOCK];
-    __shared__ double sLv[6][PIC_BLOCK];
-    __shared__ int sLi[PIC_BLOCK];
-    __shared__ int sLid[DIFF ? PIC_BLOCK : 1];
-    __shared__ int sNd[2], sNl[2];
     extern __shared__ unsigned sMask[];
     double *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
@@ -191,8 +185,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
         sFlags = 0;
         sDrop = 0;
         sDup = 0;
-        sNd[0] = sNd[1] = 0;
-        sNl[0] = sNl[1] = 0;
     }
     __syncthreads();
     if (tid == 0) {
@@ -210,217 +202,170 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_push_deposit(PushArgs a) {
 
     const double hx = g.hx, hy = g.hy, hz = g.hz, tau = g.tau;
     unsigned bad = 0;
-    /* The particles run in rounds of one per lane.  The two divergent parts
-     * of the step, the second deposit segment of a crossing particle and the
-     * receivers + departure entries of a leaver, are appended to per-round
-     * LDS lists and run after the round by consecutive lanes, so a
-     * wavefront does them with full lanes instead of once per round for
-     * the few lanes that need them.  The list counters alternate between
-     * two slots by round parity: the slot of the next round is cleared
-     * while this round's lists are processed. */
-    const int rounds = (T + PIC_BLOCK - 1) / PIC_BLOCK;
-    for (int rnd = 0; rnd < rounds; rnd++) {
-        const int fidx = rnd * PIC_BLOCK + tid;
-        const int par = rnd & 1;
-        if (fidx < T) {
-            int lo = 0, hi = C;
-            while (hi - lo > 1) {
-                int mid = (lo + hi) >> 1;
-                if (sOff[mid] <= fidx) lo = mid; else hi = mid;
-            }
-            const int c = lo;
-            const int j = fidx - sOff[c];
-            Geo G;
-            G.ca = c / (TY * TZ);
-            G.cb = (c / TZ) % TY;
-            G.cc = c % TZ;
-            const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
-            const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
-            G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
-            G.x0 = (double)gi * hx;
-            G.y0 = (double)gl * hy;
-            G.z0 = (double)gk * hz;
+    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+        int lo = 0, hi = C;
+        while (hi - lo > 1) {
+            int mid = (lo + hi) >> 1;
+            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
+        }
+        const int c = lo;
+        const int j = fidx - sOff[c];
+        Geo G;
+        G.ca = c / (TY * TZ);
+        G.cb = (c / TZ) % TY;
+        G.cc = c % TZ;
+        const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
+        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+        G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
+        G.x0 = (double)gi * hx;
+        G.y0 = (double)gl * hy;
+        G.z0 = (double)gk * hz;
 
-            const double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
-            double pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
-            const unsigned char s = a.p.sp[pidx];
-            const int pid = DIFF ? a.p.id[pidx] : 0;
-            const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
-            if (snap) {
-                a.d.snap.p[0][pidx] = x;
-                a.d.snap.p[1][pidx] = y;
-                a.d.snap.p[2][pidx] = z;
-                a.d.snap.p[3][pidx] = pu;
-                a.d.snap.p[4][pidx] = pv;
-                a.d.snap.p[5][pidx] = pw;
-                a.d.snap.sp[pidx] = s;
-                a.d.snap.id[pidx] = pid;
-            }
+        const double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
+        double pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
+        const unsigned char s = a.p.sp[pidx];
+        const int pid = DIFF ? a.p.id[pidx] : 0;
+        const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
+        if (snap) {
+            a.d.snap.p[0][pidx] = x;
+            a.d.snap.p[1][pidx] = y;
+            a.d.snap.p[2][pidx] = z;
+            a.d.snap.p[3][pidx] = pu;
+            a.d.snap.p[4][pidx] = pv;
+            a.d.snap.p[5][pidx] = pw;
+            a.d.snap.sp[pidx] = s;
+            a.d.snap.id[pidx] = pid;
+        }
 
-            const double frx = divc(x - G.x0, hx, a.dc.rhx);
-            const double fry = divc(y - G.y0, hy, a.dc.rhy);
-            const double frz = divc(z - G.z0, hz, a.dc.rhz);
-            double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
-            if (!(x < 0 || y < 0 || z < 0)) {
-                const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
-                const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
-                const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
-                             s51 = 1. - s5, s61 = 1. - s6;
-                if (!W.ok) {
-                    bad = 1;
-                } else {
-                    const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
-                    Ex = interp<BY, BZ>(sF + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, s1, s11, s4, s41, s6, s61);
-                    Ey = interp<BY, BZ>(sF + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, s2, s21, s3, s31, s6, s61);
-                    Ez = interp<BY, BZ>(sF + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, s2, s21, s4, s41, s5, s51);
-                    Hx = interp<BY, BZ>(sF + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, s2, s21, s3, s31, s5, s51);
-                    Hy = interp<BY, BZ>(sF + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, s1, s11, s4, s41, s5, s51);
-                    Hz = interp<BY, BZ>(sF + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, s1, s11, s3, s31, s6, s61);
-                }
-            }
-
-            /* Particle::Move (particle.h:160-202) */
-            const Boris B = boris_move(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, q_m * tau * 0.5);
-            pu = B.p1[0];
-            pv = B.p1[1];
-            pw = B.p1[2];
-            const double x1 = x + tau * (B.g2 * pu);
-            const double y1 = y + tau * (B.g2 * pv);
-            const double z1 = z + tau * (B.g2 * pw);
-
-            /* CurrentToMesh (cell.h:1306-1422): without a crossing the first
-             * segment is the whole path (x2 = x1); with one, the second
-             * segment x2 -> x1 goes to the round's deferred list */
-            {
-                const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
-                const int m = R.m;
-                const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
-                pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y,
-                                          e1z, mass, bad, true);
-                if (m) {
-                    const int k = atomicAdd(&sNd[par], 1);
-                    sDv[0][k] = R.x2x;
-                    sDv[1][k] = R.x2y;
-                    sDv[2][k] = R.x2z;
-                    sDv[3][k] = x1;
-                    sDv[4][k] = y1;
-                    sDv[5][k] = z1;
-                    /* indices outside 0..255 are out of the stencil anyway */
-                    const int i2x = min(max(R.i2x, 0), 255), i2y = min(max(R.i2y, 0), 255),
-                              i2z = min(max(R.i2z, 0), 255);
-                    sDi[k] = c | (i2x << 8) | (i2y << 16) | (i2z << 24);
-                    sDs[k] = s;
-                }
-            }
-
-            /* x <- x1, Reflect (cell.h:1410-1478) */
-            const double xn = reflect1(x1, g.Lx), yn = reflect1(y1, g.Ly),
-                         zn = reflect1(z1, g.Lz);
-
-            /* flyDirection (cell.h:694-701) with the owner cell's geometry */
-            int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
-                          ? 2
-                          : (((xn < G.x0) || ((xn > g.Lx - hx) && (gi == 0))) ? 0 : 1);
-            int ddy = (((yn > G.y0 + hy) && (yn < G.y0 + 2 * hy)) || ((yn < hy) && (gl == g.Ny - 1)))
-                          ? 2
-                          : (((yn < G.y0) || ((yn > g.Ly - hy) && (gl == 0))) ? 0 : 1);
-            int ddz = (((zn > G.z0 + hz) && (zn < G.z0 + 2 * hz)) || ((zn < hz) && (gk == g.Nz - 1)))
-                          ? 2
-                          : (((zn < G.z0) || ((zn > g.Lz - hz) && (gk == 0))) ? 0 : 1);
-            const int dir = ddx * 9 + ddy * 3 + ddz;
-            const bool inside = (G.x0 <= xn) && (xn < G.x0 + hx) && (G.y0 <= yn) &&
-                                (yn < G.y0 + hy) && (G.z0 <= zn) && (zn < G.z0 + hz);
-            if (!inside) atomicAdd(&sOut[c], 1);
-
-            if (dir == 13) {
-                a.p.p[0][pidx] = xn;
-                a.p.p[1][pidx] = yn;
-                a.p.p[2][pidx] = zn;
-                a.p.p[3][pidx] = pu;
-                a.p.p[4][pidx] = pv;
-                a.p.p[5][pidx] = pw;
+        const double frx = divc(x - G.x0, hx, a.dc.rhx);
+        const double fry = divc(y - G.y0, hy, a.dc.rhy);
+        const double frz = divc(z - G.z0, hz, a.dc.rhz);
+        double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
+        if (!(x < 0 || y < 0 || z < 0)) {
+            const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
+            const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
+            const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
+                         s51 = 1. - s5, s61 = 1. - s6;
+            if (!W.ok) {
+                bad = 1;
             } else {
-                atomicAdd(&sLeave[c], 1);
-                atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
-                const int k = atomicAdd(&sNl[par], 1);
-                sLv[0][k] = xn;
-                sLv[1][k] = yn;
-                sLv[2][k] = zn;
-                sLv[3][k] = pu;
-                sLv[4][k] = pv;
-                sLv[5][k] = pw;
-                sLi[k] = c | ((int)s << 8);
-                if (DIFF) sLid[k] = pid;
+                const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
+                Ex = interp<BY, BZ>(sF + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, s1, s11, s4, s41, s6, s61);
+                Ey = interp<BY, BZ>(sF + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, s2, s21, s3, s31, s6, s61);
+                Ez = interp<BY, BZ>(sF + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, s2, s21, s4, s41, s5, s51);
+                Hx = interp<BY, BZ>(sF + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, s2, s21, s3, s31, s5, s51);
+                Hy = interp<BY, BZ>(sF + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, s1, s11, s4, s41, s5, s51);
+                Hz = interp<BY, BZ>(sF + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, s1, s11, s3, s31, s6, s61);
             }
         }
-        __syncthreads();
-        const int nd = sNd[par], nl = sNl[par];
-        if (tid == 0) {
-            sNd[par ^ 1] = 0;
-            sNl[par ^ 1] = 0;
+
+        /* Particle::Move (particle.h:160-202) */
+        const Boris B = boris_move(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, q_m * tau * 0.5);
+        pu = B.p1[0];
+        pv = B.p1[1];
+        pw = B.p1[2];
+        const double x1 = x + tau * (B.g2 * pu);
+        const double y1 = y + tau * (B.g2 * pv);
+        const double z1 = z + tau * (B.g2 * pw);
+
+        /* CurrentToMesh (cell.h:1306-1422): one code path for every lane;
+         * without a crossing the first segment is the whole path (x2 = x1)
+         * and the second is skipped */
+        {
+            const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
+            const int m = R.m;
+            const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
+            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y, e1z,
+                                      mass, bad, true);
+            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i2x, R.i2y, R.i2z, R.x2x, R.x2y, R.x2z, x1,
+                                      y1, z1, mass, bad, m != 0);
         }
-        /* deferred second segments: pqr(i2; x2 -> x1) */
-        if (tid < nd) {
-            const int pk = sDi[tid];
-            const int c = pk & 255;
-            Geo G;
-            G.ca = c / (TY * TZ);
-            G.cb = (c / TZ) % TY;
-            G.cc = c % TZ;
-            G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
-            G.x0 = (double)(I0 + G.ca) * hx;
-            G.y0 = (double)(L0 + G.cb) * hy;
-            G.z0 = (double)(K0 + G.cc) * hz;
-            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, (pk >> 8) & 255, (pk >> 16) & 255,
-                                      (pk >> 24) & 255, sDv[0][tid], sDv[1][tid], sDv[2][tid],
-                                      sDv[3][tid], sDv[4][tid], sDv[5][tid], a.sp.m[sDs[tid]], bad,
-                                      true);
-        }
-        /* deferred leavers: GPU_ArrangeFlights (kernels.cu:316-370): every
-         * bulk cell of the 27-neighbourhood (getWrapCellNumber) whose
-         * isPointInCell (cell.h:717) holds receives a copy; per axis the test
-         * is separable, so the receivers are a product of per-axis sets */
-        if (tid < nl) {
-            const int c = sLi[tid] & 255;
-            const unsigned char s = (unsigned char)((sLi[tid] >> 8) & 255);
-            const int gi = I0 + c / (TY * TZ), gl = L0 + (c / TZ) % TY, gk = K0 + c % TZ;
-            const double xn = sLv[0][tid], yn = sLv[1][tid], zn = sLv[2][tid];
-            int rx[3], ry[3], rz[3], nxr = 0, nyr = 0, nzr = 0;
+
+        /* x <- x1, Reflect (cell.h:1410-1478) */
+        const double xn = reflect1(x1, g.Lx), yn = reflect1(y1, g.Ly), zn = reflect1(z1, g.Lz);
+
+        /* flyDirection (cell.h:694-701) with the owner cell's geometry */
+        int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
+                      ? 2
+                      : (((xn < G.x0) || ((xn > g.Lx - hx) && (gi == 0))) ? 0 : 1);
+        int ddy = (((yn > G.y0 + hy) && (yn < G.y0 + 2 * hy)) || ((yn < hy) && (gl == g.Ny - 1)))
+                      ? 2
+                      : (((yn < G.y0) || ((yn > g.Ly - hy) && (gl == 0))) ? 0 : 1);
+        int ddz = (((zn > G.z0 + hz) && (zn < G.z0 + 2 * hz)) || ((zn < hz) && (gk == g.Nz - 1)))
+                      ? 2
+                      : (((zn < G.z0) || ((zn > g.Lz - hz) && (gk == 0))) ? 0 : 1);
+        const int dir = ddx * 9 + ddy * 3 + ddz;
+        const bool inside = (G.x0 <= xn) && (xn < G.x0 + hx) && (G.y0 <= yn) &&
+                            (yn < G.y0 + hy) && (G.z0 <= zn) && (zn < G.z0 + hz);
+        if (!inside) atomicAdd(&sOut[c], 1);
+
+        if (dir == 13) {
+            a.p.p[0][pidx] = xn;
+            a.p.p[1][pidx] = yn;
+            a.p.p[2][pidx] = zn;
+            a.p.p[3][pidx] = pu;
+            a.p.p[4][pidx] = pv;
+            a.p.p[5][pidx] = pw;
+        } else {
+            atomicAdd(&sLeave[c], 1);
+            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
+            /* GPU_ArrangeFlights (kernels.cu:316-370): every bulk cell of the
+             * 27-neighbourhood (getWrapCellNumber) whose isPointInCell
+             * (cell.h:717) holds receives a copy; per axis the test is
+             * separable, so the receivers are a product of per-axis sets. */
+            /* per axis, bit d+1 set when neighbour d in {-1,0,1} accepts */
+            unsigned bx = 0, by = 0, bz = 0;
             for (int d = -1; d <= 1; d++) {
                 int r = gi + d;
                 if (r >= g.Nx) r -= g.Nx; else if (r < 0) r += g.Nx;
                 double r0 = (double)r * hx;
-                if (r0 <= xn && xn < r0 + hx) rx[nxr++] = r;
+                if (r0 <= xn && xn < r0 + hx) bx |= 1u << (d + 1);
                 r = gl + d;
                 if (r >= g.Ny) r -= g.Ny; else if (r < 0) r += g.Ny;
                 r0 = (double)r * hy;
-                if (r0 <= yn && yn < r0 + hy) ry[nyr++] = r;
+                if (r0 <= yn && yn < r0 + hy) by |= 1u << (d + 1);
                 r = gk + d;
                 if (r >= g.Nz) r -= g.Nz; else if (r < 0) r += g.Nz;
                 r0 = (double)r * hz;
-                if (r0 <= zn && zn < r0 + hz) rz[nzr++] = r;
+                if (r0 <= zn && zn < r0 + hz) bz |= 1u << (d + 1);
             }
-            const int ncopy = nxr * nyr * nzr;
+            const int ncopy = __popc(bx) * __popc(by) * __popc(bz);
             if (ncopy == 0) atomicAdd(&sDrop, 1ull);
             if (ncopy > 1) atomicAdd(&sDup, (unsigned long long)(ncopy - 1));
             if (ncopy > 0) {
                 int slot = atomicAdd(&sQ, ncopy);
                 const long qb = tile * (long)a.q.qcap;
                 int kq = 0;
-                for (int ax = 0; ax < nxr; ax++)
-                    for (int ay = 0; ay < nyr; ay++)
-                        for (int az = 0; az < nzr; az++, kq++) {
-                            int idx = slot + kq;
+                /* receivers in the reference's scan order (x, then y, then z) */
+                for (int ax = 0; ax < 3; ax++) {
+                    if (!((bx >> ax) & 1u)) continue;
+                    int rx = gi + ax - 1;
+                    if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
+                    for (int ay = 0; ay < 3; ay++) {
+                        if (!((by >> ay) & 1u)) continue;
+                        int ry = gl + ay - 1;
+                        if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
+                        for (int az = 0; az < 3; az++) {
+                            if (!((bz >> az) & 1u)) continue;
+                            int rz = gk + az - 1;
+                            if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
+                            const int idx = slot + kq++;
                             if (idx >= a.q.qcap) continue;
-                            long e = qb + idx;
-                            for (int q = 0; q < 6; q++) a.q.q[q][e] = sLv[q][tid];
+                            const long e = qb + idx;
+                            a.q.q[0][e] = xn;
+                            a.q.q[1][e] = yn;
+                            a.q.q[2][e] = zn;
+                            a.q.q[3][e] = pu;
+                            a.q.q[4][e] = pv;
+                            a.q.q[5][e] = pw;
                             a.q.sp[e] = s;
-                            if (DIFF) a.q.id[e] = sLid[tid];
-                            a.q.dest[e] = (int)(((long)rx[ax] * g.Ny + ry[ay]) * g.Nz + rz[az]);
+                            if (DIFF) a.q.id[e] = pid;
+                            a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
                         }
+                    }
+                }
             }
         }
-        __syncthreads();
     }
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();
@@ -622,7 +567,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles(long ncell, cons
     d4[dst] = p.p[4][e];
     d5[dst] = p.p[5][e];
     dsp[dst] = p.sp[e];
-    dcell[dst] = (int)c;
+    if (dcell) dcell[dst] = (int)c;
     if (did) did[dst] = p.id ? p.id[e] : -1;
 }
 

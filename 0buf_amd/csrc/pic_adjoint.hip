@@ -4,9 +4,10 @@
  * per-step records, and the reverse sweep through the adjoint kernels of
  * k_adjoint.hip.
  *
- * Memory (one GPU): a checkpoint is the bucketed particle state (six FP64
- * arrays, species, id, counts) plus E and H; checkpoints are kept every K
- * steps and at the end of the run.  During the reverse sweep one segment of
+ * Memory (one GPU): a checkpoint is the compacted particle state (six FP64
+ * arrays, species and id of the occupied slots, with per-cell offsets and
+ * counts) plus E and H; checkpoints are kept every K steps and at the end of
+ * the run.  During the reverse sweep one segment of
  * K steps is recomputed from its checkpoint and each step's pre-push
  * particle state (written by the push kernel itself) and E^n, H^{n+1/2} are
  * recorded; the segment is then reversed step by step.  The J deposit of
@@ -36,11 +37,21 @@ struct FieldSet {
     double *f[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   /* E, H */
 };
 
+/* compacted particle state: cell c's particles at [off[c], off[c+1]) */
+struct Compact {
+    long P = 0;
+    long *off = nullptr;      /* NC + 1 */
+    int *count = nullptr;     /* NC */
+    double *v[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    unsigned char *sp = nullptr;
+    int *id = nullptr;
+};
+
 struct AdjointStore {
     int nsteps = 0, K = 0;
     bool recorded = false;
     long long anomalies = 0;          /* dropped + duplicated leavers in the forward run */
-    std::vector<PartPtrs> ck;         /* checkpoints at 0, K, 2K, ... and at nsteps */
+    std::vector<Compact> ck;          /* checkpoints at 0, K, 2K, ... and at nsteps */
     std::vector<FieldSet> ckf;
     std::vector<PartPtrs> rec;        /* per-step particle records of one segment */
     std::vector<FieldSet> recf;       /* E^n, H^{n+1/2} of each recorded step */
@@ -79,17 +90,42 @@ void free_lam(AdjointStore *s) {
     s->lp = LamParts{};
 }
 
-int save_state(pic_ctx *c, PartPtrs &dst, FieldSet &f) {
-    if (dst.p[0] && dst.cap != c->p.cap) free_parts(dst);
-    if (!dst.p[0]) {
-        int rc = alloc_parts(dst, c->NC, c->p.cap, true, true);
-        if (rc) return rc;
+void free_compact(Compact &k) {
+    if (k.off) (void)hipFree(k.off);
+    if (k.count) (void)hipFree(k.count);
+    for (int j = 0; j < 6; j++)
+        if (k.v[j]) (void)hipFree(k.v[j]);
+    if (k.sp) (void)hipFree(k.sp);
+    if (k.id) (void)hipFree(k.id);
+    k = Compact{};
+}
+
+int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
+    const long NC = c->NC;
+    std::vector<int> cnt(NC);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<long> off(NC + 1, 0);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    const long P = off[NC];
+    if (dst.off && dst.P < P) free_compact(dst);
+    if (!dst.off) {
+        const size_t Pn = (size_t)std::max(P, 1L);
+        HIPCHK(hipMalloc(&dst.off, (NC + 1) * sizeof(long)));
+        HIPCHK(hipMalloc(&dst.count, NC * sizeof(int)));
+        for (int j = 0; j < 6; j++) HIPCHK(hipMalloc(&dst.v[j], Pn * sizeof(double)));
+        HIPCHK(hipMalloc(&dst.sp, Pn));
+        HIPCHK(hipMalloc(&dst.id, Pn * sizeof(int)));
+        dst.P = P;
     }
+    HIPCHK(hipMemcpy(dst.off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(dst.count, c->p.count, NC * sizeof(int), hipMemcpyDeviceToDevice,
+                          c->stream));
     if (!f.f[0]) {
         int rc = alloc_fields(f, c->g.G);
         if (rc) return rc;
     }
-    launch_copy_parts(c->NC, c->p, dst, c->stream);
+    launch_gather_particles(NC, dst.off, c->p, dst.v, dst.sp, nullptr, dst.id, c->stream);
     const size_t bytes = c->g.G * sizeof(double);
     for (int k = 0; k < 6; k++)
         HIPCHK(hipMemcpyAsync(f.f[k], c->f.f[PIC_EX + k], bytes, hipMemcpyDeviceToDevice,
@@ -98,9 +134,10 @@ int save_state(pic_ctx *c, PartPtrs &dst, FieldSet &f) {
     return PIC_SUCCESS;
 }
 
-int restore_state(pic_ctx *c, const PartPtrs &src, const FieldSet &f) {
-    if (src.cap > c->p.cap) return PIC_ERR_STATE;
-    launch_copy_parts(c->NC, src, c->p, c->stream);
+int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
+    const double *v[6] = {src.v[0], src.v[1], src.v[2], src.v[3], src.v[4], src.v[5]};
+    launch_scatter_load(src.off, c->NC, v, src.sp, c->p, c->stream);
+    launch_scatter_ids(src.off, c->NC, src.id, c->p, c->stream);
     const size_t bytes = c->g.G * sizeof(double);
     for (int k = 0; k < 6; k++)
         HIPCHK(hipMemcpyAsync(c->f.f[PIC_EX + k], f.f[k], bytes, hipMemcpyDeviceToDevice,
@@ -147,7 +184,7 @@ namespace picx {
 void free_adjoint(pic_ctx *c) {
     AdjointStore *s = c->adj;
     if (!s) return;
-    for (auto &p : s->ck) free_parts(p);
+    for (auto &p : s->ck) free_compact(p);
     for (auto &f : s->ckf) free_fields(f);
     for (auto &p : s->rec) free_parts(p);
     for (auto &f : s->recf) free_fields(f);

@@ -99,10 +99,13 @@ int queue_capacity(const TileGeom &t, int cap) {
  * in the FP64 ones): a cell's particles then occupy whole lines and a
  * wavefront's loads and in-place stores touch no partial line at a bucket
  * start */
+#ifndef PIC_CAP_ALIGN
+#define PIC_CAP_ALIGN 32
+#endif
 int auto_capacity(int maxc) {
     int cap = (int)(maxc * 1.5) + 8;
-    cap = (cap + 31) / 32 * 32;
-    return std::max(cap, 32);
+    cap = (cap + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN;
+    return std::max(cap, PIC_CAP_ALIGN);
 }
 
 /* upload staged particles into cell buckets (cell-sorted, stable) */
@@ -211,7 +214,8 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     int maxc = 0;
     for (long k = 0; k < NC; k++) maxc = std::max(maxc, cnt[k]);
     const int oldcap = parts.cap;
-    const int cap = std::max(auto_capacity(maxc), (oldcap + 8 + 31) / 32 * 32);
+    const int cap = std::max(auto_capacity(maxc),
+                             (oldcap + 8 + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN);
     P np;
     int rc = alloc_parts(np, NC, cap, false, c->diff);
     if (rc) return rc;

@@ -127,8 +127,9 @@ def test_reader_rejects_damaged_files(picmod, tmp_path):
 @pytest.mark.gpu
 def test_gpu_dump_round_trip_and_oracle_step(picmod, orc, tmp_path):
     """A dump of the GPU state after 10 config-0 steps, loaded into a fresh
-    context and into the oracle: the next steps agree bitwise with the
-    original context, and the oracle's migration from the dump matches."""
+    context and into the oracle: the next steps agree with the original
+    context (to the last bits that FP64 atomics leave run-dependent, counts
+    exactly), and the oracle's migration from the dump matches."""
     from helpers import load_workload
     params, parts = picmod.generate_workload(0)
     a = picmod.Pic(params)
@@ -153,7 +154,7 @@ def test_gpu_dump_round_trip_and_oracle_step(picmod, orc, tmp_path):
     for nt in range(11, 16):
         ea, eb = a.step(nt), b.step(nt)
         eo, _ = o.step(nt)
-        assert ea == eb
+        assert ea == pytest.approx(eb, rel=1e-13)   # FP64 atomic summation order
         assert np.array_equal(a.cell_counts(), b.cell_counts())
         assert np.array_equal(b.cell_counts(), o.cell_counts())
         assert eb == pytest.approx(eo, rel=1e-12)

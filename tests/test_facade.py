@@ -4,8 +4,9 @@ and the main.cu-style driver built on it (0buf_amd/examples/plasma_main.cpp).
 CPU: the driver links against libpic_amd.so and, without a device, ends with
 the library's PIC_ERR_NO_DEVICE status and a message instead of a crash.
 GPU: Initialize() from a parameter file + Compute() writes energy.dat lines
-("%10d %25.15e \\n", gpu_plasma.h:681) identical, character for character, to
-the energies of the same seeded run driven through the Python binding.
+in the reference's format ("%10d %25.15e \\n", gpu_plasma.h:681) with the
+energies of the same seeded run driven through the Python binding (to 1e-12:
+the plain FP64 form's atomics leave the last bits run-dependent).
 """
 import os
 import subprocess
@@ -62,6 +63,12 @@ def test_driver_energy_file_matches_binding(picmod, tmp_path):
         g.load_particles(sp.sort, x, y, z, px, py, pz, m, sp.q_m)
     e = [g.step(nt) for nt in range(1, 9)]
     g.close()
-    want = ["%10d %25.15e " % (nt, v) for nt, v in zip(range(1, 9), e)]
-    assert lines == want
+    # the reference's line format, then the values: FP64 atomics add J in a
+    # run-dependent order, so two runs may differ in the last digits
+    assert len(lines) == 8
+    for nt, (line, v) in enumerate(zip(lines, e), start=1):
+        assert len(line) == len("%10d %25.15e " % (nt, v)) and line.endswith(" ")
+        step, val = line.split()
+        assert int(step) == nt
+        assert float(val) == pytest.approx(v, rel=1e-12)
     assert np.all(np.array(e) >= 0)

@@ -51,7 +51,7 @@ def test_cell_moments_against_oracle_state(picmod, orc, precision):
         o.step(nt)
         g.step(nt)
     po = o.particles()
-    tol = 1e-12 if precision == 0 else 2e-6
+    tol = 1e-12 if precision == 0 else 1e-5     # FP32 momenta: ~1e-7 each, summed
     for sort in range(3):
         for op in ("density", "kinetic_energy", "velocity", "current"):
             want = expected(po, g.ncell, sort, op)
