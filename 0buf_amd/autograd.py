@@ -1,7 +1,8 @@
 """torch.autograd.Function over the C ABI of the differentiable form.
 
 EnergyHistory.apply(run, x, y, z, px, py, pz, E, H) runs `run.nsteps` steps of
-the FP64 step on the GPU from the given initial state and returns the energy
+the step on the GPU (FP64, or the FP32 form with FP64 gradient accumulation,
+per `run.params.precision`) from the given initial state and returns the energy
 history W_1..W_T (W_t = sum over all ghosted nodes of Ex^2+Ey^2+Ez^2, the
 reference's Diagnose, gpu_plasma.h:621-686).  Its backward calls
 pic_adjoint: the checkpointed recompute and the hand-written adjoint
@@ -99,11 +100,14 @@ class BeamPlasma:
     differences at steps above ~1e-7 pick up the jumps."""
 
     def __init__(self, cfg, drift0=0.3, scale_n=None, nsteps=None, ckpt_every=20,
-                 stream_mode=None, ghost_mode=1):
+                 stream_mode=None, ghost_mode=1, precision=pic.FP64, workload=None):
         spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
-        params, parts = pic.generate_workload(spec_cfg, scale_n=scale_n,
-                                              stream_mode=stream_mode)
+        if workload is None:
+            workload = pic.generate_workload(spec_cfg, scale_n=scale_n, stream_mode=stream_mode)
+        params, parts = workload
+        params = pic.PicParams.from_buffer_copy(params)
         params.ghost_mode = ghost_mode
+        params.precision = precision
         self.params = params
         self.lx = params.lx
         self.species = []
@@ -116,6 +120,7 @@ class BeamPlasma:
             beam.append(np.full(len(x), sort == pic.BEAM_ELECTRON))
         base = [torch.from_numpy(np.concatenate(c)) for c in cols]
         self.beam = torch.from_numpy(np.concatenate(beam))
+        self.beam_idx = torch.nonzero(self.beam).flatten()
         self.x0, self.y0, self.z0 = base[0], base[1], base[2]
         p = torch.stack(base[3:6])
         u = p / torch.sqrt(1.0 + (p * p).sum(0))
@@ -129,10 +134,11 @@ class BeamPlasma:
         drift, amps = theta[0], theta[1:9]
         bx0 = theta[9] if theta.numel() > 9 else theta.new_zeros(())
         k = torch.arange(1, 9, dtype=theta.dtype)
-        disp = (amps[:, None] * torch.sin(2.0 * math.pi * k[:, None] * self.x0[None, :] /
+        xb = self.x0[self.beam_idx]
+        disp = (amps[:, None] * torch.sin(2.0 * math.pi * k[:, None] * xb[None, :] /
                                           self.lx)).sum(0)
-        x = torch.where(self.beam, self.x0 + disp, self.x0)
-        ux = torch.where(self.beam, self.u_th[0] + drift, self.u_th[0])
+        x = self.x0.index_put((self.beam_idx,), xb + disp)
+        ux = self.u_th[0].index_put((self.beam_idx,), self.u_th[0][self.beam_idx] + drift)
         u = torch.stack([ux, self.u_th[1], self.u_th[2]])
         p = u / torch.sqrt(1.0 - (u * u).sum(0))
         E = torch.zeros(3, self.G, dtype=theta.dtype)
@@ -182,13 +188,63 @@ def fd_check(cfg=3, deltas=(1e-5,), ghost_mode=1, theta=None, first=150, last=20
     return {"theta": base, "grad": grad, "fd": fds, "loss": float(L)}
 
 
+def default_theta(cfg):
+    """drift 0.3, a_k = 1e-3, and Bx0 at the workload's value"""
+    spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
+    return [0.3] + [1e-3] * 8 + [spec_cfg["bx0"]]
+
+
+def precision_compare(cfg=4, theta=None, first=150, last=200, ckpt_every=20, scale_n=None,
+                      nsteps=None):
+    """Gradient of the windowed energy loss by the FP32 form (FP32 forward and
+    particle adjoint, FP64 field adjoints and accumulation) against the FP64
+    adjoint on the same shape and seeded particles.  Returns, per precision,
+    the loss, the gradient with respect to theta = (drift, a_1..a_8, Bx0) and
+    the device times (ms) of the forward with checkpoints, the recompute and
+    the adjoint steps."""
+    spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
+    theta = default_theta(spec_cfg) if theta is None else theta
+    workload = pic.generate_workload(spec_cfg, scale_n=scale_n)
+    out = {}
+    for prec in (pic.FP64, pic.FP32):
+        bp = BeamPlasma(spec_cfg, precision=prec, nsteps=nsteps, ckpt_every=ckpt_every,
+                        workload=workload)
+        th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+        L = bp.loss(th, first, last)
+        L.backward()
+        out[prec] = {"loss": float(L), "grad": th.grad.numpy().copy(),
+                     "ms": bp.run.last.adjoint_timing(), "n": int(bp.beam.numel())}
+        bp.run.last.close()
+        bp.run.last = None
+    return out
+
+
 def _main():
     import argparse
     ap = argparse.ArgumentParser(description="adjoint vs central finite differences")
     ap.add_argument("--config", type=int, default=3)
     ap.add_argument("--ghost-mode", type=int, default=1)
     ap.add_argument("--deltas", default="1e-5")
+    ap.add_argument("--compare-precision", action="store_true",
+                    help="FP32 adjoint against the FP64 adjoint instead of differences")
     args = ap.parse_args()
+    if args.compare_precision:
+        r = precision_compare(args.config)
+        g64, g32 = r[pic.FP64]["grad"], r[pic.FP32]["grad"]
+        names = ["drift"] + [f"a_{k}" for k in range(1, 9)] + ["Bx0"]
+        print(f"config {args.config}, {r[pic.FP64]['n']} particles, loss FP64 "
+              f"{r[pic.FP64]['loss']:.12e} FP32 {r[pic.FP32]['loss']:.12e}")
+        print("| parameter | FP64 adjoint | FP32 adjoint | abs diff / max abs FP64 |")
+        print("|---|---|---|---|")
+        gmax = np.max(np.abs(g64))
+        for i, nm in enumerate(names):
+            print(f"| {nm} | {g64[i]:.9e} | {g32[i]:.9e} | {abs(g32[i] - g64[i]) / gmax:.2e} |")
+        print(f"| normwise | | | {np.linalg.norm(g32 - g64) / np.linalg.norm(g64):.2e} |")
+        for prec, nm in ((pic.FP64, "FP64"), (pic.FP32, "FP32")):
+            f, rc, rv = r[prec]["ms"]
+            print(f"{nm}: forward+checkpoints {f:.1f} ms, recompute {rc:.1f} ms, adjoint steps "
+                  f"{rv:.1f} ms, (recompute+adjoint)/forward {(rc + rv) / f:.2f}")
+        return
     deltas = [float(v) for v in args.deltas.split(",")]
     r = fd_check(args.config, deltas, args.ghost_mode)
     names = ["drift"] + [f"a_{k}" for k in range(1, 9)]

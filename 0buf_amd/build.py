@@ -4,7 +4,7 @@ Usage: python 0buf_amd/build.py   (or __graft_entry__.build())
 
 Every HIP file is compiled with hipcc --offload-arch=gfx950; the host-only
 initializer with g++.  Objects go to 0buf_amd/_build/, the library to
-0buf_amd/libpic_amd.so.  No cmake/ninja: the dependency graph is ten files.
+0buf_amd/libpic_amd.so.  No cmake/ninja: the dependency graph is eleven files.
 """
 import os
 import subprocess
@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "k_gauss.hip": ["-ffp-contract=off"],
     "k_adjoint.hip": ["-ffp-contract=off"],
     "k_init.hip": ["-ffp-contract=off"],
+    "k_adjoint32.hip": ["-ffp-contract=fast"],
     "pic_api.hip": ["-ffp-contract=off"],
     "pic_adjoint.hip": ["-ffp-contract=off"],
 }

@@ -83,7 +83,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_generate(InitArgs a, P p) {
 
 /* differentiable form: ids in load order (species-major, then the host
  * generator's cell order i-major with per-(l,k) offsets, then draw order) */
-__global__ __launch_bounds__(PIC_BLOCK) void k_generate_ids(InitArgs a, PartPtrs p,
+template <class P>
+__global__ __launch_bounds__(PIC_BLOCK) void k_generate_ids(InitArgs a, P p,
                                                             const int *start_of_cell) {
     const long c = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     const Grid &g = a.g;
@@ -124,10 +125,16 @@ void launch_generate_species(const Grid &g, const pic_species_spec &spec, const 
         hipLaunchKernelGGL(k_generate<PartPtrs>, dim3(blocks_for(ncell)), dim3(PIC_BLOCK), 0, s,
                            a, *p64);
         if (p64->id)
-            hipLaunchKernelGGL(k_generate_ids, dim3(blocks_for(ncell)), dim3(PIC_BLOCK), 0, s, a,
-                               *p64, (const int *)start_of_cell);
+            hipLaunchKernelGGL(k_generate_ids<PartPtrs>, dim3(blocks_for(ncell)), dim3(PIC_BLOCK),
+                               0, s, a, *p64, (const int *)start_of_cell);
     } else {
+        if (p32->id)
+            (void)hipMemcpyAsync(start_of_cell, p32->count, ncell * sizeof(int),
+                                 hipMemcpyDeviceToDevice, s);
         hipLaunchKernelGGL(k_generate<PartPtrs32>, dim3(blocks_for(ncell)), dim3(PIC_BLOCK), 0,
                            s, a, *p32);
+        if (p32->id)
+            hipLaunchKernelGGL(k_generate_ids<PartPtrs32>, dim3(blocks_for(ncell)),
+                               dim3(PIC_BLOCK), 0, s, a, *p32, (const int *)start_of_cell);
     }
 }

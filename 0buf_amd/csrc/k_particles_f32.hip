@@ -16,7 +16,7 @@
  *   * migration by the cell-local offset: a particle whose offset leaves
  *     [0,1) moves to the neighbouring cell (periodic), exactly one receiver.
  */
-#include "pic_internal.h"
+#include "k_push_math32.h"
 
 /* copies of the LDS J box: lane l deposits into copy l % JCOPIES, which
  * divides the same-address conflicts of the 64-bit LDS atomics (consecutive
@@ -45,6 +45,8 @@ struct PushArgs32 {
     float tau;
     int dep_cap;
     DevStatus *st;
+    PartPtrs32 snap;               /* differentiable form: record of the pre-step state */
+    int snap_on;
 };
 
 /* round-to-nearest int64 of v (|v| < 2^51) by the 1.5*2^52 shifter */
@@ -57,16 +59,6 @@ __device__ __forceinline__ void add64(long long *p, long long v) {
     atomicAdd((unsigned long long *)p, (unsigned long long)v);
 }
 
-template <int BY, int BZ>
-__device__ __forceinline__ float tri(const float *F, int u, int v, int w, float x0, float x1,
-                                     float y0, float y1, float z0, float z1) {
-    const float *p = F + (u * BY + v) * BZ + w;
-    float a = z0 * p[0] + z1 * p[1];
-    float b = z0 * p[BZ] + z1 * p[BZ + 1];
-    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
-    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
-    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
-}
 
 /* One straight segment a -> b inside dual cell rd (relative to the owner
  * cell; the dual cell spans centres rd-1/2 .. rd+1/2): the 12 VB node
@@ -118,7 +110,7 @@ __device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from
 }
 
 /* 8 waves per SIMD (at most 64 VGPRs, no spills) */
-template <int TX, int TY, int TZ>
+template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
@@ -134,6 +126,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
 
     const Grid &g = a.g;
     const int tid = threadIdx.x;
+    const bool snap = DIFF && a.snap_on;
     long tile;
     {
         long nt = a.t.ntiles, b = blockIdx.x;
@@ -151,8 +144,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
         int cnt = 0;
         if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
-            cnt = a.p.count[((long)gi * g.Ny + gl) * g.Nz + gk];
+            const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+            cnt = a.p.count[cell];
             cnt = cnt < cap ? cnt : cap;
+            if (snap) a.snap.count[cell] = cnt;
         }
         sCnt[c] = cnt;
         sLeave[c] = 0;
@@ -201,15 +196,24 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         const float ox = a.p.p[0][pidx], oy = a.p.p[1][pidx], oz = a.p.p[2][pidx];
         float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
         const unsigned char s = a.p.sp[pidx];
+        const int pid = DIFF ? a.p.id[pidx] : 0;
+        if (snap) {
+            a.snap.p[0][pidx] = ox;
+            a.snap.p[1][pidx] = oy;
+            a.snap.p[2][pidx] = oz;
+            a.snap.p[3][pidx] = pu;
+            a.snap.p[4][pidx] = pv;
+            a.snap.p[5][pidx] = pw;
+            a.snap.sp[pidx] = s;
+            a.snap.id[pidx] = pid;
+        }
 
-        /* gather: node-type axes interpolate between faces i, i+1 with
-         * weights (1-o, o); centre-type axes between the two nearest centres
-         * (rd = 1 when o >= 1/2; rd is also the start's dual cell) */
-        const int rdx = ox >= 0.5f, rdy = oy >= 0.5f, rdz = oz >= 0.5f;
-        const float cx0 = rdx ? 1.5f - ox : 0.5f - ox, cx1 = 1.0f - cx0;
-        const float cy0 = rdy ? 1.5f - oy : 0.5f - oy, cy1 = 1.0f - cy0;
-        const float cz0 = rdz ? 1.5f - oz : 0.5f - oz, cz1 = 1.0f - cz0;
-        const float nx0 = 1.0f - ox, ny0 = 1.0f - oy, nz0 = 1.0f - oz;
+        /* gather (weights as in k_push_math32.h), Boris push, deposit */
+        const Weights32 W = weights32(ox, oy, oz);
+        const int rdx = W.rd[0], rdy = W.rd[1], rdz = W.rd[2];
+        const float cx0 = W.c0[0], cx1 = 1.0f - cx0, cy0 = W.c0[1], cy1 = 1.0f - cy0,
+                    cz0 = W.c0[2], cz1 = 1.0f - cz0;
+        const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
         const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
         const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
         const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
@@ -217,61 +221,20 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
         const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
 
-        /* Boris push, Cayley form (particle.h:160-202) */
-        const float tau1 = a.qm[s] * a.tau * 0.5f;
-        pu += tau1 * Ex;
-        pv += tau1 * Ey;
-        pw += tau1 * Ez;
-        const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
-        const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
-        const float su = pu + pv * bz - pw * by;
-        const float sv = pv + pw * bx - pu * bz;
-        const float sw = pw + pu * by - pv * bx;
-        const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
-        const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-        const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
-        pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
-        pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
-        pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
-        const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
-        const float dux = a.tdh[0] * gi2 * pu, duy = a.tdh[1] * gi2 * pv,
-                    duz = a.tdh[2] * gi2 * pw;
-        const float o1x = ox + dux, o1y = oy + duy, o1z = oz + duz;
-
-        /* deposit: dual cell of each end (floor(o + 1/2)), crossing code m
-         * and relay on the crossed dual faces (o = 1/2), as CurrentToMesh */
+        const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, a.qm[s] * a.tau * 0.5f, a.tdh);
+        pu = B.p1[0];
+        pv = B.p1[1];
+        pw = B.p1[2];
+        const float o1x = ox + B.du[0], o1y = oy + B.du[1], o1z = oz + B.du[2];
         {
-            const int rx1 = o1x >= 0.5f, ry1 = o1y >= 0.5f, rz1 = o1z >= 0.5f;
-            const int m = 4 * (rdx != rx1) + 2 * (rdy != ry1) + (rdz != rz1);
-            float num = 0.0f, den = 1.0f;
-            switch (m) {
-            case 1: num = 0.5f - oz; den = duz; break;
-            case 2: num = 0.5f - oy; den = duy; break;
-            case 4: num = 0.5f - ox; den = dux; break;
-            case 3:
-                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - oy) * duy * a.hh[1];
-                den = duz * duz * a.hh[2] + duy * duy * a.hh[1];
-                break;
-            case 5:
-                num = (0.5f - oz) * duz * a.hh[2] + (0.5f - ox) * dux * a.hh[0];
-                den = duz * duz * a.hh[2] + dux * dux * a.hh[0];
-                break;
-            case 6:
-                num = (0.5f - oy) * duy * a.hh[1] + (0.5f - ox) * dux * a.hh[0];
-                den = duy * duy * a.hh[1] + dux * dux * a.hh[0];
-                break;
-            default: break;
-            }
-            const float sr = num / den;
-            const float x2x = (m & 4) ? 0.5f : ox + dux * sr;
-            const float x2y = (m & 2) ? 0.5f : oy + duy * sr;
-            const float x2z = (m & 1) ? 0.5f : oz + duz * sr;
+            const Relay32 R = relay32(ox, oy, oz, o1x, o1y, o1z, B.du, W.rd, a.hh);
+            const int m = R.m;
             const float *cj = a.cj[s];
-            const float e1x = m ? x2x : o1x, e1y = m ? x2y : o1y, e1z = m ? x2z : o1z;
+            const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
             vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
             if (m)
-                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rx1, ry1, rz1, x2x, x2y, x2z, o1x, o1y, o1z,
-                                   cj);
+                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
+                                   R.x2[2], o1x, o1y, o1z, cj);
         }
 
         /* migration by the cell-local offset */
@@ -304,6 +267,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
                 a.q.q[4][e] = pv;
                 a.q.q[5][e] = pw;
                 a.q.sp[e] = s;
+                if (DIFF) a.q.id[e] = pid;
                 a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
             }
         }
@@ -324,6 +288,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
                 const long from = cell * cap + sidx, to = cell * cap + h;
                 for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
                 a.p.sp[to] = a.p.sp[from];
+                if (DIFF) a.p.id[to] = a.p.id[from];
                 h = next_bit(sMask, base, h + 1, np, true);
                 sidx = next_bit(sMask, base, sidx + 1, n, false);
             }
@@ -374,6 +339,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(PartPtrs32 p, QueuePtrs3
             const long to = (long)d * p.cap + slot;
             for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
             p.sp[to] = q.sp[e];
+            if (p.id) p.id[to] = q.id[e];
         } else {
             over = true;
         }
@@ -416,6 +382,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_repack32(long ncell, PartPtrs32 f
     const long dst = c * to.cap + j;
     for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
     to.sp[dst] = from.sp[e];
+    if (to.id) to.id[dst] = from.id[e];
 }
 
 __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(QueuePtrs32 q, int oldcap,
@@ -430,6 +397,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(QueuePtrs32 q, int
         const long dst = (long)q.dest[e] * to.cap + slot;
         for (int k = 0; k < 6; k++) to.p[k][dst] = q.q[k][e];
         to.sp[dst] = q.sp[e];
+        if (to.id) to.id[dst] = q.id[e];
     }
 }
 
@@ -439,7 +407,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles32(Grid g, long n
                                                                  double *d0, double *d1,
                                                                  double *d2, double *d3,
                                                                  double *d4, double *d5,
-                                                                 unsigned char *dsp, int *dcell) {
+                                                                 unsigned char *dsp, int *dcell,
+                                                                 int *did) {
     const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (e >= ncell * p.cap) return;
     const long c = e / p.cap;
@@ -455,7 +424,39 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles32(Grid g, long n
     d4[dst] = p.p[4][e];
     d5[dst] = p.p[5][e];
     dsp[dst] = p.sp[e];
-    dcell[dst] = (int)c;
+    if (dcell) dcell[dst] = (int)c;
+    if (did) did[dst] = p.id ? p.id[e] : -1;
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_gather_raw32(long ncell, const long *off,
+                                                           PartPtrs32 p, float *d0, float *d1,
+                                                           float *d2, float *d3, float *d4,
+                                                           float *d5, unsigned char *dsp,
+                                                           int *did) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    if (j >= off[c + 1] - off[c]) return;
+    const long dst = off[c] + j;
+    d0[dst] = p.p[0][e];
+    d1[dst] = p.p[1][e];
+    d2[dst] = p.p[2][e];
+    d3[dst] = p.p[3][e];
+    d4[dst] = p.p[4][e];
+    d5[dst] = p.p[5][e];
+    dsp[dst] = p.sp[e];
+    if (did) did[dst] = p.id[e];
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_scatter_ids32(const long *off, long ncell,
+                                                            const int *src, PartPtrs32 p) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long c = e / p.cap;
+    const int j = (int)(e % p.cap);
+    if (j >= off[c + 1] - off[c]) return;
+    p.id[e] = src[off[c] + j];
 }
 
 __global__ __launch_bounds__(PIC_BLOCK) void k_moments32(Grid g, long ncell, PartPtrs32 p,
@@ -515,7 +516,7 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             hipStream_t s) {
+                             const PartPtrs32 *diff, hipStream_t s) {
     PushArgs32 a;
     a.g = g;
     a.t = t;
@@ -534,8 +535,14 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.tau = (float)g.tau;
     a.dep_cap = dep_cap;
     a.st = st;
-    hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
-                       dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+    a.snap = diff ? *diff : PartPtrs32{};
+    a.snap_on = diff && diff->p[0] != nullptr;
+    if (diff)
+        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+    else
+        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
 }
 
 void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
@@ -561,10 +568,24 @@ void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtr
 }
 
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
-                               double *dst[6], unsigned char *dsp, int *dcell, hipStream_t s) {
+                               double *dst[6], unsigned char *dsp, int *dcell, int *did,
+                               hipStream_t s) {
     hipLaunchKernelGGL(k_gather_particles32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0,
                        s, g, ncell, offsets, p, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5],
-                       dsp, dcell);
+                       dsp, dcell, did);
+}
+
+void launch_gather_raw32(long ncell, const long *offsets, PartPtrs32 p, float *dst[6],
+                         unsigned char *dsp, int *did, hipStream_t s) {
+    hipLaunchKernelGGL(k_gather_raw32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       ncell, offsets, p, dst[0], dst[1], dst[2], dst[3], dst[4], dst[5], dsp,
+                       did);
+}
+
+void launch_scatter_ids32(const long *offsets, long ncell, const int *src, PartPtrs32 p,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_scatter_ids32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
+                       offsets, ncell, src, p);
 }
 
 void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,

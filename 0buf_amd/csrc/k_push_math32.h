@@ -1,0 +1,144 @@
+/*
+ * k_push_math32.h -- per-particle arithmetic of the FP32 precision form,
+ * shared by the FP32 push (k_particles_f32.hip) and the FP32 particle
+ * adjoint (k_adjoint32.hip), both built with -ffp-contract=fast: the adjoint
+ * recomputes the forward's dual cells, crossing code and relay point from
+ * the same expressions.
+ *
+ * Positions are offsets o in [0,1) inside the owning cell (cell units).
+ * Node-type axes interpolate between faces i, i+1 with weights (1-o, o);
+ * centre-type axes between the two nearest centres, rd = (o >= 1/2), near
+ * weight rd ? 3/2 - o : 1/2 - o (rd is also the start's dual cell).
+ */
+#ifndef K_PUSH_MATH32_H
+#define K_PUSH_MATH32_H
+
+#include "pic_internal.h"
+
+template <int BY, int BZ>
+__device__ __forceinline__ float tri(const float *F, int u, int v, int w, float x0, float x1,
+                                     float y0, float y1, float z0, float z1) {
+    const float *p = F + (u * BY + v) * BZ + w;
+    float a = z0 * p[0] + z1 * p[1];
+    float b = z0 * p[BZ] + z1 * p[BZ + 1];
+    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
+    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
+    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
+}
+
+struct Weights32 {
+    int rd[3];        /* dual cell of the start (0/1 per axis) */
+    float c0[3];      /* centre-type near weights */
+    float n0[3];      /* node-type near weights 1 - o */
+};
+
+__device__ __forceinline__ Weights32 weights32(float ox, float oy, float oz) {
+    Weights32 W;
+    W.rd[0] = ox >= 0.5f;
+    W.rd[1] = oy >= 0.5f;
+    W.rd[2] = oz >= 0.5f;
+    W.c0[0] = W.rd[0] ? 1.5f - ox : 0.5f - ox;
+    W.c0[1] = W.rd[1] ? 1.5f - oy : 0.5f - oy;
+    W.c0[2] = W.rd[2] ? 1.5f - oz : 0.5f - oz;
+    W.n0[0] = 1.0f - ox;
+    W.n0[1] = 1.0f - oy;
+    W.n0[2] = 1.0f - oz;
+    return W;
+}
+
+/* Boris push, Cayley form (particle.h:160-202), FP32, with the
+ * intermediates of the adjoint; du = (tau/h) p1 / gamma1 in cell units */
+struct Boris32 {
+    float pm[3], ps, b[3], s[3], rden, p1[3], gi2, du[3];
+};
+
+__device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float Ex, float Ey,
+                                           float Ez, float Hx, float Hy, float Hz, float tau1,
+                                           const float tdh[3]) {
+    Boris32 B;
+    pu += tau1 * Ex;
+    pv += tau1 * Ey;
+    pw += tau1 * Ez;
+    B.pm[0] = pu;
+    B.pm[1] = pv;
+    B.pm[2] = pw;
+    const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+    B.ps = ps;
+    const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
+    B.b[0] = bx;
+    B.b[1] = by;
+    B.b[2] = bz;
+    const float su = pu + pv * bz - pw * by;
+    const float sv = pv + pw * bx - pu * bz;
+    const float sw = pw + pu * by - pv * bx;
+    B.s[0] = su;
+    B.s[1] = sv;
+    B.s[2] = sw;
+    const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
+    const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
+    const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
+    B.rden = rden;
+    pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
+    pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
+    pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
+    B.p1[0] = pu;
+    B.p1[1] = pv;
+    B.p1[2] = pw;
+    const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+    B.gi2 = gi2;
+    B.du[0] = tdh[0] * gi2 * pu;
+    B.du[1] = tdh[1] * gi2 * pv;
+    B.du[2] = tdh[2] * gi2 * pw;
+    return B;
+}
+
+/* deposit geometry (CurrentToMesh, cell.h:1306-1422, in cell units): dual
+ * cell of each end (floor(o + 1/2)), crossing code m and the relay point on
+ * the crossed dual faces (o = 1/2); for two crossed faces the third
+ * coordinate follows the path at the parameter of the h^2-weighted
+ * projection, as the reference's two-face cases */
+struct Relay32 {
+    int r1[3], m;
+    float num, den, sr, x2[3];
+};
+
+__device__ __forceinline__ Relay32 relay32(float ox, float oy, float oz, float o1x, float o1y,
+                                           float o1z, const float du[3], const int rd[3],
+                                           const float hh[3]) {
+    Relay32 R;
+    R.r1[0] = o1x >= 0.5f;
+    R.r1[1] = o1y >= 0.5f;
+    R.r1[2] = o1z >= 0.5f;
+    const int m = 4 * (rd[0] != R.r1[0]) + 2 * (rd[1] != R.r1[1]) + (rd[2] != R.r1[2]);
+    R.m = m;
+    const float dux = du[0], duy = du[1], duz = du[2];
+    float num = 0.0f, den = 1.0f;
+    switch (m) {
+    case 1: num = 0.5f - oz; den = duz; break;
+    case 2: num = 0.5f - oy; den = duy; break;
+    case 4: num = 0.5f - ox; den = dux; break;
+    case 3:
+        num = (0.5f - oz) * duz * hh[2] + (0.5f - oy) * duy * hh[1];
+        den = duz * duz * hh[2] + duy * duy * hh[1];
+        break;
+    case 5:
+        num = (0.5f - oz) * duz * hh[2] + (0.5f - ox) * dux * hh[0];
+        den = duz * duz * hh[2] + dux * dux * hh[0];
+        break;
+    case 6:
+        num = (0.5f - oy) * duy * hh[1] + (0.5f - ox) * dux * hh[0];
+        den = duy * duy * hh[1] + dux * dux * hh[0];
+        break;
+    default: break;
+    }
+    R.num = num;
+    R.den = den;
+    const float sr = num / den;
+    R.sr = sr;
+    R.x2[0] = (m & 4) ? 0.5f : ox + dux * sr;
+    R.x2[1] = (m & 2) ? 0.5f : oy + duy * sr;
+    R.x2[2] = (m & 1) ? 0.5f : oz + duz * sr;
+    return R;
+}
+
+#endif

@@ -4,14 +4,18 @@
  * per-step records, and the reverse sweep through the adjoint kernels of
  * k_adjoint.hip.
  *
+ * Both precisions: the FP64 plain form's differentiable variant and the
+ * FP32 precision form (FP32 forward and particle adjoint, field adjoints
+ * accumulated in FP64, k_adjoint32.hip).
+ *
  * Memory (one GPU): a checkpoint is the compacted particle state (six FP64
- * arrays, species and id of the occupied slots, with per-cell offsets and
- * counts) plus E and H; checkpoints are kept every K steps and at the end of
+ * arrays, or the FP32 form's offsets and momenta, species and id of the
+ * occupied slots, with per-cell offsets and counts) plus E and H; checkpoints are kept every K steps and at the end of
  * the run.  During the reverse sweep one segment of
  * K steps is recomputed from its checkpoint and each step's pre-push
  * particle state (written by the push kernel itself) and E^n, H^{n+1/2} are
  * recorded; the segment is then reversed step by step.  The J deposit of
- * this form is order-free (int64 fixed point), so the recompute is bitwise
+ * both forms is order-free (int64 fixed point), so the recompute is bitwise
  * the first forward run.
  */
 #include <algorithm>
@@ -33,16 +37,18 @@ void enq_e_periodic(pic_ctx *c);
 
 using namespace picx;
 
+/* E, H of one time level, in the form's own precision (8 or 4 bytes) */
 struct FieldSet {
-    double *f[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};   /* E, H */
+    void *f[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
 };
 
-/* compacted particle state: cell c's particles at [off[c], off[c+1]) */
+/* compacted particle state: cell c's particles at [off[c], off[c+1]);
+ * v[] holds FP64 positions/momenta, or the FP32 form's raw offsets/momenta */
 struct Compact {
     long P = 0;
     long *off = nullptr;      /* NC + 1 */
     int *count = nullptr;     /* NC */
-    double *v[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    void *v[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     unsigned char *sp = nullptr;
     int *id = nullptr;
 };
@@ -53,9 +59,10 @@ struct AdjointStore {
     long long anomalies = 0;          /* dropped + duplicated leavers in the forward run */
     std::vector<Compact> ck;          /* checkpoints at 0, K, 2K, ... and at nsteps */
     std::vector<FieldSet> ckf;
-    std::vector<PartPtrs> rec;        /* per-step particle records of one segment */
+    std::vector<PartPtrs> rec;        /* per-step particle records of one segment (FP64) */
+    std::vector<PartPtrs32> rec32;    /* the same for the FP32 form */
     std::vector<FieldSet> recf;       /* E^n, H^{n+1/2} of each recorded step */
-    double *Eend[3] = {nullptr, nullptr, nullptr};
+    void *Eend[3] = {nullptr, nullptr, nullptr};   /* E at the end of the segment */
     AdjLam lam{};
     LamParts lp{};
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -64,14 +71,25 @@ struct AdjointStore {
 
 namespace {
 
+size_t elem(const pic_ctx *c) { return c->fp32 ? sizeof(float) : sizeof(double); }
+
+/* live E (k < 3) and H (k >= 3) arrays of the context */
+void *live_field(pic_ctx *c, int k) {
+    if (c->fp32) return k < 3 ? (void *)c->f32.E[k] : (void *)c->f32.H[k - 3];
+    return c->f.f[PIC_EX + k];
+}
+
+int *live_count(pic_ctx *c) { return c->fp32 ? c->p32.count : c->p.count; }
+int live_cap(const pic_ctx *c) { return c->fp32 ? c->p32.cap : c->p.cap; }
+
 void free_fields(FieldSet &f) {
     for (int k = 0; k < 6; k++)
         if (f.f[k]) (void)hipFree(f.f[k]);
     f = FieldSet{};
 }
 
-int alloc_fields(FieldSet &f, long G) {
-    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&f.f[k], G * sizeof(double)));
+int alloc_fields(FieldSet &f, long G, size_t es) {
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&f.f[k], G * es));
     return PIC_SUCCESS;
 }
 
@@ -102,74 +120,94 @@ void free_compact(Compact &k) {
 
 int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     const long NC = c->NC;
+    const size_t es = elem(c);
     std::vector<int> cnt(NC);
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(cnt.data(), live_count(c), NC * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<long> off(NC + 1, 0);
-    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], live_cap(c));
     const long P = off[NC];
     if (dst.off && dst.P < P) free_compact(dst);
     if (!dst.off) {
         const size_t Pn = (size_t)std::max(P, 1L);
         HIPCHK(hipMalloc(&dst.off, (NC + 1) * sizeof(long)));
         HIPCHK(hipMalloc(&dst.count, NC * sizeof(int)));
-        for (int j = 0; j < 6; j++) HIPCHK(hipMalloc(&dst.v[j], Pn * sizeof(double)));
+        for (int j = 0; j < 6; j++) HIPCHK(hipMalloc(&dst.v[j], Pn * es));
         HIPCHK(hipMalloc(&dst.sp, Pn));
         HIPCHK(hipMalloc(&dst.id, Pn * sizeof(int)));
         dst.P = P;
     }
     HIPCHK(hipMemcpy(dst.off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpyAsync(dst.count, c->p.count, NC * sizeof(int), hipMemcpyDeviceToDevice,
+    HIPCHK(hipMemcpyAsync(dst.count, live_count(c), NC * sizeof(int), hipMemcpyDeviceToDevice,
                           c->stream));
     if (!f.f[0]) {
-        int rc = alloc_fields(f, c->g.G);
+        int rc = alloc_fields(f, c->g.G, es);
         if (rc) return rc;
     }
-    launch_gather_particles(NC, dst.off, c->p, dst.v, dst.sp, nullptr, dst.id, c->stream);
-    const size_t bytes = c->g.G * sizeof(double);
+    if (c->fp32) {
+        float *v[6];
+        for (int j = 0; j < 6; j++) v[j] = (float *)dst.v[j];
+        launch_gather_raw32(NC, dst.off, c->p32, v, dst.sp, dst.id, c->stream);
+    } else {
+        double *v[6];
+        for (int j = 0; j < 6; j++) v[j] = (double *)dst.v[j];
+        launch_gather_particles(NC, dst.off, c->p, v, dst.sp, nullptr, dst.id, c->stream);
+    }
+    const size_t bytes = c->g.G * es;
     for (int k = 0; k < 6; k++)
-        HIPCHK(hipMemcpyAsync(f.f[k], c->f.f[PIC_EX + k], bytes, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(f.f[k], live_field(c, k), bytes, hipMemcpyDeviceToDevice,
                               c->stream));
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
 int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
-    const double *v[6] = {src.v[0], src.v[1], src.v[2], src.v[3], src.v[4], src.v[5]};
-    launch_scatter_load(src.off, c->NC, v, src.sp, c->p, c->stream);
-    launch_scatter_ids(src.off, c->NC, src.id, c->p, c->stream);
-    const size_t bytes = c->g.G * sizeof(double);
+    if (c->fp32) {
+        const float *v[6];
+        for (int j = 0; j < 6; j++) v[j] = (const float *)src.v[j];
+        launch_scatter_load32(src.off, c->NC, v, src.sp, c->p32, c->stream);
+        launch_scatter_ids32(src.off, c->NC, src.id, c->p32, c->stream);
+    } else {
+        const double *v[6];
+        for (int j = 0; j < 6; j++) v[j] = (const double *)src.v[j];
+        launch_scatter_load(src.off, c->NC, v, src.sp, c->p, c->stream);
+        launch_scatter_ids(src.off, c->NC, src.id, c->p, c->stream);
+    }
+    const size_t bytes = c->g.G * elem(c);
     for (int k = 0; k < 6; k++)
-        HIPCHK(hipMemcpyAsync(c->f.f[PIC_EX + k], f.f[k], bytes, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(live_field(c, k), f.f[k], bytes, hipMemcpyDeviceToDevice,
                               c->stream));
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }
 
-/* one forward step that records the pre-push particle state into `snap`
- * (through the push kernel) and E^n, H^{n+1/2} into `f` */
-int recorded_step(pic_ctx *c, PartPtrs &snap, FieldSet &f) {
-    const size_t bytes = c->g.G * sizeof(double);
+/* one forward step that records the pre-push particle state into record r
+ * of the segment (through the push kernel) and E^n, H^{n+1/2} into `f` */
+int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
+    FieldSet &f = s->recf[r];
+    const size_t bytes = c->g.G * elem(c);
     launch_reset_status(c->d_st, c->stream);
     enq_h_half(c);
     for (int k = 0; k < 6; k++)
-        HIPCHK(hipMemcpyAsync(f.f[k], c->f.f[PIC_EX + k], bytes, hipMemcpyDeviceToDevice,
+        HIPCHK(hipMemcpyAsync(f.f[k], live_field(c, k), bytes, hipMemcpyDeviceToDevice,
                               c->stream));
-    c->rec_snap = snap;
+    if (c->fp32) c->rec_snap32 = s->rec32[r];
+    else c->rec_snap = s->rec[r];
     enq_push(c);
     c->rec_snap = PartPtrs{};
+    c->rec_snap32 = PartPtrs32{};
     enq_arrive(c);
     enq_fold(c);
     enq_h2(c);
     enq_e(c);
     enq_e_periodic(c);
     HIPCHK(hipGetLastError());
-    const int cap = c->p.cap;
+    const int cap = live_cap(c);
     int rc = finish_step(c);
     if (rc) return rc;
     /* the recompute sees the forward's own counts, which the live capacity
      * (the largest of the run) already holds */
-    if (c->p.cap != cap) return PIC_ERR_STATE;
+    if (live_cap(c) != cap) return PIC_ERR_STATE;
     return PIC_SUCCESS;
 }
 
@@ -187,6 +225,7 @@ void free_adjoint(pic_ctx *c) {
     for (auto &p : s->ck) free_compact(p);
     for (auto &f : s->ckf) free_fields(f);
     for (auto &p : s->rec) free_parts(p);
+    for (auto &p : s->rec32) free_parts(p);
     for (auto &f : s->recf) free_fields(f);
     free_lam(s);
     for (int k = 0; k < 3; k++)
@@ -204,7 +243,7 @@ int pic_get_particle_ids(pic_ctx *c, int *id, long cap, long *n_out) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!id) return PIC_ERR_INVALID;
-    if (c->fp32 || !c->p.id) {
+    if (c->fp32 ? !c->p32.id : !c->p.id) {
         const long n = pic_particle_count(c);
         if (n_out) *n_out = n;
         if (n > cap) return PIC_ERR_INVALID;
@@ -215,9 +254,11 @@ int pic_get_particle_ids(pic_ctx *c, int *id, long cap, long *n_out) {
     HIPCHK(hipStreamSynchronize(c->stream));
     const long NC = c->NC;
     std::vector<int> cnt(NC);
-    HIPCHK(hipMemcpy(cnt.data(), c->p.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    const int bcap = c->fp32 ? c->p32.cap : c->p.cap;
+    HIPCHK(hipMemcpy(cnt.data(), c->fp32 ? c->p32.count : c->p.count, NC * sizeof(int),
+                     hipMemcpyDeviceToHost));
     std::vector<long> off(NC + 1, 0);
-    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], c->p.cap);
+    for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], bcap);
     const long P = off[NC];
     if (n_out) *n_out = P;
     if (P > cap) return PIC_ERR_INVALID;
@@ -232,7 +273,10 @@ int pic_get_particle_ids(pic_ctx *c, int *id, long cap, long *n_out) {
     HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
     HIPCHK(hipMalloc(&d_id, Pn * sizeof(int)));
     HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
-    launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, d_id, c->stream);
+    if (c->fp32)
+        launch_gather_particles32(c->g, NC, d_off, c->p32, d, d_sp, d_cell, d_id, c->stream);
+    else
+        launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, d_id, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     if (P) HIPCHK(hipMemcpy(id, d_id, P * sizeof(int), hipMemcpyDeviceToHost));
@@ -304,7 +348,7 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
         HIPCHK(hipMalloc(&s->lam.H[k], G * sizeof(double)));
         HIPCHK(hipMalloc(&s->lam.J[k], G * sizeof(double)));
         HIPCHK(hipMalloc(&s->lam.Q[k], G * sizeof(double)));
-        HIPCHK(hipMalloc(&s->Eend[k], G * sizeof(double)));
+        HIPCHK(hipMalloc(&s->Eend[k], G * elem(c)));
         if ((rc = zero_async(s->lam.E[k], G * sizeof(double), st))) return rc;
         if ((rc = zero_async(s->lam.H[k], G * sizeof(double), st))) return rc;
     }
@@ -324,51 +368,93 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
                 HIPCHK(hipMemcpyAsync(k < 3 ? s->lam.E[k] : s->lam.H[k - 3], seed->field[k],
                                       G * sizeof(double), hipMemcpyHostToDevice, st));
     }
+    /* the FP32 sweep keeps the position adjoints in offset units: d/do = h d/dx */
+    if (c->fp32) launch_scale_lam(s->lp, g.hx, g.hy, g.hz, st);
     const int K = s->K;
-    if ((int)s->rec.size() != K) {
-        s->rec.resize(K);
-        s->recf.resize(K);
-    }
+    const int cap = live_cap(c);
+    s->recf.resize(K);
+    if (c->fp32) s->rec32.resize(K);
+    else s->rec.resize(K);
     for (int r = 0; r < K; r++) {
-        if (s->rec[r].p[0] && s->rec[r].cap != c->p.cap) free_parts(s->rec[r]);
-        if (!s->rec[r].p[0] && (rc = alloc_parts(s->rec[r], c->NC, c->p.cap, true, true)))
-            return rc;
-        if (!s->recf[r].f[0] && (rc = alloc_fields(s->recf[r], G))) return rc;
+        if (c->fp32) {
+            PartPtrs32 &q = s->rec32[r];
+            if (q.p[0] && q.cap != cap) free_parts(q);
+            if (!q.p[0] && (rc = alloc_parts(q, c->NC, cap, true, true))) return rc;
+        } else {
+            PartPtrs &q = s->rec[r];
+            if (q.p[0] && q.cap != cap) free_parts(q);
+            if (!q.p[0] && (rc = alloc_parts(q, c->NC, cap, true, true))) return rc;
+        }
+        if (!s->recf[r].f[0] && (rc = alloc_fields(s->recf[r], G, elem(c)))) return rc;
     }
 
     s->ms[1] = s->ms[2] = 0.0;
     const int nseg = (s->nsteps - 1) / K + 1;
+    /* with no seed on s_T the adjoint state stays exactly zero back to the
+     * last step whose energy weight is nonzero: segments after it are not
+     * recomputed */
+    int t_last = s->nsteps;
+    if (!seed) {
+        t_last = 0;
+        for (int t = s->nsteps; t >= 1 && weights; t--)
+            if (weights[t - 1] != 0.0) {
+                t_last = t;
+                break;
+            }
+    }
     for (int seg = nseg - 1; seg >= 0; seg--) {
+        if (seg * K >= t_last) continue;
         const int t0 = seg * K, t1 = std::min(t0 + K, s->nsteps);
         HIPCHK(hipEventRecord(s->ev[0], st));
         if ((rc = restore_state(c, s->ck[seg], s->ckf[seg]))) return rc;
         for (int t = t0 + 1; t <= t1; t++)
-            if ((rc = recorded_step(c, s->rec[t - t0 - 1], s->recf[t - t0 - 1]))) return rc;
+            if ((rc = recorded_step(c, s, t - t0 - 1))) return rc;
         for (int k = 0; k < 3; k++)
-            HIPCHK(hipMemcpyAsync(s->Eend[k], c->f.f[PIC_EX + k], G * sizeof(double),
+            HIPCHK(hipMemcpyAsync(s->Eend[k], live_field(c, k), G * elem(c),
                                   hipMemcpyDeviceToDevice, st));
         HIPCHK(hipEventRecord(s->ev[1], st));
         launch_reset_status(c->d_st, st);
         /* the recomputed segment must end on the stored state, bit for bit:
          * fields and per-cell counts */
         for (int k = 0; k < 6; k++)
-            launch_mismatch(c->f.f[PIC_EX + k], s->ckf[seg + 1].f[k], G, 8, c->d_st, st);
-        launch_mismatch(c->p.count, s->ck[seg + 1].count, c->NC, 4, c->d_st, st);
+            launch_mismatch(live_field(c, k), s->ckf[seg + 1].f[k], G, (int)elem(c), c->d_st,
+                            st);
+        launch_mismatch(live_count(c), s->ck[seg + 1].count, c->NC, 4, c->d_st, st);
         for (int t = t1; t > t0; t--) {
             const int r = t - t0 - 1;
-            const double *Et[3];
+            const void *Et[3];
             for (int k = 0; k < 3; k++) Et[k] = (t == t1) ? s->Eend[k] : s->recf[r + 1].f[k];
-            if (weights && weights[t - 1] != 0.0)
-                launch_adj_seed_energy(g, Et, weights[t - 1], s->lam, st);
+            if (weights && weights[t - 1] != 0.0) {
+                if (c->fp32) {
+                    const float *E32[3] = {(const float *)Et[0], (const float *)Et[1],
+                                           (const float *)Et[2]};
+                    launch_adj_seed_energy32(g, E32, weights[t - 1], s->lam, st);
+                } else {
+                    const double *E64[3] = {(const double *)Et[0], (const double *)Et[1],
+                                            (const double *)Et[2]};
+                    launch_adj_seed_energy(g, E64, weights[t - 1], s->lam, st);
+                }
+            }
             launch_adj_e_periodic(g, s->lam, st);
             launch_adj_curl_e(g, s->lam, st);
             launch_adj_fold_j(g, s->lam, st);
-            AdjFwd fwd;
-            for (int k = 0; k < 3; k++) {
-                fwd.E[k] = s->recf[r].f[k];
-                fwd.H[k] = s->recf[r].f[3 + k];
+            if (c->fp32) {
+                AdjFwd32 fwd;
+                for (int k = 0; k < 3; k++) {
+                    fwd.E[k] = (const float *)s->recf[r].f[k];
+                    fwd.H[k] = (const float *)s->recf[r].f[3 + k];
+                }
+                launch_adj_particles_f32(g, c->t, fwd, s->rec32[r], c->sp, s->lam, s->lp,
+                                         c->d_st, st);
+            } else {
+                AdjFwd fwd;
+                for (int k = 0; k < 3; k++) {
+                    fwd.E[k] = (const double *)s->recf[r].f[k];
+                    fwd.H[k] = (const double *)s->recf[r].f[3 + k];
+                }
+                launch_adj_particles_f64(g, c->t, fwd, s->rec[r], c->sp, s->lam, s->lp,
+                                         c->d_st, st);
             }
-            launch_adj_particles_f64(g, c->t, fwd, s->rec[r], c->sp, s->lam, s->lp, c->d_st, st);
             launch_adj_curl_h(g, s->lam, st);
         }
         HIPCHK(hipGetLastError());
@@ -387,6 +473,7 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
         s->ms[1] += a;
         s->ms[2] += b;
     }
+    if (c->fp32) launch_scale_lam(s->lp, 1.0 / g.hx, 1.0 / g.hy, 1.0 / g.hz, st);
     if (grad) {
         double *pg[6] = {grad->x, grad->y, grad->z, grad->px, grad->py, grad->pz};
         for (int k = 0; k < 6; k++)

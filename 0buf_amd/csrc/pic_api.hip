@@ -35,8 +35,7 @@ void free_parts(P &p) {
         if (p.p[k]) (void)hipFree(p.p[k]);
     if (p.sp) (void)hipFree(p.sp);
     if (p.count) (void)hipFree(p.count);
-    if constexpr (std::is_same<P, PartPtrs>::value)
-        if (p.id) (void)hipFree(p.id);
+    if (p.id) (void)hipFree(p.id);
     p = P{};
 }
 
@@ -50,13 +49,14 @@ int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id) {
     for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&p.p[k], n * sizeof(T)));
     HIPCHK(hipMalloc(&p.sp, n));
     if (with_count) HIPCHK(hipMalloc(&p.count, ncell * sizeof(int)));
-    if constexpr (std::is_same<P, PartPtrs>::value)
-        if (with_id) HIPCHK(hipMalloc(&p.id, n * sizeof(int)));
+    if (with_id) HIPCHK(hipMalloc(&p.id, n * sizeof(int)));
     return PIC_SUCCESS;
 }
 
 template int alloc_parts<PartPtrs>(PartPtrs &, long, int, bool, bool);
 template void free_parts<PartPtrs>(PartPtrs &);
+template int alloc_parts<PartPtrs32>(PartPtrs32 &, long, int, bool, bool);
+template void free_parts<PartPtrs32>(PartPtrs32 &);
 
 template <class Q>
 void free_queue(Q &q) {
@@ -66,8 +66,7 @@ void free_queue(Q &q) {
     if (q.dest) (void)hipFree(q.dest);
     if (q.slot) (void)hipFree(q.slot);
     if (q.count) (void)hipFree(q.count);
-    if constexpr (std::is_same<Q, QueuePtrs>::value)
-        if (q.id) (void)hipFree(q.id);
+    if (q.id) (void)hipFree(q.id);
     q = Q{};
 }
 
@@ -83,8 +82,7 @@ int alloc_queue(Q &q, const TileGeom &t, int qcap, bool with_id) {
     HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
     HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
     HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
-    if constexpr (std::is_same<Q, QueuePtrs>::value)
-        if (with_id) HIPCHK(hipMalloc(&q.id, n * sizeof(int)));
+    if (with_id) HIPCHK(hipMalloc(&q.id, n * sizeof(int)));
     return PIC_SUCCESS;
 }
 
@@ -166,13 +164,13 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
     else
         launch_scatter_load(d_off, NC, src, d_sp, parts, c->stream);
     int *d_id = nullptr;
-    if constexpr (std::is_same<P, PartPtrs>::value) {
-        if (c->diff) {
-            HIPCHK(hipMalloc(&d_id, Pn * sizeof(int)));
-            if (P_ > 0)
-                HIPCHK(hipMemcpy(d_id, sid.data(), P_ * sizeof(int), hipMemcpyHostToDevice));
+    if (c->diff) {
+        HIPCHK(hipMalloc(&d_id, Pn * sizeof(int)));
+        if (P_ > 0) HIPCHK(hipMemcpy(d_id, sid.data(), P_ * sizeof(int), hipMemcpyHostToDevice));
+        if constexpr (std::is_same<P, PartPtrs>::value)
             launch_scatter_ids(d_off, NC, d_id, parts, c->stream);
-        }
+        else
+            launch_scatter_ids32(d_off, NC, d_id, parts, c->stream);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -259,7 +257,7 @@ void enq_h_half(pic_ctx *c) {
 void enq_push(pic_ctx *c) {
     if (c->fp32) {
         launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->q32, c->sp, c->prm.departure_cap,
-                                c->d_st, c->stream);
+                                c->d_st, c->diff ? &c->rec_snap32 : nullptr, c->stream);
     } else if (c->diff) {
         DiffArgs d{};
         for (int k = 0; k < 3; k++) d.jfix[k] = c->d_jfix[k];
@@ -430,8 +428,6 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return PIC_ERR_INVALID;
     if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
     if (prm->precision != PIC_FP64 && prm->precision != PIC_FP32) return PIC_ERR_INVALID;
-    /* the differentiable form runs on the FP64 step */
-    if (prm->differentiable && prm->precision != PIC_FP64) return PIC_ERR_INVALID;
     if (prm->ghost_mode != 0 && prm->ghost_mode != 1) return PIC_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
@@ -481,7 +477,7 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
         if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
     }
     c->diff = prm->differentiable != 0;
-    for (int k = 0; c->diff && k < 3; k++) {
+    for (int k = 0; c->diff && !c->fp32 && k < 3; k++) {
         if ((e = hipMalloc(&c->d_jfix[k], g.G * sizeof(long long))) != hipSuccess)
             return fail(record_hip(e, "hipMalloc jfix"));
         if ((e = hipMemset(c->d_jfix[k], 0, g.G * sizeof(long long))) != hipSuccess)
@@ -622,10 +618,10 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
     if (maxc > cap) cap = auto_capacity(maxc);
     HIPCHK(hipSetDevice(c->dev));
-    int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, false)
+    int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, c->diff)
                      : alloc_parts(c->p, NC, cap, true, c->diff);
     if (rc) return rc;
-    rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), false)
+    rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), c->diff)
                  : alloc_queue(c->q, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
     HIPCHK(hipMemsetAsync(count_ptr(c), 0, NC * sizeof(int), c->stream));
@@ -902,7 +898,8 @@ int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, d
     HIPCHK(hipMalloc(&d_sp, Pn));
     HIPCHK(hipMalloc(&d_cell, Pn * sizeof(int)));
     HIPCHK(hipMemcpy(d_off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
-    if (c->fp32) launch_gather_particles32(c->g, NC, d_off, c->p32, d, d_sp, d_cell, c->stream);
+    if (c->fp32)
+        launch_gather_particles32(c->g, NC, d_off, c->p32, d, d_sp, d_cell, nullptr, c->stream);
     else launch_gather_particles(NC, d_off, c->p, d, d_sp, d_cell, nullptr, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));

@@ -42,12 +42,14 @@ struct pic_ctx {
     long long *d_rho = nullptr;   /* Gauss residual scratch */
     double *d_gres = nullptr;
     std::vector<double> gres_ref;
-    /* differentiable form: particle ids, fixed-point current, adjoint state */
+    /* differentiable form: particle ids, fixed-point current (the FP64 form's
+     * int64 J arrays; the FP32 form's J is fixed point already), adjoint state */
     bool diff = false;
     long long *d_jfix[3] = {nullptr, nullptr, nullptr};
     long n_ids = 0;
     AdjointStore *adj = nullptr;
     PartPtrs rec_snap{};      /* record target of the next push (adjoint recompute) */
+    PartPtrs32 rec_snap32{};
     SpeciesTable sp{};
     int slot_sort[PIC_MAX_SLOTS] = {0};
     int nslots = 0;

@@ -78,6 +78,7 @@ struct PartPtrs32 {
     unsigned char *sp;
     int *count;
     int cap;
+    int *id;                 /* differentiable form only */
 };
 
 struct QueuePtrs32 {
@@ -87,6 +88,7 @@ struct QueuePtrs32 {
     int *slot;
     int *count;
     int qcap;
+    int *id;                 /* differentiable form only */
 };
 
 struct QueuePtrs {
@@ -158,9 +160,11 @@ void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sor
                     double *partials, int nblocks, double *out27, hipStream_t s);
 
 /* FP32 form (k_particles_f32.hip) */
+/* diff: differentiable form (ids; snapshot of the pre-step state into
+ * *snap when snap->p[0] is set), null otherwise */
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             hipStream_t s);
+                             const PartPtrs32 *diff, hipStream_t s);
 void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
                        hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
@@ -169,7 +173,13 @@ void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);
 void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtrs32 to,
                            hipStream_t s);
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
-                               double *dst[6], unsigned char *dsp, int *dcell, hipStream_t s);
+                               double *dst[6], unsigned char *dsp, int *dcell, int *did,
+                               hipStream_t s);
+/* compacted copy of the raw FP32 state (offsets, momenta, species, ids) */
+void launch_gather_raw32(long ncell, const long *offsets, PartPtrs32 p, float *dst[6],
+                         unsigned char *dsp, int *did, hipStream_t s);
+void launch_scatter_ids32(const long *offsets, long ncell, const int *src, PartPtrs32 p,
+                          hipStream_t s);
 void launch_moments32(const Grid &g, long ncell, PartPtrs32 p, SpeciesTable sp,
                       const int *slot_sort, double *partials, int nblocks, double *out27,
                       hipStream_t s);
@@ -224,6 +234,19 @@ void launch_adj_particles_f64(const Grid &g, const TileGeom &t, AdjFwd fwd, Part
                               SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
                               hipStream_t s);
 void launch_adj_curl_h(const Grid &g, AdjLam l, hipStream_t s);
+
+/* FP32 form (k_adjoint32.hip): forward fields are floats, the particle
+ * adjoints by id are in offset units (d/do = h d/dx) during the sweep */
+struct AdjFwd32 {
+    const float *E[3];
+    const float *H[3];
+};
+void launch_adj_particles_f32(const Grid &g, const TileGeom &t, AdjFwd32 fwd, PartPtrs32 snap,
+                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
+                              hipStream_t s);
+void launch_scale_lam(LamParts lp, double sx, double sy, double sz, hipStream_t s);
+void launch_adj_seed_energy32(const Grid &g, const float *const E[3], double w, AdjLam l,
+                              hipStream_t s);
 void launch_copy_parts(long ncell, PartPtrs from, PartPtrs to, hipStream_t s);
 void launch_mismatch(const void *a, const void *b, long n, int elem_bytes, DevStatus *st,
                      hipStream_t s);

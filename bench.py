@@ -146,6 +146,58 @@ def adjoint_ratio(pic):
             "grad_norm": float(theta.grad.norm())}
 
 
+def adjoint_device(pic, idx, precision, first=150, last=200, ckpt_every=20, seed=12345):
+    """Gradient cost of the differentiable form at full size with the particles
+    drawn on the GPU: a plain forward of the workload's steps, then
+    forward-with-checkpoints + the reverse sweep of the loss sum_{t=first..last}
+    W_t (steps after `last` have a zero adjoint and are not recomputed)."""
+    import torch
+    cfg = pic.beam_plasma_config(idx)
+    n = cfg["n"]
+    T = cfg["steps"]
+    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"], precision=precision,
+                             differentiable=1, ghost_mode=1)
+
+    def ctx():
+        g = pic.Pic(params)
+        g.generate_device(cfg["species"], seed=seed)
+        if cfg["bx0"]:
+            g.load_field(pic.HX, np.full(g.G, cfg["bx0"]))
+        torch.cuda.synchronize()
+        return g
+    g = ctx()
+    P = g.particle_count()
+    t0 = time.perf_counter()
+    W = g.run(1, T)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    g.close()
+    g = ctx()
+    w = np.zeros(T)
+    w[first - 1:last] = 1.0
+    t0 = time.perf_counter()
+    Wr = g.forward_record(T, ckpt_every)
+    t1 = time.perf_counter()
+    gr = g.adjoint(weights=w)
+    t2 = time.perf_counter()
+    dev = g.adjoint_timing()
+    g.close()
+    pg = np.sqrt(sum(float(np.dot(gr[k], gr[k])) for k in ("x", "y", "z", "px", "py", "pz")))
+    return {"workload": f"{workload_name(cfg, idx, tuple(s.ppc for s in cfg['species']))}, "
+                        f"{T} steps, differentiable form "
+                        f"{'FP32 (FP64 gradient accumulation)' if precision else 'FP64'}, "
+                        f"ghost_mode 1, checkpoint every {ckpt_every}, loss = sum of field "
+                        f"energy over steps {first}..{last}",
+            "particles": P, "forward_s": t_fwd, "forward_record_s": t1 - t0,
+            "backward_s": t2 - t1, "gradient_s": t2 - t0, "ratio": (t2 - t0) / t_fwd,
+            "reversed_steps": last,
+            "backward_per_reversed_step_over_forward_step": (t2 - t1) / last / (t_fwd / T),
+            "device_ms": {"forward_with_checkpoints": dev[0], "recompute": dev[1],
+                          "adjoint_steps": dev[2]},
+            "record_matches_forward": bool(np.array_equal(W, Wr)),
+            "particle_grad_norm": pg}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,6 +207,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-config4", action="store_true",
+                    help="also time the config-4 gradient (FP64 and FP32 differentiable "
+                         "forms, particles drawn on the GPU)")
     ap.add_argument("--cpu-slab", type=int, default=8)
     ap.add_argument("--host-init", action="store_true",
                     help="generate the particles on the host and upload them instead of "
@@ -247,6 +302,14 @@ def main():
         adj = adjoint_ratio(pic)
         log(f"adjoint/forward ratio {adj['ratio']:.2f}")
 
+    adj4 = None
+    if args.adjoint_config4:
+        adj4 = {}
+        for prec, nm in ((pic.FP64, "fp64"), (pic.FP32, "fp32")):
+            log(f"timing the config 4 gradient, {nm}")
+            adj4[nm] = adjoint_device(pic, 4, prec)
+            log(f"config 4 {nm}: gradient/forward {adj4[nm]['ratio']:.2f}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a slab of the workload")
@@ -277,6 +340,7 @@ def main():
         "cpu_baseline": cpu,
         "fp32": fp32,
         "adjoint": adj,
+        "adjoint_config4": adj4,
         "stage_ms": stage_ms,
         "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,
         "particles_end": r64["P_end"],

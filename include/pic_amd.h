@@ -98,11 +98,14 @@ typedef struct pic_params {
                                 <= 0 disables the check */
     int bucket_capacity;     /* particle slots per cell; 0 = automatic */
     int device;              /* HIP device ordinal */
-    int differentiable;      /* 1: differentiable form (FP64 only): particles carry
-                                their load-order id, J is accumulated in int64
-                                fixed point (2^-48, order-free, so a recomputed
-                                step is bitwise the first run) and
-                                pic_forward_record / pic_adjoint are enabled */
+    int differentiable;      /* 1: differentiable form: particles carry their
+                                load-order id, J is accumulated in int64 fixed
+                                point (FP64 form 2^-48, FP32 form its usual
+                                2^-40; order-free, so a recomputed step is
+                                bitwise the first run) and pic_forward_record /
+                                pic_adjoint are enabled.  With PIC_FP32 the
+                                forward and the particle adjoint run in FP32,
+                                field adjoints and gradients accumulate in FP64 */
     int ghost_mode;          /* 0: the reference's periodic E ghosts (only Ex along z,
                                 Ey along x and z, Ez along x: gpu_plasma.h:486-505);
                                 1: additionally Ex and Ez along y, so every ghost
@@ -279,8 +282,9 @@ int pic_forward_record(pic_ctx *ctx, int nsteps, int ckpt_every, double *energy)
 /* after pic_forward_record: recompute each checkpoint segment (recording the
  * per-step particle and field state) and run the adjoint step by step back
  * to s_0.  weights: nsteps doubles (w_t = weights[t-1]) or null; seed: the
- * adjoint of s_nsteps or null; grad receives the adjoint of s_0.  The
- * context is left at s_nsteps. */
+ * adjoint of s_nsteps or null; grad receives the adjoint of s_0.  Without a
+ * seed, segments after the last nonzero weight are skipped (their adjoint is
+ * zero).  The context is left at s_nsteps. */
 int pic_adjoint(pic_ctx *ctx, const double *weights, const pic_adjoint_io *seed,
                 pic_adjoint_io *grad);
 /* device time of the last pic_forward_record and pic_adjoint calls (ms):

@@ -1,0 +1,176 @@
+"""Differentiable form at FP32 (the FP32 precision form's forward and particle
+adjoint, field adjoints and gradients accumulated in FP64) on the MI355X.
+
+Pins:
+  * transposes: with no particles the FP32 step is linear in (E, H) up to
+    FP32 rounding, so <seed, F(v)> = <F^T seed, v> and the Euler identity
+    <grad, v> = 2 loss hold to FP32 round-off (1e-5 relative);
+  * the FP32 adjoint against the FP64 adjoint (itself pinned by central
+    differences) on the same random state and seeds, over 1, 4 and 16 steps;
+  * the checkpointed FP32 recompute is bitwise the first forward run and
+    leaves the context at s_T;
+  * the beam-plasma loss gradient (drift, 8 amplitudes, Bx0) of a reduced
+    config-4 shape (64x16x16 cells, 48 ppc, checkpoint every 20) against the
+    FP64 adjoint on the same seeded particles.
+"""
+import importlib
+
+import numpy as np
+import pytest
+
+from helpers import load_workload
+
+pytestmark = pytest.mark.gpu
+
+PK = ("x", "y", "z", "px", "py", "pz")
+FK = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+
+
+def make_ctx(picmod, n, parts, fields, prec, ghost=1):
+    L = tuple(0.1 * v for v in n)
+    g = picmod.Pic(picmod.make_params(n, L, 0.05, precision=prec, differentiable=1,
+                                      ghost_mode=ghost))
+    for (sort, x, y, z, px, py, pz, m, qm) in parts:
+        g.load_particles(sort, x, y, z, px, py, pz, m, qm)
+    for i, a in fields.items():
+        g.load_field(i, a)
+    return g
+
+
+def rand_parts(rng, n, N, pscale):
+    L = [0.1 * v for v in n]
+    out = []
+    for sort, qm, m in ((1, -1.0, -0.02), (0, 1 / 1836., 0.02), (2, -1.0, -0.005)):
+        x = [rng.uniform(0.001, L[d] - 0.001, N) for d in range(3)]
+        p = [rng.normal(0, pscale, N) for _ in range(3)]
+        out.append([sort, x[0], x[1], x[2], p[0], p[1], p[2], m, qm])
+    return out
+
+
+def phi_fields(g, seed, weights, W):
+    phi = float(np.dot(weights, W)) if weights is not None else 0.0
+    for i, k in enumerate(FK):
+        if k in seed:
+            phi += float(np.dot(seed[k], g.get_field(i)))
+    return phi
+
+
+@pytest.mark.parametrize("ghost", [0, 1])
+def test_fp32_field_transpose(picmod, ghost):
+    rng = np.random.default_rng(21)
+    n = (8, 6, 5)
+    G = 10 * 8 * 7
+    v = {i: rng.normal(size=G) for i in range(6)}
+    seed = {k: rng.normal(size=G) for k in FK}
+    g = make_ctx(picmod, n, [], v, picmod.FP32, ghost)
+    W = g.forward_record(3, 2)
+    phi = phi_fields(g, seed, None, W)
+    gr = g.adjoint(seed=seed)
+    g.close()
+    # the FP32 form holds the loaded fields rounded to FP32
+    v32 = {i: v[i].astype(np.float32).astype(np.float64) for i in range(6)}
+    dot = sum(float(np.dot(gr[FK[i]], v32[i])) for i in range(6))
+    assert abs(phi - dot) <= 1e-5 * abs(phi)
+    w = rng.uniform(0.5, 1.5, 3)
+    g = make_ctx(picmod, n, [], v, picmod.FP32, ghost)
+    W = g.forward_record(3, 2)
+    gr = g.adjoint(weights=w)
+    g.close()
+    dot = sum(float(np.dot(gr[FK[i]], v32[i])) for i in range(6))
+    assert abs(2 * np.dot(w, W) - dot) <= 1e-5 * abs(dot)
+
+
+@pytest.mark.parametrize("T,tol", [(1, 1e-4), (4, 1e-4), (16, 1e-3)])
+def test_fp32_adjoint_against_fp64_adjoint(picmod, T, tol):
+    """Same random state, energy weights and seeds on s_T; the two adjoints
+    are compared component by component (fields and particle adjoints by
+    id), normwise over each.  Their difference is the FP32 rounding of the
+    forward trajectory and of the adjoint arithmetic."""
+    rng = np.random.default_rng(300 + T)
+    n = (8, 8, 8)
+    G = 10 * 10 * 10
+    parts = rand_parts(rng, n, 3000, 0.3)
+    f0 = {i: 0.05 * rng.normal(size=G) for i in range(6)}
+    grads = {}
+    for prec in (picmod.FP64, picmod.FP32):
+        g = make_ctx(picmod, n, parts, f0, prec)
+        nid = g.id_count()
+        r2 = np.random.default_rng(7)
+        seed = {k: 0.1 * r2.normal(size=G) for k in FK}
+        for k in PK:
+            seed[k] = r2.normal(size=nid)
+        w = r2.uniform(0.5, 1.5, T)
+        g.forward_record(T, 3)
+        grads[prec] = g.adjoint(weights=w, seed=seed)
+        g.close()
+    worst = 0.0
+    for k in FK + PK:
+        a, b = grads[picmod.FP64][k], grads[picmod.FP32][k]
+        rel = np.linalg.norm(b - a) / np.linalg.norm(a)
+        print(f"T {T} {k}: normwise FP32 vs FP64 {rel:.2e}")
+        worst = max(worst, rel)
+    assert worst <= tol
+
+
+def test_fp32_recompute_is_bitwise_and_restores(picmod):
+    _, parts = picmod.generate_workload(0)
+    prm = picmod.make_params((32, 8, 8), (3.2, 0.8, 0.8), 0.05, precision=picmod.FP32,
+                             differentiable=1)
+    a = picmod.Pic(prm)
+    load_workload(a, parts)
+    Wa = a.run(1, 25)
+    b = picmod.Pic(prm)
+    load_workload(b, parts)
+    Wb = b.forward_record(25, 10)
+    assert np.array_equal(Wa, Wb)
+    b.adjoint(weights=np.ones(25))     # raises if a recomputed segment differs
+    for i in range(6):
+        assert np.array_equal(a.get_field(i), b.get_field(i))
+    assert np.array_equal(a.cell_counts(), b.cell_counts())
+    assert np.array_equal(a.run(26, 5), b.run(26, 5))
+    for h in (a, b):
+        h.close()
+
+
+def test_trailing_zero_weights_skip_segments(picmod):
+    """Segments after the last nonzero weight are not recomputed; the
+    gradient equals the one of a run that stops at that step."""
+    _, parts = picmod.generate_workload(0)
+    prm = picmod.make_params((32, 8, 8), (3.2, 0.8, 0.8), 0.05, differentiable=1)
+    w = np.zeros(30)
+    w[9:17] = 1.0
+    a = picmod.Pic(prm)
+    load_workload(a, parts)
+    a.forward_record(30, 10)
+    ga = a.adjoint(weights=w)
+    b = picmod.Pic(prm)
+    load_workload(b, parts)
+    b.forward_record(20, 10)
+    gb = b.adjoint(weights=w[:20])
+    # FP64 atomics of the field adjoints leave the last bits order-dependent
+    for k in FK + PK:
+        assert np.linalg.norm(ga[k] - gb[k]) <= 1e-12 * np.linalg.norm(gb[k]), k
+    a.close()
+    b.close()
+
+
+def test_config4_shape_fp32_gradient_against_fp64(picmod):
+    """Config-4 workload (Gaussian beam profile, 48 ppc, checkpoint every 20)
+    on a 64x16x16 grid, 200 steps, loss = field energy over steps 150..200:
+    the FP32 form's gradient with respect to (drift, a_1..a_8, Bx0) against
+    the FP64 adjoint on the same seeded particles, normwise over the ten
+    parameters."""
+    pytest.importorskip("torch")
+    ag = importlib.import_module("0buf_amd.autograd")
+    cfg = dict(picmod.beam_plasma_config(4))
+    cfg["n"] = (64, 16, 16)
+    cfg["l"] = tuple(0.1 * v for v in cfg["n"])
+    r = ag.precision_compare(cfg, nsteps=200)
+    g64, g32 = r[picmod.FP64]["grad"], r[picmod.FP32]["grad"]
+    rel = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
+    print("FP64", g64)
+    print("FP32", g32)
+    print(f"normwise {rel:.2e}, loss FP64 {r[picmod.FP64]['loss']:.9e} "
+          f"FP32 {r[picmod.FP32]['loss']:.9e}")
+    assert abs(r[picmod.FP32]["loss"] / r[picmod.FP64]["loss"] - 1) <= 1e-3
+    assert rel <= 1e-2
