@@ -36,6 +36,7 @@ HIP_SOURCES = {
     "k_gauss.hip": ["-ffp-contract=off"],
     "k_adjoint.hip": ["-ffp-contract=off"],
     "k_init.hip": ["-ffp-contract=off"],
+    "k_records.hip": ["-ffp-contract=off"],
     "k_adjoint32.hip": ["-ffp-contract=fast"],
     "pic_api.hip": ["-ffp-contract=off"],
     "pic_adjoint.hip": ["-ffp-contract=off"],

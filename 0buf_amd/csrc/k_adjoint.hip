@@ -219,7 +219,8 @@ struct AdjArgs {
     DivConst dc;
     TileGeom t;
     AdjFwd fwd;
-    PartPtrs p;          /* snapshot of s_{t-1} */
+    PartPtrs p;          /* compact record of s_{t-1} */
+    const long *snap_off;
     SpeciesTable sp;
     AdjLam lam;
     LamParts lp;
@@ -396,13 +397,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
             if (sOff[mid] <= fidx) lo = mid; else hi = mid;
         }
         const int c = lo;
-        const int j = fidx - sOff[c];
         Geo G;
         G.ca = c / (TY * TZ);
         G.cb = (c / TZ) % TY;
         G.cc = c % TZ;
         const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
-        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+        const long pidx = a.snap_off[tile] + fidx;    /* compact record */
         G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
         G.x0 = (double)gi * hx;
         G.y0 = (double)gl * hy;
@@ -638,14 +638,15 @@ void launch_adj_fold_j(const Grid &g, AdjLam l, hipStream_t s) {
 }
 
 void launch_adj_particles_f64(const Grid &g, const TileGeom &t, AdjFwd fwd, PartPtrs snap,
-                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
-                              hipStream_t s) {
+                              const long *snap_off, SpeciesTable sp, AdjLam l, LamParts lp,
+                              DevStatus *st, hipStream_t s) {
     AdjArgs a;
     a.g = g;
     a.dc = make_divconst(g);
     a.t = t;
     a.fwd = fwd;
     a.p = snap;
+    a.snap_off = snap_off;
     a.sp = sp;
     a.lam = l;
     a.lp = lp;

@@ -25,7 +25,8 @@ struct AdjArgs32 {
     Grid g;
     TileGeom t;
     AdjFwd32 fwd;
-    PartPtrs32 p;          /* snapshot of s_{t-1} */
+    PartPtrs32 p;          /* compact record of s_{t-1} */
+    const long *snap_off;
     float qm[PIC_MAX_SLOTS];
     float kj[PIC_MAX_SLOTS][3];   /* m h_d / tau: J per unit cell displacement along d */
     float tdh[3], hh[3], tau;
@@ -189,10 +190,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
             if (sOff[mid] <= fidx) blo = mid; else bhi = mid;
         }
         const int c = blo;
-        const int j = fidx - sOff[c];
         const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
+        const long pidx = a.snap_off[tile] + fidx;    /* compact record */
         const float o[3] = {a.p.p[0][pidx], a.p.p[1][pidx], a.p.p[2][pidx]};
         const float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
         const unsigned char s = a.p.sp[pidx];
@@ -366,13 +365,14 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 }  // namespace
 
 void launch_adj_particles_f32(const Grid &g, const TileGeom &t, AdjFwd32 fwd, PartPtrs32 snap,
-                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
-                              hipStream_t s) {
+                              const long *snap_off, SpeciesTable sp, AdjLam l, LamParts lp,
+                              DevStatus *st, hipStream_t s) {
     AdjArgs32 a;
     a.g = g;
     a.t = t;
     a.fwd = fwd;
     a.p = snap;
+    a.snap_off = snap_off;
     const double h[3] = {g.hx, g.hy, g.hz};
     for (int k = 0; k < PIC_MAX_SLOTS; k++) {
         a.qm[k] = (float)sp.q_m[k];

@@ -46,6 +46,8 @@ struct PushArgs32 {
     int dep_cap;
     DevStatus *st;
     PartPtrs32 snap;               /* differentiable form: record of the pre-step state */
+    const long *snap_off;          /* record offset of each tile */
+    long snap_room;
     int snap_on;
 };
 
@@ -197,15 +199,16 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
         const unsigned char s = a.p.sp[pidx];
         const int pid = DIFF ? a.p.id[pidx] : 0;
-        if (snap) {
-            a.snap.p[0][pidx] = ox;
-            a.snap.p[1][pidx] = oy;
-            a.snap.p[2][pidx] = oz;
-            a.snap.p[3][pidx] = pu;
-            a.snap.p[4][pidx] = pv;
-            a.snap.p[5][pidx] = pw;
-            a.snap.sp[pidx] = s;
-            a.snap.id[pidx] = pid;
+        const long r = snap ? a.snap_off[tile] + fidx : 0;
+        if (snap && r < a.snap_room) {
+            a.snap.p[0][r] = ox;
+            a.snap.p[1][r] = oy;
+            a.snap.p[2][r] = oz;
+            a.snap.p[3][r] = pu;
+            a.snap.p[4][r] = pv;
+            a.snap.p[5][r] = pw;
+            a.snap.sp[r] = s;
+            a.snap.id[r] = pid;
         }
 
         /* gather (weights as in k_push_math32.h), Boris push, deposit */
@@ -516,7 +519,8 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             const PartPtrs32 *diff, hipStream_t s) {
+                             const PartPtrs32 *diff, const long *snap_off, long snap_room,
+                             hipStream_t s) {
     PushArgs32 a;
     a.g = g;
     a.t = t;
@@ -537,6 +541,8 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.st = st;
     a.snap = diff ? *diff : PartPtrs32{};
     a.snap_on = diff && diff->p[0] != nullptr;
+    a.snap_off = snap_off;
+    a.snap_room = snap_room;
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);

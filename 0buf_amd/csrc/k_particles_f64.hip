@@ -226,15 +226,16 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         const unsigned char s = a.p.sp[pidx];
         const int pid = DIFF ? a.p.id[pidx] : 0;
         const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
-        if (snap) {
-            a.d.snap.p[0][pidx] = x;
-            a.d.snap.p[1][pidx] = y;
-            a.d.snap.p[2][pidx] = z;
-            a.d.snap.p[3][pidx] = pu;
-            a.d.snap.p[4][pidx] = pv;
-            a.d.snap.p[5][pidx] = pw;
-            a.d.snap.sp[pidx] = s;
-            a.d.snap.id[pidx] = pid;
+        const long r = snap ? a.d.snap_off[tile] + fidx : 0;
+        if (snap && r < a.d.snap_room) {
+            a.d.snap.p[0][r] = x;
+            a.d.snap.p[1][r] = y;
+            a.d.snap.p[2][r] = z;
+            a.d.snap.p[3][r] = pu;
+            a.d.snap.p[4][r] = pv;
+            a.d.snap.p[5][r] = pw;
+            a.d.snap.sp[r] = s;
+            a.d.snap.id[r] = pid;
         }
 
         const double frx = divc(x - G.x0, hx, a.dc.rhx);

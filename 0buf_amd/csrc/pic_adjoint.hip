@@ -13,14 +13,16 @@
  * occupied slots, with per-cell offsets and counts) plus E and H; checkpoints are kept every K steps and at the end of
  * the run.  During the reverse sweep one segment of
  * K steps is recomputed from its checkpoint and each step's pre-push
- * particle state (written by the push kernel itself) and E^n, H^{n+1/2} are
- * recorded; the segment is then reversed step by step.  The J deposit of
+ * particle state (written by the push kernel itself into a compact record,
+ * tile by tile in the push's order) and E^n, H^{n+1/2} are recorded; the
+ * segment is then reversed step by step.  The J deposit of
  * both forms is order-free (int64 fixed point), so the recompute is bitwise
  * the first forward run.
  */
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "pic_ctx.h"
@@ -59,8 +61,13 @@ struct AdjointStore {
     long long anomalies = 0;          /* dropped + duplicated leavers in the forward run */
     std::vector<Compact> ck;          /* checkpoints at 0, K, 2K, ... and at nsteps */
     std::vector<FieldSet> ckf;
-    std::vector<PartPtrs> rec;        /* per-step particle records of one segment (FP64) */
-    std::vector<PartPtrs32> rec32;    /* the same for the FP32 form */
+    /* per-step particle records of one segment, compact: tile t's particles
+     * in the push's flat order at recoff[r][t] (FP64 or FP32 form) */
+    std::vector<PartPtrs> rec;
+    std::vector<PartPtrs32> rec32;
+    std::vector<long *> recoff;       /* ntiles + 1 offsets per record */
+    long room = 0;                    /* particles a record holds */
+    long *tile_sum = nullptr;         /* scratch of the offset scan */
     std::vector<FieldSet> recf;       /* E^n, H^{n+1/2} of each recorded step */
     void *Eend[3] = {nullptr, nullptr, nullptr};   /* E at the end of the segment */
     AdjLam lam{};
@@ -191,11 +198,17 @@ int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
     for (int k = 0; k < 6; k++)
         HIPCHK(hipMemcpyAsync(f.f[k], live_field(c, k), bytes, hipMemcpyDeviceToDevice,
                               c->stream));
+    launch_record_offsets(c->g, c->t, live_count(c), live_cap(c), s->tile_sum, s->recoff[r],
+                          s->room, c->d_st, c->stream);
     if (c->fp32) c->rec_snap32 = s->rec32[r];
     else c->rec_snap = s->rec[r];
+    c->rec_off = s->recoff[r];
+    c->rec_room = s->room;
     enq_push(c);
     c->rec_snap = PartPtrs{};
     c->rec_snap32 = PartPtrs32{};
+    c->rec_off = nullptr;
+    c->rec_room = 0;
     enq_arrive(c);
     enq_fold(c);
     enq_h2(c);
@@ -208,6 +221,20 @@ int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
     /* the recompute sees the forward's own counts, which the live capacity
      * (the largest of the run) already holds */
     if (live_cap(c) != cap) return PIC_ERR_STATE;
+    return PIC_SUCCESS;
+}
+
+/* a compact record of `room` particles with per-cell counts */
+template <class P>
+int alloc_record(P &r, long ncell, long room, int cap) {
+    using T = std::remove_pointer_t<std::remove_reference_t<decltype(r.p[0])>>;
+    r = P{};
+    r.cap = cap;
+    const size_t n = (size_t)std::max(room, 1L);
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&r.p[k], n * sizeof(T)));
+    HIPCHK(hipMalloc(&r.sp, n));
+    HIPCHK(hipMalloc(&r.id, n * sizeof(int)));
+    HIPCHK(hipMalloc(&r.count, ncell * sizeof(int)));
     return PIC_SUCCESS;
 }
 
@@ -226,6 +253,9 @@ void free_adjoint(pic_ctx *c) {
     for (auto &f : s->ckf) free_fields(f);
     for (auto &p : s->rec) free_parts(p);
     for (auto &p : s->rec32) free_parts(p);
+    for (auto *o : s->recoff)
+        if (o) (void)hipFree(o);
+    if (s->tile_sum) (void)hipFree(s->tile_sum);
     for (auto &f : s->recf) free_fields(f);
     free_lam(s);
     for (int k = 0; k < 3; k++)
@@ -372,19 +402,31 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
     if (c->fp32) launch_scale_lam(s->lp, g.hx, g.hy, g.hz, st);
     const int K = s->K;
     const int cap = live_cap(c);
+    /* records hold the largest particle count of the run (constant without
+     * dropped or duplicated leavers) */
+    long room = 0;
+    for (const Compact &k : s->ck) room = std::max(room, k.P);
+    if (room != s->room) {
+        for (auto &p : s->rec) free_parts(p);
+        for (auto &p : s->rec32) free_parts(p);
+        s->room = room;
+    }
     s->recf.resize(K);
+    s->recoff.resize(K, nullptr);
     if (c->fp32) s->rec32.resize(K);
     else s->rec.resize(K);
+    if (!s->tile_sum) HIPCHK(hipMalloc(&s->tile_sum, c->t.ntiles * sizeof(long)));
     for (int r = 0; r < K; r++) {
         if (c->fp32) {
             PartPtrs32 &q = s->rec32[r];
             if (q.p[0] && q.cap != cap) free_parts(q);
-            if (!q.p[0] && (rc = alloc_parts(q, c->NC, cap, true, true))) return rc;
+            if (!q.p[0] && (rc = alloc_record(q, c->NC, room, cap))) return rc;
         } else {
             PartPtrs &q = s->rec[r];
             if (q.p[0] && q.cap != cap) free_parts(q);
-            if (!q.p[0] && (rc = alloc_parts(q, c->NC, cap, true, true))) return rc;
+            if (!q.p[0] && (rc = alloc_record(q, c->NC, room, cap))) return rc;
         }
+        if (!s->recoff[r]) HIPCHK(hipMalloc(&s->recoff[r], (c->t.ntiles + 1) * sizeof(long)));
         if (!s->recf[r].f[0] && (rc = alloc_fields(s->recf[r], G, elem(c)))) return rc;
     }
 
@@ -444,16 +486,16 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
                     fwd.E[k] = (const float *)s->recf[r].f[k];
                     fwd.H[k] = (const float *)s->recf[r].f[3 + k];
                 }
-                launch_adj_particles_f32(g, c->t, fwd, s->rec32[r], c->sp, s->lam, s->lp,
-                                         c->d_st, st);
+                launch_adj_particles_f32(g, c->t, fwd, s->rec32[r], s->recoff[r], c->sp, s->lam,
+                                         s->lp, c->d_st, st);
             } else {
                 AdjFwd fwd;
                 for (int k = 0; k < 3; k++) {
                     fwd.E[k] = (const double *)s->recf[r].f[k];
                     fwd.H[k] = (const double *)s->recf[r].f[3 + k];
                 }
-                launch_adj_particles_f64(g, c->t, fwd, s->rec[r], c->sp, s->lam, s->lp,
-                                         c->d_st, st);
+                launch_adj_particles_f64(g, c->t, fwd, s->rec[r], s->recoff[r], c->sp, s->lam,
+                                         s->lp, c->d_st, st);
             }
             launch_adj_curl_h(g, s->lam, st);
         }

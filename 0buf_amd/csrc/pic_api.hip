@@ -257,11 +257,14 @@ void enq_h_half(pic_ctx *c) {
 void enq_push(pic_ctx *c) {
     if (c->fp32) {
         launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->q32, c->sp, c->prm.departure_cap,
-                                c->d_st, c->diff ? &c->rec_snap32 : nullptr, c->stream);
+                                c->d_st, c->diff ? &c->rec_snap32 : nullptr, c->rec_off,
+                                c->rec_room, c->stream);
     } else if (c->diff) {
         DiffArgs d{};
         for (int k = 0; k < 3; k++) d.jfix[k] = c->d_jfix[k];
         d.snap = c->rec_snap;
+        d.snap_off = c->rec_off;
+        d.snap_room = c->rec_room;
         launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap,
                                 c->d_st, &d, c->stream);
         launch_jfix_to_f64(c->g, c->f, d, c->stream);
@@ -358,6 +361,10 @@ int finish_step(pic_ctx *c) {
     }
     if (st.flags & PIC_F_QUEUE_OVERFLOW) return PIC_ERR_QUEUE_OVERFLOW;
     if (st.flags & PIC_F_RANGE) return PIC_ERR_CELL_RANGE;
+    if (st.flags & PIC_F_RECORD) {
+        snprintf(g_hip_err, sizeof(g_hip_err), "step record holds fewer particles than the box");
+        return PIC_ERR_STATE;
+    }
     if (st.flags & PIC_F_TOO_MANY) return PIC_ERR_TOO_MANY_PARTICLES;
     return PIC_SUCCESS;
 }
@@ -639,7 +646,7 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     for (int s = 0; s < nspec; s++) {
         const pic_species_spec &sp = specs[s];
         /* getMassCharge (maxwell.cu:606-619): weight = q n / ppc */
-        const double m = sp.charge * sp.density / (double)(sp.ppc > 0 ? sp.ppc : 1);
+        const double m = pic_species_weight(&c->prm, &sp);
         int slot = -1;
         for (int q = 0; q < c->nslots; q++)
             if (c->slot_sort[q] == sp.sort && c->sp.m[q] == m && c->sp.q_m[q] == sp.q_m) slot = q;

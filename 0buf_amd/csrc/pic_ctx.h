@@ -50,6 +50,8 @@ struct pic_ctx {
     AdjointStore *adj = nullptr;
     PartPtrs rec_snap{};      /* record target of the next push (adjoint recompute) */
     PartPtrs32 rec_snap32{};
+    const long *rec_off = nullptr;   /* tile offsets of that record */
+    long rec_room = 0;               /* entries that record holds */
     SpeciesTable sp{};
     int slot_sort[PIC_MAX_SLOTS] = {0};
     int nslots = 0;

@@ -143,9 +143,13 @@ PIC_HD inline void draw_particle(double lx, const pic_species_spec *s, pic_rng *
     o[5] = uz / vb0;
 }
 
-/* per-(l,k) particle count of a species (transverse Gaussian occupancy for
- * profile 1: floor(ppc exp(-r^2 / 2 sigma^2) + 1/2), r from the box axis in
- * cells); host only */
+/* per-(l,k) particle count of a species; host only.  Profile 1 (transverse
+ * Gaussian): floor(A exp(-r^2 / 2 sigma^2) + 1/2), r from the box axis in
+ * cells, with the axis count A = ppc ny nz / sum exp(...) so that ppc is the
+ * mean count per cell */
 void pic_species_cell_counts(const pic_params *g, const pic_species_spec *s, int *counts_lk);
+/* particle weight m = q n / (count per cell), the axis count for profile 1
+ * (the density n is the axis density); host only */
+double pic_species_weight(const pic_params *g, const pic_species_spec *s);
 
 #endif

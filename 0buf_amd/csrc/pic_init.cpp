@@ -13,6 +13,7 @@
 #include "pic_amd.h"
 #include "pic_draw.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <thread>
@@ -39,28 +40,52 @@ double pic_rng_normal(pic_rng *r, double a, double b) { return wh_normal(r, a, b
 
 namespace {
 
-/* transverse Gaussian occupancy of cell (l,k): peak count at the axis */
-inline int cell_count(const pic_params *g, const pic_species_spec *s, int l, int k) {
-    if (s->profile == 0) return s->ppc;
+/* transverse Gaussian shape of cell (l,k), 1 on the box axis */
+inline double profile_shape(const pic_params *g, const pic_species_spec *s, int l, int k) {
     double yc = (l + 0.5) - 0.5 * g->ny, zc = (k + 0.5) - 0.5 * g->nz;
     double sg = s->sigma_cells > 0 ? s->sigma_cells : 1.0;
-    double f = std::exp(-(yc * yc + zc * zc) / (2.0 * sg * sg));
-    return (int)std::floor(s->ppc * f + 0.5);
+    return std::exp(-(yc * yc + zc * zc) / (2.0 * sg * sg));
+}
+
+/* count on the axis of a profiled species: ppc is the mean over the cross
+ * section, so the axis carries ppc * ny nz / sum(shape) */
+inline double axis_count(const pic_params *g, const pic_species_spec *s) {
+    double sum = 0.0;
+    for (int l = 0; l < g->ny; l++)
+        for (int k = 0; k < g->nz; k++) sum += profile_shape(g, s, l, k);
+    return s->ppc * ((double)g->ny * g->nz) / sum;
+}
+
+/* per-(l,k) counts of one x-plane */
+std::vector<int> plane_counts(const pic_params *g, const pic_species_spec *s) {
+    std::vector<int> c((size_t)g->ny * g->nz, s->ppc);
+    if (s->profile == 0) return c;
+    const double peak = axis_count(g, s);
+    for (int l = 0; l < g->ny; l++)
+        for (int k = 0; k < g->nz; k++)
+            c[(size_t)l * g->nz + k] = (int)std::floor(peak * profile_shape(g, s, l, k) + 0.5);
+    return c;
 }
 
 }  // namespace
 
 void pic_species_cell_counts(const pic_params *g, const pic_species_spec *s, int *counts_lk) {
-    for (int l = 0; l < g->ny; l++)
-        for (int k = 0; k < g->nz; k++) counts_lk[(size_t)l * g->nz + k] = cell_count(g, s, l, k);
+    const std::vector<int> c = plane_counts(g, s);
+    std::copy(c.begin(), c.end(), counts_lk);
+}
+
+double pic_species_weight(const pic_params *g, const pic_species_spec *s) {
+    /* getMassCharge (maxwell.cu:606-619): weight = q n / ppc; a profiled
+     * species carries its density n on the axis, with the axis count */
+    const double per_cell = s->profile == 0 ? (double)s->ppc : axis_count(g, s);
+    return s->charge * s->density / (per_cell > 0 ? per_cell : 1.0);
 }
 
 extern "C" {
 
 long pic_species_count(const pic_params *g, const pic_species_spec *s) {
     long n = 0;
-    for (int l = 0; l < g->ny; l++)
-        for (int k = 0; k < g->nz; k++) n += cell_count(g, s, l, k);
+    for (int c : plane_counts(g, s)) n += c;
     return n * g->nx;
 }
 
@@ -73,18 +98,15 @@ int pic_generate_species(const pic_params *g, const pic_species_spec *s, pic_rng
     if (n_out) *n_out = total;
     if (total > cap) return PIC_ERR_INVALID;
     double hx = g->lx / (double)g->nx, hy = g->ly / (double)g->ny, hz = g->lz / (double)g->nz;
-    /* getMassCharge (maxwell.cu:606-619): weight = q n / ppc; for a profiled
-     * species the peak density is carried by the peak count. */
-    if (m_out) *m_out = s->charge * s->density / (double)(s->ppc > 0 ? s->ppc : 1);
+    if (m_out) *m_out = pic_species_weight(g, s);
     /* per-(l,k) counts and offsets of each x-plane */
+    const std::vector<int> cnt_lk = plane_counts(g, s);
     std::vector<long> off((size_t)g->ny * g->nz + 1, 0);
-    for (int l = 0; l < g->ny; l++)
-        for (int k = 0; k < g->nz; k++)
-            off[(size_t)l * g->nz + k + 1] = off[(size_t)l * g->nz + k] + cell_count(g, s, l, k);
+    for (size_t lk = 0; lk < cnt_lk.size(); lk++) off[lk + 1] = off[lk] + cnt_lk[lk];
     long plane = off.back();
     auto gen_cell = [&](pic_rng *r, int i, int l, int k) {
         long base = (long)i * plane + off[(size_t)l * g->nz + k];
-        int cnt = cell_count(g, s, l, k);
+        int cnt = cnt_lk[(size_t)l * g->nz + k];
         double o[6];
         for (int j = 0; j < cnt; j++) {
             draw_particle(g->lx, s, r, i, l, k, hx, hy, hz, o, nullptr);

@@ -28,7 +28,8 @@ enum {
     PIC_F_QUEUE_OVERFLOW = 2,
     PIC_F_BUCKET_OVERFLOW = 4,
     PIC_F_RANGE = 8,
-    PIC_F_RECOMPUTE = 16     /* differentiable form: recompute differs from the first run */
+    PIC_F_RECOMPUTE = 16,    /* differentiable form: recompute differs from the first run */
+    PIC_F_RECORD = 32        /* differentiable form: a step record ran out of room */
 };
 
 struct DevStatus {
@@ -134,11 +135,14 @@ TileGeom make_tile_geom(int Nx, int Ny, int Nz);
  * point with resolution 2^-48 (order-free, so a recomputed step is bitwise
  * equal to the first run), converted to the FP64 J arrays before the fold;
  * when snap.p[0] is set the push also writes the pre-step particle state into
- * the record buckets `snap` (same capacity as the live buckets). */
+ * the compact record `snap`: tile t's particles, in the push's flat order,
+ * at snap_off[t] + (flat index), and each cell's count in snap.count. */
 #define PIC_J_SCALE64 281474976710656.0       /* 2^48 */
 struct DiffArgs {
     long long *jfix[3];
     PartPtrs snap;
+    const long *snap_off;   /* record of tile t starts at snap_off[t] (flat tile order) */
+    long snap_room;         /* entries the record holds */
 };
 
 /* particle launchers (k_particles_f64.hip) */
@@ -147,6 +151,11 @@ void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, Part
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
                              const DiffArgs *diff, hipStream_t s);
 void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s);
+/* compact record offsets (k_records.hip): off[t] = particles of the tiles
+ * before t (counts capped at cap), off[ntiles] the total; a total beyond
+ * `room` raises PIC_F_RECORD (the push writes no record entry past room) */
+void launch_record_offsets(const Grid &g, const TileGeom &t, const int *count, int cap,
+                           long *tile_sum, long *off, long room, DevStatus *st, hipStream_t s);
 void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s);
 void launch_scatter_load(const long *offsets, long ncell, const double *src[6],
                          const unsigned char *srcsp, PartPtrs p, hipStream_t s);
@@ -160,11 +169,13 @@ void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sor
                     double *partials, int nblocks, double *out27, hipStream_t s);
 
 /* FP32 form (k_particles_f32.hip) */
-/* diff: differentiable form (ids; snapshot of the pre-step state into
- * *snap when snap->p[0] is set), null otherwise */
+/* diff: differentiable form (ids; compact record of the pre-step state into
+ * *diff at snap_off[tile] + flat index when diff->p[0] is set), null
+ * otherwise */
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             const PartPtrs32 *diff, hipStream_t s);
+                             const PartPtrs32 *diff, const long *snap_off, long snap_room,
+                             hipStream_t s);
 void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
                        hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
@@ -231,8 +242,8 @@ void launch_adj_e_periodic(const Grid &g, AdjLam l, hipStream_t s);
 void launch_adj_curl_e(const Grid &g, AdjLam l, hipStream_t s);
 void launch_adj_fold_j(const Grid &g, AdjLam l, hipStream_t s);
 void launch_adj_particles_f64(const Grid &g, const TileGeom &t, AdjFwd fwd, PartPtrs snap,
-                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
-                              hipStream_t s);
+                              const long *snap_off, SpeciesTable sp, AdjLam l, LamParts lp,
+                              DevStatus *st, hipStream_t s);
 void launch_adj_curl_h(const Grid &g, AdjLam l, hipStream_t s);
 
 /* FP32 form (k_adjoint32.hip): forward fields are floats, the particle
@@ -242,8 +253,8 @@ struct AdjFwd32 {
     const float *H[3];
 };
 void launch_adj_particles_f32(const Grid &g, const TileGeom &t, AdjFwd32 fwd, PartPtrs32 snap,
-                              SpeciesTable sp, AdjLam l, LamParts lp, DevStatus *st,
-                              hipStream_t s);
+                              const long *snap_off, SpeciesTable sp, AdjLam l, LamParts lp,
+                              DevStatus *st, hipStream_t s);
 void launch_scale_lam(LamParts lp, double sx, double sy, double sz, hipStream_t s);
 void launch_adj_seed_energy32(const Grid &g, const float *const E[3], double w, AdjLam l,
                               hipStream_t s);

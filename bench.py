@@ -156,7 +156,8 @@ def adjoint_device(pic, idx, precision, first=150, last=200, ckpt_every=20, seed
     n = cfg["n"]
     T = cfg["steps"]
     params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"], precision=precision,
-                             differentiable=1, ghost_mode=1)
+                             differentiable=1, ghost_mode=1,
+                             departure_cap=cfg.get("departure_cap", 50))
 
     def ctx():
         g = pic.Pic(params)
@@ -246,7 +247,8 @@ def main():
     log(f"workload {workload_name(cfg, args.config, ppc)}")
     parts = None
     n = cfg["n"]
-    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"])
+    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"],
+                             departure_cap=cfg.get("departure_cap", 50))
     if args.host_init:
         t0 = time.perf_counter()
         params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
@@ -269,13 +271,15 @@ def main():
     yee_ms = sum(prof[k][0] for k in ("h_half", "h2", "e", "e_periodic", "fold_j")) / args.steps
     stage_ms = {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()}
 
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", "push_deposit_f64_traffic.json")
-    if os.path.exists(tf):
+    def measured_traffic(name):
+        """HBM bytes per launch of a push kernel from the committed counter
+        summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None"""
+        tf = os.path.join(ROOT, "profiles", name)
         try:
-            traffic = json.load(open(tf)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
+            return json.load(open(tf)).get("bytes_per_launch")
+        except (OSError, ValueError):
+            return None
+    traffic = measured_traffic("push_deposit_f64_traffic.json")
 
     fp32 = None
     if r32 is not None:
@@ -290,6 +294,7 @@ def main():
                 "energy_history_steps_compared": int(len(rel)),
                 "roofline": {"bound": "hbm", "achieved": ach32, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": ach32 / HBM_PEAK_GBS,
+                             "traffic": measured_traffic("push_deposit_f32_traffic.json"),
                              "kernel": "k_push_deposit32", "algorithmic_bytes_per_launch": alg32,
                              "mean_launch_ms": p32_ms},
                 "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in r32["prof"].items()},

@@ -307,8 +307,9 @@ double pic_rng_normal(pic_rng *r, double mean, double sd);/* g05dde_ */
 
 typedef struct pic_species_spec {
     int sort;                 /* enum pic_sort */
-    int ppc;                  /* particles per cell (peak count if profiled) */
-    double density;           /* |charge density| n of the species */
+    int ppc;                  /* particles per cell (mean count if profiled) */
+    double density;           /* |charge density| n of the species (on the box axis
+                                 if profiled) */
     double charge;            /* +1 ions, -1 electrons */
     double q_m;               /* charge/mass */
     double drift[3];          /* mean velocity (units of c) */

@@ -134,6 +134,10 @@ def test_generate_beam_momentum_and_weights(picmod):
 
 
 def test_generate_gaussian_profile_counts(picmod):
+    """Profile 1: ppc is the mean count per cell; the transverse Gaussian
+    (sigma 8 cells) puts A = ppc ny nz / sum(f) on the axis and
+    floor(A f + 1/2) in each cell; the weight carries the axis density with
+    the axis count, and the config-4 ions neutralise the mean beam charge."""
     cfg = picmod.beam_plasma_config(4)
     params = picmod.make_params((4, 64, 64), (0.4, 6.4, 6.4), 0.05)
     spec = cfg["species"][2]
@@ -141,7 +145,16 @@ def test_generate_gaussian_profile_counts(picmod):
     n = lib.pic_species_count(C.byref(params), C.byref(spec))
     yc = np.arange(64) + 0.5 - 32
     f = np.exp(-(yc[:, None] ** 2 + yc[None, :] ** 2) / (2 * 8.0 ** 2))
-    assert n == 4 * int(np.floor(spec.ppc * f + 0.5).sum())
+    A = spec.ppc * f.size / f.sum()
+    counts = np.floor(A * f + 0.5)
+    assert n == 4 * int(counts.sum())
+    assert abs(counts.mean() / spec.ppc - 1) < 1e-3
+    assert counts.max() > 9 * spec.ppc and counts.min() == 0
+    x, y, z, px, py, pz, m = picmod.generate_species(params, spec, None, 1, seed=1)
+    assert len(x) == n
+    assert m == pytest.approx(-0.05 / A, rel=1e-14)
+    ions = cfg["species"][0]
+    assert ions.density - 1.0 == pytest.approx(0.05 * f.mean(), rel=1e-12)
 
 
 def test_create_without_gpu_reports_no_device(picmod):
