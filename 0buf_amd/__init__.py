@@ -111,7 +111,7 @@ EXPORTS = [
     "pic_save_state_dump", "pic_load_state_dump", "pic_control_point",
     # differentiable form
     "pic_particle_id_count", "pic_get_particle_ids", "pic_forward_record", "pic_adjoint",
-    "pic_adjoint_timing",
+    "pic_adjoint_timing", "pic_rewind",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
     "pic_generate_species", "pic_generate_device",
@@ -189,6 +189,7 @@ def load_library(path=LIB_PATH):
         "pic_forward_record": (C.c_int, [vp, C.c_int, C.c_int, dp]),
         "pic_adjoint": (C.c_int, [vp, dp, C.POINTER(PicAdjointIO), C.POINTER(PicAdjointIO)]),
         "pic_adjoint_timing": (C.c_int, [vp, dp]),
+        "pic_rewind": (C.c_int, [vp]),
         "pic_rng_seed": (None, [C.POINTER(PicRng), C.c_int, C.c_int, C.c_int]),
         "pic_rng_uniform": (C.c_double, [C.POINTER(PicRng)]),
         "pic_rng_normal": (C.c_double, [C.POINTER(PicRng), C.c_double, C.c_double]),
@@ -489,6 +490,10 @@ class Pic:
         a = np.zeros(3)
         _check(self.lib.pic_adjoint_timing(self._ctx, _dptr(a)), "pic_adjoint_timing")
         return tuple(float(v) for v in a)
+
+    def rewind(self):
+        """Back to s_0 of the last forward_record."""
+        _check(self.lib.pic_rewind(self._ctx), "pic_rewind")
 
 
 # ---------------------------------------------------------------------------

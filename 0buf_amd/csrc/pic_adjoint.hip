@@ -72,6 +72,7 @@ struct AdjointStore {
     void *Eend[3] = {nullptr, nullptr, nullptr};   /* E at the end of the segment */
     AdjLam lam{};
     LamParts lp{};
+    long lam_ni = 0;                  /* particle adjoint entries allocated */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double ms[3] = {0, 0, 0};
 };
@@ -125,6 +126,23 @@ void free_compact(Compact &k) {
     k = Compact{};
 }
 
+/* device memory of a checkpoint of P particles */
+int reserve_state(pic_ctx *c, Compact &dst, FieldSet &f, long P) {
+    const size_t es = elem(c);
+    if (dst.off && dst.P < P) free_compact(dst);
+    if (!dst.off) {
+        const size_t Pn = (size_t)std::max(P, 1L);
+        HIPCHK(hipMalloc(&dst.off, (c->NC + 1) * sizeof(long)));
+        HIPCHK(hipMalloc(&dst.count, c->NC * sizeof(int)));
+        for (int j = 0; j < 6; j++) HIPCHK(hipMalloc(&dst.v[j], Pn * es));
+        HIPCHK(hipMalloc(&dst.sp, Pn));
+        HIPCHK(hipMalloc(&dst.id, Pn * sizeof(int)));
+        dst.P = P;
+    }
+    if (!f.f[0]) return alloc_fields(f, c->g.G, es);
+    return PIC_SUCCESS;
+}
+
 int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     const long NC = c->NC;
     const size_t es = elem(c);
@@ -134,23 +152,11 @@ int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     std::vector<long> off(NC + 1, 0);
     for (long k = 0; k < NC; k++) off[k + 1] = off[k] + std::min(cnt[k], live_cap(c));
     const long P = off[NC];
-    if (dst.off && dst.P < P) free_compact(dst);
-    if (!dst.off) {
-        const size_t Pn = (size_t)std::max(P, 1L);
-        HIPCHK(hipMalloc(&dst.off, (NC + 1) * sizeof(long)));
-        HIPCHK(hipMalloc(&dst.count, NC * sizeof(int)));
-        for (int j = 0; j < 6; j++) HIPCHK(hipMalloc(&dst.v[j], Pn * es));
-        HIPCHK(hipMalloc(&dst.sp, Pn));
-        HIPCHK(hipMalloc(&dst.id, Pn * sizeof(int)));
-        dst.P = P;
-    }
+    int rc = reserve_state(c, dst, f, P);
+    if (rc) return rc;
     HIPCHK(hipMemcpy(dst.off, off.data(), (NC + 1) * sizeof(long), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpyAsync(dst.count, live_count(c), NC * sizeof(int), hipMemcpyDeviceToDevice,
                           c->stream));
-    if (!f.f[0]) {
-        int rc = alloc_fields(f, c->g.G, es);
-        if (rc) return rc;
-    }
     if (c->fp32) {
         float *v[6];
         for (int j = 0; j < 6; j++) v[j] = (float *)dst.v[j];
@@ -324,15 +330,26 @@ int pic_forward_record(pic_ctx *c, int nsteps, int ckpt_every, double *energy) {
     int rc = check_ready(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->dev));
-    free_adjoint(c);
-    AdjointStore *s = new AdjointStore();
-    c->adj = s;
-    s->nsteps = nsteps;
-    s->K = ckpt_every;
-    for (int k = 0; k < 3; k++) HIPCHK(hipEventCreate(&s->ev[k]));
+    /* a store of the same shape is reused (its checkpoints and records keep
+     * their device memory); any other is replaced */
+    AdjointStore *s = c->adj;
+    if (!s || s->nsteps != nsteps || s->K != ckpt_every) {
+        free_adjoint(c);
+        s = new AdjointStore();
+        c->adj = s;
+        s->nsteps = nsteps;
+        s->K = ckpt_every;
+        for (int k = 0; k < 3; k++) HIPCHK(hipEventCreate(&s->ev[k]));
+    }
+    s->recorded = false;
+    s->anomalies = 0;
     const int nck = (nsteps - 1) / ckpt_every + 1;
     s->ck.resize(nck + 1);
     s->ckf.resize(nck + 1);
+    /* checkpoint memory up front, outside the timed run */
+    const long P = pic_particle_count(c);
+    for (int k = 0; k <= nck; k++)
+        if ((rc = reserve_state(c, s->ck[k], s->ckf[k], P))) return rc;
     HIPCHK(hipEventRecord(s->ev[0], c->stream));
     for (int t = 0; t < nsteps; t++) {
         if (t % ckpt_every == 0) {
@@ -372,20 +389,25 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
     hipStream_t st = c->stream;
     int rc;
 
-    free_lam(s);
+    /* adjoint arrays: kept between calls of the same particle count */
+    if (!s->lam.E[0] || s->lam_ni != NI) {
+        free_lam(s);
+        for (int k = 0; k < 3; k++) {
+            HIPCHK(hipMalloc(&s->lam.E[k], G * sizeof(double)));
+            HIPCHK(hipMalloc(&s->lam.H[k], G * sizeof(double)));
+            HIPCHK(hipMalloc(&s->lam.J[k], G * sizeof(double)));
+            HIPCHK(hipMalloc(&s->lam.Q[k], G * sizeof(double)));
+            HIPCHK(hipMalloc(&s->Eend[k], G * elem(c)));
+        }
+        for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&s->lp.v[k], NI * sizeof(double)));
+        s->lam_ni = NI;
+    }
     for (int k = 0; k < 3; k++) {
-        HIPCHK(hipMalloc(&s->lam.E[k], G * sizeof(double)));
-        HIPCHK(hipMalloc(&s->lam.H[k], G * sizeof(double)));
-        HIPCHK(hipMalloc(&s->lam.J[k], G * sizeof(double)));
-        HIPCHK(hipMalloc(&s->lam.Q[k], G * sizeof(double)));
-        HIPCHK(hipMalloc(&s->Eend[k], G * elem(c)));
         if ((rc = zero_async(s->lam.E[k], G * sizeof(double), st))) return rc;
         if ((rc = zero_async(s->lam.H[k], G * sizeof(double), st))) return rc;
     }
-    for (int k = 0; k < 6; k++) {
-        HIPCHK(hipMalloc(&s->lp.v[k], NI * sizeof(double)));
+    for (int k = 0; k < 6; k++)
         if ((rc = zero_async(s->lp.v[k], NI * sizeof(double), st))) return rc;
-    }
     s->lp.n = c->n_ids;
     if (seed) {
         const double *ps[6] = {seed->x, seed->y, seed->z, seed->px, seed->py, seed->pz};
@@ -530,6 +552,17 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
     /* leave the context at s_nsteps */
     if ((rc = restore_state(c, s->ck.back(), s->ckf.back()))) return rc;
     HIPCHK(hipStreamSynchronize(st));
+    return PIC_SUCCESS;
+}
+
+int pic_rewind(pic_ctx *c) {
+    if (!c) return PIC_ERR_INVALID;
+    AdjointStore *s = c->adj;
+    if (!c->diff || !s || !s->recorded) return PIC_ERR_STATE;
+    HIPCHK(hipSetDevice(c->dev));
+    int rc = restore_state(c, s->ck[0], s->ckf[0]);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(c->stream));
     return PIC_SUCCESS;
 }
 

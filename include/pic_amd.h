@@ -287,6 +287,10 @@ int pic_forward_record(pic_ctx *ctx, int nsteps, int ckpt_every, double *energy)
  * zero).  The context is left at s_nsteps. */
 int pic_adjoint(pic_ctx *ctx, const double *weights, const pic_adjoint_io *seed,
                 pic_adjoint_io *grad);
+/* back to s_0 of the last pic_forward_record (its first checkpoint); a later
+ * pic_forward_record of the same nsteps and ckpt_every reuses the
+ * checkpoint and record memory */
+int pic_rewind(pic_ctx *ctx);
 /* device time of the last pic_forward_record and pic_adjoint calls (ms):
  * out[0] forward with checkpoints, out[1] recompute, out[2] adjoint steps */
 int pic_adjoint_timing(pic_ctx *ctx, double *out3);

@@ -174,3 +174,29 @@ def test_config4_shape_fp32_gradient_against_fp64(picmod):
           f"FP32 {r[picmod.FP32]['loss']:.9e}")
     assert abs(r[picmod.FP32]["loss"] / r[picmod.FP64]["loss"] - 1) <= 1e-3
     assert rel <= 1e-2
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_rewind_and_store_reuse(picmod, prec):
+    """pic_rewind puts the context back at s_0 of the last forward_record; a
+    second forward_record of the same shape reuses the checkpoint and record
+    memory and reproduces the first run bit for bit (order-free current), and
+    its adjoint matches the first one's."""
+    _, parts = picmod.generate_workload(0)
+    prm = picmod.make_params((32, 8, 8), (3.2, 0.8, 0.8), 0.05, precision=prec,
+                             differentiable=1)
+    g = picmod.Pic(prm)
+    load_workload(g, parts)
+    w = np.ones(20)
+    W1 = g.forward_record(20, 10)
+    g1 = g.adjoint(weights=w)
+    g.rewind()
+    W0 = g.run(1, 20)
+    assert np.array_equal(W0, W1)
+    g.rewind()
+    W2 = g.forward_record(20, 10)
+    assert np.array_equal(W1, W2)
+    g2 = g.adjoint(weights=w)
+    for k in FK + PK:
+        assert np.linalg.norm(g2[k] - g1[k]) <= 1e-12 * np.linalg.norm(g1[k]), k
+    g.close()
