@@ -34,6 +34,15 @@ namespace {
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
 
+/* copies of the LDS adjoint box the gather adjoint scatters into: lane l adds
+ * into copy l % ACOPIES, dividing the same-address conflicts of the FP64 LDS
+ * atomics (the particles of one cell scatter to the same nodes); the copies
+ * are summed before the global add */
+#ifndef PIC_ACOPIES64
+#define PIC_ACOPIES64 3
+#endif
+constexpr int ACOPIES = PIC_ACOPIES64;
+
 inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC_BLOCK); }
 
 __device__ __forceinline__ void decode(const Grid &g, long n, int &i, int &l, int &k) {
@@ -332,7 +341,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ double sF[6 * NB];    /* E, H' of the forward step */
     __shared__ double sL[3 * NB];    /* lamJ */
-    __shared__ double sA[6 * NB];    /* lamE, lamH' accumulated by the gather adjoint */
+    __shared__ double sA[ACOPIES * 6 * NB];   /* lamE, lamH' accumulated by the gather adjoint */
+    double *const myA = sA + (threadIdx.x % ACOPIES) * (6 * NB);
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ unsigned sFlags;
@@ -372,9 +382,9 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
             if (fi < 3) lj = a.lam.J[fi][gidx];
         }
         sF[e] = val;
-        sA[e] = 0.0;
         if (fi < 3) sL[e] = lj;
     }
+    for (int e = tid; e < ACOPIES * 6 * NB; e += PIC_BLOCK) sA[e] = 0.0;
     if (tid == 0) sFlags = 0;
     __syncthreads();
     if (tid == 0) {
@@ -556,12 +566,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
         if (live && W.ok) {
             const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
             double dsx = 0, dsy = 0, dsz = 0;
-            interp_adj<BY, BZ>(sF + 0 * NB, sA + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, W.s1, W.s4, W.s6, lE[0], dsx, dsy, dsz);
-            interp_adj<BY, BZ>(sF + 1 * NB, sA + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, W.s2, W.s3, W.s6, lE[1], dsx, dsy, dsz);
-            interp_adj<BY, BZ>(sF + 2 * NB, sA + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, W.s2, W.s4, W.s5, lE[2], dsx, dsy, dsz);
-            interp_adj<BY, BZ>(sF + 3 * NB, sA + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, W.s2, W.s3, W.s5, lH[0], dsx, dsy, dsz);
-            interp_adj<BY, BZ>(sF + 4 * NB, sA + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, W.s1, W.s4, W.s5, lH[1], dsx, dsy, dsz);
-            interp_adj<BY, BZ>(sF + 5 * NB, sA + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, W.s1, W.s3, W.s6, lH[2], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 0 * NB, myA + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, W.s1, W.s4, W.s6, lE[0], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 1 * NB, myA + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, W.s2, W.s3, W.s6, lE[1], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 2 * NB, myA + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, W.s2, W.s4, W.s5, lE[2], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 3 * NB, myA + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, W.s2, W.s3, W.s5, lH[0], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 4 * NB, myA + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, W.s1, W.s4, W.s5, lH[1], dsx, dsy, dsz);
+            interp_adj<BY, BZ>(sF + 5 * NB, myA + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, W.s1, W.s3, W.s6, lH[2], dsx, dsy, dsz);
             lx[0] -= dsx * a.dc.rhx;
             lx[1] -= dsy * a.dc.rhy;
             lx[2] -= dsz * a.dc.rhz;
@@ -576,7 +586,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        const double val = sA[e];
+        double val = sA[e];
+        for (int r = 1; r < ACOPIES; r++) val += sA[r * 6 * NB + e];
         if (val == 0.0) continue;
         int fi = e / NB, nn = e % NB;
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;

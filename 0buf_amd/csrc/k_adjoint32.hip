@@ -21,6 +21,15 @@ namespace {
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
 
+/* copies of the LDS adjoint box the gather adjoint scatters into: lane l adds
+ * into copy l % ACOPIES, dividing the same-address conflicts of the FP64 LDS
+ * atomics (the particles of one cell scatter to the same nodes); the copies
+ * are summed before the global add */
+#ifndef PIC_ACOPIES32
+#define PIC_ACOPIES32 4
+#endif
+constexpr int ACOPIES = PIC_ACOPIES32;
+
 struct AdjArgs32 {
     Grid g;
     TileGeom t;
@@ -129,7 +138,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ float sF[6 * NB];
     __shared__ float sL[3 * NB];
-    __shared__ double sA[6 * NB];
+    __shared__ double sA[ACOPIES * 6 * NB];
+    double *const myA = sA + (threadIdx.x % ACOPIES) * (6 * NB);
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
 
@@ -168,9 +178,9 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
             if (fi < 3) lj = (float)a.lam.J[fi][gidx];
         }
         sF[e] = val;
-        sA[e] = 0.0;
         if (fi < 3) sL[e] = lj;
     }
+    for (int e = tid; e < ACOPIES * 6 * NB; e += PIC_BLOCK) sA[e] = 0.0;
     __syncthreads();
     if (tid == 0) {
         int s = 0;
@@ -310,12 +320,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         /* gather adjoint: every near weight has d/do = -1 */
         {
             float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-            tri_adj<BY, BZ>(sF + 0 * NB, sA + 0 * NB, ca, cb + rdy, cc + rdz, nx0, cy0, cz0, lE[0], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 1 * NB, sA + 1 * NB, ca + rdx, cb, cc + rdz, cx0, ny0, cz0, lE[1], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 2 * NB, sA + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cy0, nz0, lE[2], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 3 * NB, sA + 3 * NB, ca + rdx, cb, cc, cx0, ny0, nz0, lH[0], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 4 * NB, sA + 4 * NB, ca, cb + rdy, cc, nx0, cy0, nz0, lH[1], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 5 * NB, sA + 5 * NB, ca, cb, cc + rdz, nx0, ny0, cz0, lH[2], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 0 * NB, myA + 0 * NB, ca, cb + rdy, cc + rdz, nx0, cy0, cz0, lE[0], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 1 * NB, myA + 1 * NB, ca + rdx, cb, cc + rdz, cx0, ny0, cz0, lE[1], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 2 * NB, myA + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cy0, nz0, lE[2], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 3 * NB, myA + 3 * NB, ca + rdx, cb, cc, cx0, ny0, nz0, lH[0], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 4 * NB, myA + 4 * NB, ca, cb + rdy, cc, nx0, cy0, nz0, lH[1], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + 5 * NB, myA + 5 * NB, ca, cb, cc + rdz, nx0, ny0, cz0, lH[2], dx, dy, dz);
             lo[0] -= dx;
             lo[1] -= dy;
             lo[2] -= dz;
@@ -329,7 +339,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
     }
     __syncthreads();
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        const double val = sA[e];
+        double val = sA[e];
+        for (int r = 1; r < ACOPIES; r++) val += sA[r * 6 * NB + e];
         if (val == 0.0) continue;
         int fi = e / NB, nn = e % NB;
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
