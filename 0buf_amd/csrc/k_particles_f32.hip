@@ -16,6 +16,7 @@
  *   * migration by the cell-local offset: a particle whose offset leaves
  *     [0,1) moves to the neighbouring cell (periodic), exactly one receiver.
  */
+#include "k_arrive.h"
 #include "k_push_math32.h"
 
 /* copies of the LDS J box: lane l deposits into copy l % JCOPIES, which
@@ -272,6 +273,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
                 a.q.sp[e] = s;
                 if (DIFF) a.q.id[e] = pid;
                 a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
+                a.q.halo[e] = (unsigned char)(((ca + shx + 1) * BY + (cb + shy + 1)) * BZ +
+                                              (cc + shz + 1));
             }
         }
     }
@@ -327,28 +330,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     }
 }
 
-__global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(PartPtrs32 p, QueuePtrs32 q,
-                                                       DevStatus *st) {
-    const long tile = blockIdx.x;
-    const int n = q.count[tile];
-    const long qb = tile * (long)q.qcap;
-    bool over = false;
-    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
-        const long e = qb + j;
-        const int d = q.dest[e];
-        const int slot = atomicAdd(&p.count[d], 1);
-        q.slot[e] = slot;
-        if (slot < p.cap) {
-            const long to = (long)d * p.cap + slot;
-            for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
-            p.sp[to] = q.sp[e];
-            if (p.id) p.id[to] = q.id[e];
-        } else {
-            over = true;
-        }
-    }
-    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
-}
 
 __global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load32(const long *off, long ncell,
                                                              const float *s0, const float *s1,
@@ -551,9 +532,10 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                            dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
 }
 
-void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
-                       hipStream_t s) {
-    hipLaunchKernelGGL(k_arrive32, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, p, q, st);
+void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
+                       DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL((k_arrive_tile<TXC, TYC, TZC, PartPtrs32, QueuePtrs32>),
+                       dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, g, t, p, q, st);
 }
 
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],

@@ -16,6 +16,7 @@
  * The arithmetic mirrors cell.h / particle.h in the reference's evaluation
  * order and is built with -ffp-contract=off.
  */
+#include "k_arrive.h"
 #include "k_push_math.h"
 
 #pragma clang fp contract(off)
@@ -362,6 +363,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
                             a.q.sp[e] = s;
                             if (DIFF) a.q.id[e] = pid;
                             a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
+                            a.q.halo[e] = (unsigned char)(((G.ca + ax) * BY + (G.cb + ay)) * BZ +
+                                                          (G.cc + az));
                         }
                     }
                 }
@@ -447,29 +450,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_jfix_to_f64(long G, FieldPtrs f, 
     }
 }
 
-/* arrivals: each queued leaver claims the next slot of its destination
- * bucket (the Insert of GPU_ArrangeFlights, kernels.cu:357-360) */
-__global__ __launch_bounds__(PIC_BLOCK) void k_arrive(PartPtrs p, QueuePtrs q, DevStatus *st) {
-    const long tile = blockIdx.x;
-    const int n = q.count[tile];
-    const long qb = tile * (long)q.qcap;
-    bool over = false;
-    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
-        const long e = qb + j;
-        const int d = q.dest[e];
-        const int slot = atomicAdd(&p.count[d], 1);
-        q.slot[e] = slot;
-        if (slot < p.cap) {
-            const long to = (long)d * p.cap + slot;
-            for (int k = 0; k < 6; k++) p.p[k][to] = q.q[k][e];
-            p.sp[to] = q.sp[e];
-            if (p.id) p.id[to] = q.id[e];
-        } else {
-            over = true;
-        }
-    }
-    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
-}
 
 /* initial upload: compact cell-sorted arrays -> buckets */
 __global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load(const long *off, long ncell,
@@ -653,8 +633,10 @@ void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s) {
     hipLaunchKernelGGL(k_jfix_to_f64, dim3(blocks_for(g.G)), dim3(PIC_BLOCK), 0, s, g.G, f, d);
 }
 
-void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s) {
-    hipLaunchKernelGGL(k_arrive, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, p, q, st);
+void launch_arrive_f64(const Grid &g, const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st,
+                       hipStream_t s) {
+    hipLaunchKernelGGL((k_arrive_tile<TXC, TYC, TZC, PartPtrs, QueuePtrs>), dim3((unsigned)t.ntiles),
+                       dim3(PIC_BLOCK), 0, s, g, t, p, q, st);
 }
 
 void launch_scatter_load(const long *offsets, long ncell, const double *src[6],

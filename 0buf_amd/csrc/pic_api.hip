@@ -67,6 +67,7 @@ void free_queue(Q &q) {
     if (q.slot) (void)hipFree(q.slot);
     if (q.count) (void)hipFree(q.count);
     if (q.id) (void)hipFree(q.id);
+    if (q.halo) (void)hipFree(q.halo);
     q = Q{};
 }
 
@@ -83,6 +84,7 @@ int alloc_queue(Q &q, const TileGeom &t, int qcap, bool with_id) {
     HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
     HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
     if (with_id) HIPCHK(hipMalloc(&q.id, n * sizeof(int)));
+    HIPCHK(hipMalloc(&q.halo, n));
     return PIC_SUCCESS;
 }
 
@@ -274,8 +276,8 @@ void enq_push(pic_ctx *c) {
     }
 }
 void enq_arrive(pic_ctx *c) {
-    if (c->fp32) launch_arrive_f32(c->t, c->p32, c->q32, c->d_st, c->stream);
-    else launch_arrive_f64(c->t, c->p, c->q, c->d_st, c->stream);
+    if (c->fp32) launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, c->stream);
+    else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, c->stream);
 }
 void enq_fold(pic_ctx *c) {
     if (c->fp32) launch_fold_j32(c->g, c->f32, c->stream);

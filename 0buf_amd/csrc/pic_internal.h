@@ -90,6 +90,7 @@ struct QueuePtrs32 {
     int *count;
     int qcap;
     int *id;                 /* differentiable form only */
+    unsigned char *halo;     /* destination in the source tile's halo box */
 };
 
 struct QueuePtrs {
@@ -100,6 +101,7 @@ struct QueuePtrs {
     int *count;              /* per tile */
     int qcap;                /* entries per tile */
     int *id;                 /* differentiable form only */
+    unsigned char *halo;     /* destination in the source tile's (TX+2)(TY+2)(TZ+2) halo box */
 };
 
 struct TileGeom {
@@ -156,7 +158,8 @@ void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s);
  * `room` raises PIC_F_RECORD (the push writes no record entry past room) */
 void launch_record_offsets(const Grid &g, const TileGeom &t, const int *count, int cap,
                            long *tile_sum, long *off, long room, DevStatus *st, hipStream_t s);
-void launch_arrive_f64(const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st, hipStream_t s);
+void launch_arrive_f64(const Grid &g, const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st,
+                       hipStream_t s);
 void launch_scatter_load(const long *offsets, long ncell, const double *src[6],
                          const unsigned char *srcsp, PartPtrs p, hipStream_t s);
 void launch_repack(long ncell, PartPtrs from, PartPtrs to, hipStream_t s);
@@ -176,8 +179,8 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
                              const PartPtrs32 *diff, const long *snap_off, long snap_room,
                              hipStream_t s);
-void launch_arrive_f32(const TileGeom &t, PartPtrs32 p, QueuePtrs32 q, DevStatus *st,
-                       hipStream_t s);
+void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
+                       DevStatus *st, hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
 void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);

@@ -220,6 +220,8 @@ def main():
     ap.add_argument("--adjoint-config4", action="store_true",
                     help="also time the config-4 gradient (FP64 and FP32 differentiable "
                          "forms, particles drawn on the GPU)")
+    ap.add_argument("--adjoint-config4-only", action="store_true",
+                    help="only the config-4 gradient timing (for profiling its kernels)")
     ap.add_argument("--cpu-slab", type=int, default=8)
     ap.add_argument("--host-init", action="store_true",
                     help="generate the particles on the host and upload them instead of "
@@ -242,7 +244,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the step would shard as x-slabs; see DESIGN.md)", file=sys.stderr)
+              "(the step would shard as x-slabs with halo and leaver exchange)", file=sys.stderr)
         sys.exit(2)
 
     # torch carries its own HIP runtime beside the system one the library links;
@@ -251,6 +253,13 @@ def main():
     torch.cuda.init()
     torch.cuda.synchronize()
     pic = importlib.import_module("0buf_amd")
+    if args.adjoint_config4_only:
+        out = {}
+        for prec, nm in ((pic.FP64, "fp64"), (pic.FP32, "fp32")):
+            log(f"timing the config 4 gradient, {nm}")
+            out[nm] = adjoint_device(pic, 4, prec)
+        print(json.dumps({"adjoint_config4": out}))
+        return
     cfg = pic.beam_plasma_config(args.config)
     ppc = tuple(s.ppc for s in cfg["species"])
     log(f"workload {workload_name(cfg, args.config, ppc)}")
