@@ -119,6 +119,20 @@ def push_roofline(res, width):
     return push_ms, alg, achieved
 
 
+def yee_roofline(ncell, yee_ms, width):
+    """Field stages of one step (H half step, H second half, E update, periodic
+    ghosts, J fold) against the HBM roofline.  The fused minimum per cell reads
+    E, H and J once and writes E and H once: 5 x 3 x 8 = 120 B in FP64; in the
+    FP32 form (FP32 E and H, int64 J) 4 x 3 x 4 + 3 x 8 = 72 B."""
+    per_cell = 120 if width == 8 else 72
+    if yee_ms <= 0:
+        return None
+    achieved = ncell * per_cell / (yee_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_cell": per_cell,
+            "ms_per_step": yee_ms}
+
+
 def adjoint_ratio(pic):
     """Config 3 (64x16x16 cells, 32 ppc, 200 steps, checkpoints every 20):
     wall time of a gradient (forward with checkpoints + recompute + adjoint
@@ -305,6 +319,7 @@ def main():
                              "algorithmic_bytes_per_launch": alg, "mean_launch_ms": push_ms},
                 "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
                 "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,
+                "yee_roofline": yee_roofline(ncell, yee_ms, width),
                 "particles_end": res["P_end"], "status": res["status"]}
 
     fp64 = form_result(r64, 8, "k_push_deposit (gather+Boris+VB deposit+departures, FP64)",
