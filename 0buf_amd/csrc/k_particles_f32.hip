@@ -184,22 +184,39 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
     __syncthreads();
 
-    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+    /* the next particle of the lane is loaded one iteration ahead, so its
+     * memory latency overlaps this iteration's arithmetic */
+    int nfc = 0;
+    long npidx = 0;
+    float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
+    unsigned char qs = 0;
+    int qid = 0;
+    auto fetch = [&](int f) {
         int lo = 0, hi = C;
         while (hi - lo > 1) {
             int mid = (lo + hi) >> 1;
-            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
+            if (sOff[mid] <= f) lo = mid; else hi = mid;
         }
-        const int c = lo;
-        const int j = fidx - sOff[c];
+        nfc = lo;
+        const int jj = f - sOff[lo];
+        const int a0 = lo / (TY * TZ), b0 = (lo / TZ) % TY, c0 = lo % TZ;
+        npidx = (((long)(I0 + a0) * g.Ny + (L0 + b0)) * g.Nz + (K0 + c0)) * cap + jj;
+        qox = a.p.p[0][npidx]; qoy = a.p.p[1][npidx]; qoz = a.p.p[2][npidx];
+        qpu = a.p.p[3][npidx]; qpv = a.p.p[4][npidx]; qpw = a.p.p[5][npidx];
+        qs = a.p.sp[npidx];
+        if (DIFF) qid = a.p.id[npidx];
+    };
+    if (tid < T) fetch(tid);
+    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
+        const int c = nfc;
+        const long pidx = npidx;
         const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
         const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
-
-        const float ox = a.p.p[0][pidx], oy = a.p.p[1][pidx], oz = a.p.p[2][pidx];
-        float pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
-        const unsigned char s = a.p.sp[pidx];
-        const int pid = DIFF ? a.p.id[pidx] : 0;
+        const float ox = qox, oy = qoy, oz = qoz;
+        float pu = qpu, pv = qpv, pw = qpw;
+        const unsigned char s = qs;
+        const int pid = DIFF ? qid : 0;
+        if (fidx + PIC_BLOCK < T) fetch(fidx + PIC_BLOCK);
         const long r = snap ? a.snap_off[tile] + fidx : 0;
         if (snap && r < a.snap_room) {
             a.snap.p[0][r] = ox;
