@@ -385,6 +385,7 @@ def main():
         "adjoint_config4": adj4,
         "stage_ms": head["stage_ms"],
         "yee_cell_updates_per_s": head["yee_cell_updates_per_s"],
+        "yee_roofline": head["yee_roofline"],
         "particles_end": head["particles_end"],
         "status": head["status"],
     }
