@@ -55,26 +55,40 @@ def algorithmic_bytes_push(P, ncell, width):
     return P * 2 * 6 * width + ncell * per_cell
 
 
-def cpu_baseline(pic, cfg, slab, steps=2, threads=1):
+def cpu_baseline(pic, cfg, slab, steps=10, threads=1, all_threads=None):
+    """The FP64 CPU restatement (oracle/) on a slab of the workload: `steps`
+    full steps after one warm-up, single-threaded (about 10 s of CPU work at
+    the default 16x64x64 slab), then the same with the oracle's cell loops
+    spread over `all_threads` host threads."""
     import oracle
     n = (slab, cfg["n"][1], cfg["n"][2])
     params, parts = pic.generate_workload(cfg, stream_mode=1, scale_n=n, nthreads=8)
-    o = oracle.Oracle(n, (params.lx, params.ly, params.lz), params.tau, threads=threads)
-    for (sort, x, y, z, px, py, pz, m, qm) in parts:
-        o.load_particles(sort, x, y, z, px, py, pz, m, qm)
-    P = o.particle_count()
-    o.step(1)
-    t0 = time.perf_counter()
-    for nt in range(2, 2 + steps):
-        o.step(nt)
-    dt = time.perf_counter() - t0
-    o.close()
-    return {"value": P * steps / dt, "unit": "particle pushes/s", "cores": threads,
-            "kind": "port",
-            "sample": f"oracle FP64 restatement of Plasma::Step, full steps on a "
-                      f"{n[0]}x{n[1]}x{n[2]}-cell slab of the same workload "
-                      f"({P} particles), {steps} timed steps after 1 warm-up, "
-                      f"{dt:.1f} s, single thread"}
+
+    def timed(nthreads):
+        o = oracle.Oracle(n, (params.lx, params.ly, params.lz), params.tau, threads=nthreads)
+        for (sort, x, y, z, px, py, pz, m, qm) in parts:
+            o.load_particles(sort, x, y, z, px, py, pz, m, qm)
+        P = o.particle_count()
+        o.step(1)
+        t0 = time.perf_counter()
+        for nt in range(2, 2 + steps):
+            o.step(nt)
+        dt = time.perf_counter() - t0
+        o.close()
+        return P, dt
+    P, dt = timed(threads)
+    out = {"value": P * steps / dt, "unit": "particle pushes/s", "cores": threads,
+           "kind": "port",
+           "sample": f"oracle FP64 restatement of Plasma::Step, full steps on a "
+                     f"{n[0]}x{n[1]}x{n[2]}-cell slab of the same workload "
+                     f"({P} particles), {steps} timed steps after 1 warm-up, "
+                     f"{dt:.1f} s, {threads} thread(s)"}
+    if all_threads and all_threads > threads:
+        P, dt = timed(all_threads)
+        out["all_cores"] = {"value": P * steps / dt, "cores": all_threads,
+                            "sample": f"the same slab and steps, cell loops over "
+                                      f"{all_threads} threads, {dt:.1f} s"}
+    return out
 
 
 def run_form(pic, params, cfg, parts, bx0, precision, steps, warmup, seed=12345):
@@ -236,7 +250,7 @@ def main():
                          "forms, particles drawn on the GPU)")
     ap.add_argument("--adjoint-config4-only", action="store_true",
                     help="only the config-4 gradient timing (for profiling its kernels)")
-    ap.add_argument("--cpu-slab", type=int, default=8)
+    ap.add_argument("--cpu-slab", type=int, default=16)
     ap.add_argument("--host-init", action="store_true",
                     help="generate the particles on the host and upload them instead of "
                          "drawing them on the GPU (same particles, bit for bit)")
@@ -350,7 +364,8 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU baseline (oracle) on a slab of the workload")
-        cpu = cpu_baseline(pic, cfg, args.cpu_slab)
+        cpu = cpu_baseline(pic, cfg, args.cpu_slab,
+                           all_threads=min(16, len(os.sched_getaffinity(0))))
         log(f"cpu baseline {cpu['value']:.3e} pushes/s")
 
     # the headline is the FP32 precision form with its error against the FP64
