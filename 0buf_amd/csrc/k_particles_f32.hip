@@ -2,8 +2,10 @@
  * k_particles_f32.hip -- particle stages of the FP32 precision form.
  *
  * Same tile structure as the FP64 plain form (one workgroup per 2x4x8 cells,
- * E/H node box staged in LDS, flat particle order, in-place stayers, tile
- * departure regions, holes filled from the bucket tail), with:
+ * E/H node box staged in LDS, tile departure regions), with:
+ *   * each wavefront owning whole cells of the tile and walking them in
+ *     chunks of whole 128-byte line groups; stayers are compacted in place in
+ *     flat order (stable), so a bucket needs no second pass after the push;
  *   * positions held as FP32 offsets in [0,1) inside the owning cell, so the
  *     resolution is h * 2^-24 everywhere in the box;
  *   * FP32 gather and Boris push (the reference's Cayley form);
@@ -30,6 +32,7 @@
 namespace {
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
+constexpr int GS32 = 32;    /* slots of an FP32 array in one 128-byte line */
 constexpr int JCOPIES = PIC_JCOPIES32;
 
 struct PushArgs32 {
@@ -95,23 +98,6 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, in
     add64(&Jz[n + BY * BZ + BZ], fix64(qz * (mx * my + sxy)));
 }
 
-__device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from, int to,
-                                        bool want) {
-    int pos = from;
-    while (pos < to) {
-        int gbit = base + pos;
-        unsigned w = mask[gbit >> 5];
-        if (!want) w = ~w;
-        w >>= (gbit & 31);
-        if (w) {
-            int p = pos + __builtin_ctz(w);
-            return p < to ? p : to;
-        }
-        pos += 32 - (gbit & 31);
-    }
-    return to;
-}
-
 /* 8 waves per SIMD (at most 64 VGPRs, no spills) */
 template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
@@ -122,9 +108,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
+    __shared__ int sBase[C];     /* first slot of each cell, relative to the tile's first cell */
+    __shared__ int sG[C + 1];    /* first line group of each cell */
     __shared__ int sQ;
     __shared__ unsigned sFlags;
-    extern __shared__ unsigned sMask[];
+    extern __shared__ unsigned sGrp[];    /* line groups of the tile */
     long long *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
     const Grid &g = a.g;
@@ -139,7 +127,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     const int tz = (int)(tile % a.t.ntz);
     const int ty = (int)((tile / a.t.ntz) % a.t.nty);
     const int tx = (int)(tile / ((long)a.t.ntz * a.t.nty));
-    const int I0 = tx * TX, L0 = ty * TY, K0 = tz * TZ;
+    /* workgroup-uniform (the 64-bit divisions above run on the vector
+     * unit; readfirstlane keeps the tile's base addresses scalar) */
+    const int I0 = __builtin_amdgcn_readfirstlane(tx * TX);
+    const int L0 = __builtin_amdgcn_readfirstlane(ty * TY);
+    const int K0 = __builtin_amdgcn_readfirstlane(tz * TZ);
     const int cap = a.p.cap;
 
     for (int c = tid; c < C; c += PIC_BLOCK) {
@@ -154,6 +146,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         }
         sCnt[c] = cnt;
         sLeave[c] = 0;
+        sBase[c] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
     }
     const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
@@ -172,126 +165,158 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     }
     __syncthreads();
     if (tid == 0) {
-        int s = 0;
+        int s = 0, ng = 0;
         for (int c = 0; c < C; c++) {
             sOff[c] = s;
+            sG[c] = ng;
             s += sCnt[c];
+            ng += (sCnt[c] + GS32 - 1) / GS32;
         }
         sOff[C] = s;
+        sG[C] = ng;
     }
     __syncthreads();
-    const int T = sOff[C];
-    for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
+    build_groups<TX, TY, TZ, GS32>(sCnt, sG, sGrp, tid, PIC_BLOCK);
     __syncthreads();
 
-    /* the next particle of the lane is loaded one iteration ahead, so its
-     * memory latency overlaps this iteration's arithmetic */
-    int nfc = 0;
-    long npidx = 0;
+    /* per-tile bases of the particle arrays: slot rel of the tile is
+     * cell-bucket slot pbase + rel */
+    const long pbase = (((long)I0 * g.Ny + L0) * g.Nz + K0) * cap;
+    float *const P0 = a.p.p[0] + pbase, *const P1 = a.p.p[1] + pbase,
+                 *const P2 = a.p.p[2] + pbase, *const P3 = a.p.p[3] + pbase,
+                 *const P4 = a.p.p[4] + pbase, *const P5 = a.p.p[5] + pbase;
+    unsigned char *const SP = a.p.sp + pbase;
+    int *const ID = DIFF ? a.p.id + pbase : nullptr;
+
+    /* the wavefront's next chunk is composed and its particles loaded one
+     * chunk ahead, so the memory latency overlaps this chunk's arithmetic */
+    const int lane = tid & 63, wv = tid >> 6;
+    int nfc = 0, nslot = 0;
     float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
     unsigned char qs = 0;
     int qid = 0;
-    auto fetch = [&](int f) {
-        int lo = 0, hi = C;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (sOff[mid] <= f) lo = mid; else hi = mid;
-        }
-        nfc = lo;
-        const int jj = f - sOff[lo];
-        const int a0 = lo / (TY * TZ), b0 = (lo / TZ) % TY, c0 = lo % TZ;
-        npidx = (((long)(I0 + a0) * g.Ny + (L0 + b0)) * g.Nz + (K0 + c0)) * cap + jj;
-        qox = a.p.p[0][npidx]; qoy = a.p.p[1][npidx]; qoz = a.p.p[2][npidx];
-        qpu = a.p.p[3][npidx]; qpv = a.p.p[4][npidx]; qpw = a.p.p[5][npidx];
-        qs = a.p.sp[npidx];
-        if (DIFF) qid = a.p.id[npidx];
+    auto fetch = [&](int g0, unsigned long long starts) {
+        const unsigned long long m = starts & (~0ull >> (63 - lane));
+        const unsigned e = sGrp[g0 + __popcll(m) - 1];
+        const int cf = (int)(e & 255u);
+        nfc = cf;
+        nslot = (int)((e >> 8) & 255u) * GS32 + (lane - (63 - __clzll(m)));
+        const unsigned nrel = (unsigned)(sBase[cf] + nslot);
+        qox = slot(P0, nrel); qoy = slot(P1, nrel); qoz = slot(P2, nrel);
+        qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
+        qs = slot(SP, nrel);
+        if (DIFF) qid = slot(ID, nrel);
     };
-    if (tid < T) fetch(tid);
-    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
-        const int c = nfc;
-        const long pidx = npidx;
-        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        const float ox = qox, oy = qoy, oz = qoz;
-        float pu = qpu, pv = qpv, pw = qpw;
+    const int ghi = __builtin_amdgcn_readfirstlane(
+        sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
+    int ng0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
+    unsigned long long nstarts = 0;
+    int nlanes = 0;
+    int gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes);
+    if (lane < nlanes) fetch(ng0, nstarts);
+    int carry_cell = -1, carry_cnt = 0;
+    while (nlanes > 0) {
+        const bool valid = lane < nlanes;
+        const int c = nfc, j = nslot;
+        const float ox0 = qox, oy0 = qoy, oz0 = qoz, pu0 = qpu, pv0 = qpv, pw0 = qpw;
         const unsigned char s = qs;
         const int pid = DIFF ? qid : 0;
-        if (fidx + PIC_BLOCK < T) fetch(fidx + PIC_BLOCK);
-        const long r = snap ? a.snap_off[tile] + fidx : 0;
-        if (snap && r < a.snap_room) {
-            a.snap.p[0][r] = ox;
-            a.snap.p[1][r] = oy;
-            a.snap.p[2][r] = oz;
-            a.snap.p[3][r] = pu;
-            a.snap.p[4][r] = pv;
-            a.snap.p[5][r] = pw;
-            a.snap.sp[r] = s;
-            a.snap.id[r] = pid;
+        ng0 = gnext;
+        gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes);
+        if (lane < nlanes) fetch(ng0, nstarts);
+        bool stays = false;
+        float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
+        if (valid) {
+            const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+            const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+            const float ox = ox0, oy = oy0, oz = oz0;
+            const long r = snap ? a.snap_off[tile] + sOff[c] + j : 0;
+            if (snap && r < a.snap_room) {
+                a.snap.p[0][r] = ox;
+                a.snap.p[1][r] = oy;
+                a.snap.p[2][r] = oz;
+                a.snap.p[3][r] = pu;
+                a.snap.p[4][r] = pv;
+                a.snap.p[5][r] = pw;
+                a.snap.sp[r] = s;
+                a.snap.id[r] = pid;
+            }
+
+            /* gather (weights as in k_push_math32.h), Boris push, deposit */
+            const Weights32 W = weights32(ox, oy, oz);
+            const int rdx = W.rd[0], rdy = W.rd[1], rdz = W.rd[2];
+            const float cx0 = W.c0[0], cx1 = 1.0f - cx0, cy0 = W.c0[1], cy1 = 1.0f - cy0,
+                        cz0 = W.c0[2], cz1 = 1.0f - cz0;
+            const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
+            const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
+            const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
+            const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
+            const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
+            const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
+            const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
+
+            const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, a.qm[s] * a.tau * 0.5f, a.tdh);
+            pu = B.p1[0];
+            pv = B.p1[1];
+            pw = B.p1[2];
+            const float o1x = ox + B.du[0], o1y = oy + B.du[1], o1z = oz + B.du[2];
+            {
+                const Relay32 R = relay32(ox, oy, oz, o1x, o1y, o1z, B.du, W.rd, a.hh);
+                const int m = R.m;
+                const float *cj = a.cj[s];
+                const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
+                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
+                if (m)
+                    vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
+                                       R.x2[2], o1x, o1y, o1z, cj);
+            }
+
+            /* migration by the cell-local offset */
+            int shx = (int)floorf(o1x), shy = (int)floorf(o1y), shz = (int)floorf(o1z);
+            nxo = o1x - (float)shx;
+            nyo = o1y - (float)shy;
+            nzo = o1z - (float)shz;
+            if (nxo >= 1.0f) { nxo -= 1.0f; shx += 1; }
+            if (nyo >= 1.0f) { nyo -= 1.0f; shy += 1; }
+            if (nzo >= 1.0f) { nzo -= 1.0f; shz += 1; }
+            stays = (shx | shy | shz) == 0;
+            if (!stays) {
+                atomicAdd(&sLeave[c], 1);
+                int rx = gi + shx, ry = gl + shy, rz = gk + shz;
+                if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
+                if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
+                if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
+                const int slot = atomicAdd(&sQ, 1);
+                if (slot < a.q.qcap) {
+                    const long e = tile * (long)a.q.qcap + slot;
+                    a.q.q[0][e] = nxo;
+                    a.q.q[1][e] = nyo;
+                    a.q.q[2][e] = nzo;
+                    a.q.q[3][e] = pu;
+                    a.q.q[4][e] = pv;
+                    a.q.q[5][e] = pw;
+                    a.q.sp[e] = s;
+                    if (DIFF) a.q.id[e] = pid;
+                    a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
+                    a.q.halo[e] = (unsigned char)(((ca + shx + 1) * BY + (cb + shy + 1)) * BZ +
+                                                  (cc + shz + 1));
+                }
+            }
         }
 
-        /* gather (weights as in k_push_math32.h), Boris push, deposit */
-        const Weights32 W = weights32(ox, oy, oz);
-        const int rdx = W.rd[0], rdy = W.rd[1], rdz = W.rd[2];
-        const float cx0 = W.c0[0], cx1 = 1.0f - cx0, cy0 = W.c0[1], cy1 = 1.0f - cy0,
-                    cz0 = W.c0[2], cz1 = 1.0f - cz0;
-        const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
-        const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
-        const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
-        const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
-        const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
-        const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
-        const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
-
-        const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, a.qm[s] * a.tau * 0.5f, a.tdh);
-        pu = B.p1[0];
-        pv = B.p1[1];
-        pw = B.p1[2];
-        const float o1x = ox + B.du[0], o1y = oy + B.du[1], o1z = oz + B.du[2];
-        {
-            const Relay32 R = relay32(ox, oy, oz, o1x, o1y, o1z, B.du, W.rd, a.hh);
-            const int m = R.m;
-            const float *cj = a.cj[s];
-            const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
-            vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
-            if (m)
-                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
-                                   R.x2[2], o1x, o1y, o1z, cj);
-        }
-
-        /* migration by the cell-local offset */
-        int shx = (int)floorf(o1x), shy = (int)floorf(o1y), shz = (int)floorf(o1z);
-        float nxo = o1x - (float)shx, nyo = o1y - (float)shy, nzo = o1z - (float)shz;
-        if (nxo >= 1.0f) { nxo -= 1.0f; shx += 1; }
-        if (nyo >= 1.0f) { nyo -= 1.0f; shy += 1; }
-        if (nzo >= 1.0f) { nzo -= 1.0f; shz += 1; }
-        if ((shx | shy | shz) == 0) {
-            a.p.p[0][pidx] = nxo;
-            a.p.p[1][pidx] = nyo;
-            a.p.p[2][pidx] = nzo;
-            a.p.p[3][pidx] = pu;
-            a.p.p[4][pidx] = pv;
-            a.p.p[5][pidx] = pw;
-        } else {
-            atomicAdd(&sLeave[c], 1);
-            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
-            int rx = gi + shx, ry = gl + shy, rz = gk + shz;
-            if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
-            if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
-            if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
-            const int slot = atomicAdd(&sQ, 1);
-            if (slot < a.q.qcap) {
-                const long e = tile * (long)a.q.qcap + slot;
-                a.q.q[0][e] = nxo;
-                a.q.q[1][e] = nyo;
-                a.q.q[2][e] = nzo;
-                a.q.q[3][e] = pu;
-                a.q.q[4][e] = pv;
-                a.q.q[5][e] = pw;
-                a.q.sp[e] = s;
-                if (DIFF) a.q.id[e] = pid;
-                a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
-                a.q.halo[e] = (unsigned char)(((ca + shx + 1) * BY + (cb + shy + 1)) * BZ +
-                                              (cc + shz + 1));
+        /* stayers compacted in flat order: slot rank of the cell */
+        const int rank = stable_rank(c, valid, stays, carry_cell, carry_cnt);
+        if (valid && stays) {
+            const unsigned dst = (unsigned)(sBase[c] + rank);
+            slot(P0, dst) = nxo;
+            slot(P1, dst) = nyo;
+            slot(P2, dst) = nzo;
+            slot(P3, dst) = pu;
+            slot(P4, dst) = pv;
+            slot(P5, dst) = pw;
+            if (rank != j) {
+                slot(SP, dst) = s;
+                if (DIFF) slot(ID, dst) = pid;
             }
         }
     }
@@ -303,20 +328,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
         if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
         const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-        const int n = sCnt[c], m = sLeave[c], np = n - m, base = sOff[c];
-        if (m > 0) {
-            int h = next_bit(sMask, base, 0, np, true);
-            int sidx = next_bit(sMask, base, np, n, false);
-            while (h < np && sidx < n) {
-                const long from = cell * cap + sidx, to = cell * cap + h;
-                for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
-                a.p.sp[to] = a.p.sp[from];
-                if (DIFF) a.p.id[to] = a.p.id[from];
-                h = next_bit(sMask, base, h + 1, np, true);
-                sidx = next_bit(sMask, base, sidx + 1, n, false);
-            }
-        }
-        a.p.count[cell] = np;
+        const int m = sLeave[c];
+        a.p.count[cell] = sCnt[c] - m;
         if (m > maxout) maxout = m;
     }
 
@@ -541,12 +554,14 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.snap_on = diff && diff->p[0] != nullptr;
     a.snap_off = snap_off;
     a.snap_room = snap_room;
+    /* dynamic LDS: one 4-byte entry per line group */
+    const size_t lds = (size_t)4 * TXC * TYC * TZC * ((p.cap + GS32 - 1) / GS32);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+                           dim3(PIC_BLOCK), lds, s, a);
     else
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+                           dim3(PIC_BLOCK), lds, s, a);
 }
 
 void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,

@@ -46,6 +46,7 @@ struct PushArgs {
     DevStatus *st;
     DiffArgs d;        /* differentiable form (DIFF instantiation only) */
     int snap_on;
+    int tab;           /* bytes of the particle-to-cell table in LDS */
 };
 
 /* one node contribution to the LDS J box: FP64 atomic add, or in the
@@ -132,9 +133,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
     __shared__ int sOut[C];
+    __shared__ int sDel[C];
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     __shared__ unsigned long long sDrop, sDup;
+    /* dynamic LDS: the leaver mask (one bit per particle slot of the tile),
+     * then the particle-to-cell table */
     extern __shared__ unsigned sMask[];
     double *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
@@ -198,34 +202,40 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     }
     __syncthreads();
     const int T = sOff[C];
+    unsigned char *const sTab = (unsigned char *)(sMask + (C * cap + 31) / 32);
     for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
+    const int sh = build_cell_table<TX, TY, TZ>(sOff, sDel, sTab, a.tab, g.Ny, g.Nz, cap, tid,
+                                                PIC_BLOCK);
     __syncthreads();
+
+    /* per-tile bases of the particle arrays: slot rel of the tile is
+     * cell-bucket slot pbase + rel */
+    const long pbase = (((long)I0 * g.Ny + L0) * g.Nz + K0) * cap;
+    double *const P0 = a.p.p[0] + pbase, *const P1 = a.p.p[1] + pbase,
+                  *const P2 = a.p.p[2] + pbase, *const P3 = a.p.p[3] + pbase,
+                  *const P4 = a.p.p[4] + pbase, *const P5 = a.p.p[5] + pbase;
+    unsigned char *const SP = a.p.sp + pbase;
+    int *const ID = DIFF ? a.p.id + pbase : nullptr;
 
     const double hx = g.hx, hy = g.hy, hz = g.hz, tau = g.tau;
     unsigned bad = 0;
     for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
-        int lo = 0, hi = C;
-        while (hi - lo > 1) {
-            int mid = (lo + hi) >> 1;
-            if (sOff[mid] <= fidx) lo = mid; else hi = mid;
-        }
-        const int c = lo;
-        const int j = fidx - sOff[c];
+        const int c = tile_cell(sTab, sOff, sh, fidx);
+        const unsigned rel = (unsigned)(fidx + sDel[c]);
         Geo G;
         G.ca = c / (TY * TZ);
         G.cb = (c / TZ) % TY;
         G.cc = c % TZ;
         const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
-        const long pidx = (((long)gi * g.Ny + gl) * g.Nz + gk) * cap + j;
         G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
         G.x0 = (double)gi * hx;
         G.y0 = (double)gl * hy;
         G.z0 = (double)gk * hz;
 
-        const double x = a.p.p[0][pidx], y = a.p.p[1][pidx], z = a.p.p[2][pidx];
-        double pu = a.p.p[3][pidx], pv = a.p.p[4][pidx], pw = a.p.p[5][pidx];
-        const unsigned char s = a.p.sp[pidx];
-        const int pid = DIFF ? a.p.id[pidx] : 0;
+        const double x = slot(P0, rel), y = slot(P1, rel), z = slot(P2, rel);
+        double pu = slot(P3, rel), pv = slot(P4, rel), pw = slot(P5, rel);
+        const unsigned char s = slot(SP, rel);
+        const int pid = DIFF ? slot(ID, rel) : 0;
         const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
         const long r = snap ? a.d.snap_off[tile] + fidx : 0;
         if (snap && r < a.d.snap_room) {
@@ -302,12 +312,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         if (!inside) atomicAdd(&sOut[c], 1);
 
         if (dir == 13) {
-            a.p.p[0][pidx] = xn;
-            a.p.p[1][pidx] = yn;
-            a.p.p[2][pidx] = zn;
-            a.p.p[3][pidx] = pu;
-            a.p.p[4][pidx] = pv;
-            a.p.p[5][pidx] = pw;
+            slot(P0, rel) = xn;
+            slot(P1, rel) = yn;
+            slot(P2, rel) = zn;
+            slot(P3, rel) = pu;
+            slot(P4, rel) = pv;
+            slot(P5, rel) = pw;
         } else {
             atomicAdd(&sLeave[c], 1);
             atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
@@ -606,6 +616,15 @@ size_t push_lds_bytes(const TileGeom &t, int cap) {
     return (size_t)((bits + 31) / 32) * 4;
 }
 
+namespace {
+/* LDS of k_push_deposit: the static arrays, the leaver mask and the cell
+ * table, which gets what is left of the share that keeps 4 workgroups
+ * (4 waves per SIMD) on a CU */
+constexpr int PUSH64_STATIC_LDS = 6 * ((TXC + 2) * (TYC + 2) * (TZC + 2)) * 8 +
+                                  JCOPIES * 3 * ((TXC + 2) * (TYC + 2) * (TZC + 2)) * 8 +
+                                  (TXC * TYC * TZC + 1) * 4 + 4 * TXC * TYC * TZC * 4 + 8 + 16;
+}  // namespace
+
 void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
                              const DiffArgs *diff, hipStream_t s) {
@@ -621,12 +640,15 @@ void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, Part
     a.st = st;
     a.d = diff ? *diff : DiffArgs{};
     a.snap_on = diff && diff->snap.p[0] != nullptr;
+    const int mask = (int)push_lds_bytes(t, p.cap);
+    a.tab = cell_table_bytes(160 * 1024 / 4, PUSH64_STATIC_LDS, mask, TXC * TYC * TZC * p.cap);
+    const size_t lds = (size_t)mask + a.tab;
     if (diff)
         hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+                           dim3(PIC_BLOCK), lds, s, a);
     else
         hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), push_lds_bytes(t, p.cap), s, a);
+                           dim3(PIC_BLOCK), lds, s, a);
 }
 
 void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s) {

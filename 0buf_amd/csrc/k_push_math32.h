@@ -76,7 +76,7 @@ __device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float E
     B.s[2] = sw;
     const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
     const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-    const float rden = 1.0f / (q1 + 1.0f + q2 + q3);
+    const float rden = __builtin_amdgcn_rcpf(q1 + 1.0f + q2 + q3);
     B.rden = rden;
     pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
     pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
@@ -133,7 +133,7 @@ __device__ __forceinline__ Relay32 relay32(float ox, float oy, float oz, float o
     }
     R.num = num;
     R.den = den;
-    const float sr = num / den;
+    const float sr = num * __builtin_amdgcn_rcpf(den);
     R.sr = sr;
     R.x2[0] = (m & 4) ? 0.5f : ox + dux * sr;
     R.x2[1] = (m & 2) ? 0.5f : oy + duy * sr;

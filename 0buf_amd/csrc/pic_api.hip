@@ -102,6 +102,17 @@ int queue_capacity(const TileGeom &t, int cap) {
 #ifndef PIC_CAP_ALIGN
 #define PIC_CAP_ALIGN 32
 #endif
+/* the push kernels address a tile's particle slots by 32-bit byte offsets
+ * from the bucket of its first cell: the tile's bucket span (from that cell
+ * to the end of its last cell's bucket) must stay below 2^32 bytes; their
+ * line groups (128 bytes of one array) number at most 256 per bucket */
+bool push_span_ok(const pic_ctx *c, int cap) {
+    const TileGeom &t = c->t;
+    const int elem = c->fp32 ? 4 : 8;
+    const long span = ((long)(t.TX - 1) * c->g.Ny * c->g.Nz + (long)(t.TY - 1) * c->g.Nz + t.TZ) * cap;
+    return span * elem < (1L << 32) && (long)cap * elem <= 256L * 128;
+}
+
 int auto_capacity(int maxc) {
     int cap = (int)(maxc * 1.5) + 8;
     cap = (cap + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN;
@@ -194,6 +205,7 @@ int commit(pic_ctx *c) {
     for (long k = 0; k < NC; k++) off[k + 1] += off[k];
     int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
     if (maxc > cap) cap = auto_capacity(maxc);
+    if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     int rc = c->fp32 ? commit_into(c, c->p32, c->q32, cap, off)
                      : commit_into(c, c->p, c->q, cap, off);
     if (rc) return rc;
@@ -216,6 +228,7 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     const int oldcap = parts.cap;
     const int cap = std::max(auto_capacity(maxc),
                              (oldcap + 8 + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN);
+    if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     P np;
     int rc = alloc_parts(np, NC, cap, false, c->diff);
     if (rc) return rc;
@@ -626,6 +639,7 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     for (long j = 0; j < NLK; j++) maxc = std::max(maxc, tot[j]);
     int cap = c->prm.bucket_capacity > 0 ? c->prm.bucket_capacity : auto_capacity(maxc);
     if (maxc > cap) cap = auto_capacity(maxc);
+    if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(c->dev));
     int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, c->diff)
                      : alloc_parts(c->p, NC, cap, true, c->diff);

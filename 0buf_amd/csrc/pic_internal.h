@@ -110,6 +110,158 @@ struct TileGeom {
     long ntiles;
 };
 
+/* The push kernels walk a tile's particles in flat order f = 0..T-1 (cell
+ * by cell, sOff[c] = first f of local cell c).  A particle's bucket slot is
+ * addressed by a 32-bit byte offset from per-tile base pointers (the host
+ * keeps a tile's bucket span below 2^32 bytes): rel(f) = f + sDel[c], with
+ * sDel[c] = the first slot of cell c relative to the tile's first cell,
+ * minus sOff[c]. */
+template <class T>
+__device__ __forceinline__ T &slot(T *base, unsigned rel) {
+    return *(T *)((char *)base + rel * (unsigned)sizeof(T));
+}
+
+/* Builds sDel and the particle-to-cell table sTab of a tile (tabn bytes of
+ * LDS, local cells numbered (a, b, c) -> (a*TY + b)*TZ + c < 256): entry k
+ * is the cell of particle k << sh, with sh the smallest shift that fits T
+ * particles into the table.  Returns sh; the caller synchronises. */
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ int build_cell_table(const int *sOff, int *sDel, unsigned char *sTab,
+                                                int tabn, int Ny, int Nz, int cap, int tid,
+                                                int nthreads) {
+    constexpr int C = TX * TY * TZ;
+    static_assert(C <= 256, "cell table entries are bytes");
+    const int T = sOff[C];
+    int sh = 0;
+    while (sh < 24 && ((T + (1 << sh) - 1) >> sh) > tabn) sh++;
+    for (int c = tid; c < C; c += nthreads) {
+        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+        sDel[c] = ((ca * Ny + cb) * Nz + cc) * cap - sOff[c];
+    }
+    for (int w = tid; w < 4 * C; w += nthreads) {
+        const int c = w >> 2, part = w & 3;
+        const int k1 = (sOff[c + 1] + (1 << sh) - 1) >> sh;
+        for (int k = ((sOff[c] + (1 << sh) - 1) >> sh) + part; k < k1; k += 4)
+            sTab[k] = (unsigned char)c;
+    }
+    return sh;
+}
+
+/* local cell of flat particle f */
+__device__ __forceinline__ int tile_cell(const unsigned char *sTab, const int *sOff, int sh,
+                                         int f) {
+    int c = sTab[f >> sh];
+    if (sh)
+        while (sOff[c + 1] <= f) c++;
+    return c;
+}
+
+/* Stable in-place compaction by wavefronts.  Each wavefront of a push
+ * workgroup owns whole cells of the tile: the flat range [split(w),
+ * split(w+1)), split at the first cell boundary at or after w*T/nw.  It walks
+ * the range in chunks of 64 consecutive particles; a stayer is stored at
+ * slot rank of its cell, rank = the stayers of its cell before it in flat
+ * order.  Only slots of particles already loaded by the wavefront are
+ * overwritten (rank <= its old slot), so no other wavefront and no later
+ * chunk is disturbed. */
+__device__ __forceinline__ int wave_split(const unsigned char *sTab, const int *sOff, int sh,
+                                          int T, int w, int nw) {
+    const int f = (int)(((long)w * T) / nw);
+    if (f >= T) return T;
+    const int c = tile_cell(sTab, sOff, sh, f);
+    return sOff[c] == f ? f : sOff[c + 1];
+}
+
+/* rank of this lane's particle among the stayers of its cell (valid lanes
+ * form a prefix of the wavefront and hold consecutive flat indices); the
+ * wavefront-uniform carry holds the cell of the previous chunk's last
+ * particle and that cell's stayers so far */
+__device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &carry_cell,
+                                           int &carry_cnt) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int cprev = __shfl_up(c, 1);
+    const unsigned long long R = __ballot(valid && (lane == 0 || c != cprev));
+    const unsigned long long S = __ballot(valid && stays);
+    const unsigned long long V = __ballot(valid);
+    const unsigned long long le = ~0ull >> (63 - lane);
+    const int rs = 63 - __clzll(R & le);
+    const int base = (rs == 0 && c == carry_cell) ? carry_cnt : 0;
+    const int rank = base + __popcll(S & (le >> 1) & (~0ull << rs));
+    if (V) {
+        const int L = 63 - __clzll(V);
+        const int cL = __builtin_amdgcn_readlane(c, L);
+        const unsigned long long leL = ~0ull >> (63 - L);
+        const int rsL = 63 - __clzll(R & leL);
+        const int baseL = (rsL == 0 && cL == carry_cell) ? carry_cnt : 0;
+        carry_cnt = baseL + __popcll(S & leL & (~0ull << rsL));
+        carry_cell = cL;
+    }
+    return rank;
+}
+
+/* Line groups.  A cell's bucket is split into groups of GS slots (GS slots =
+ * one 128-byte line of one particle array: 32 FP32 or 16 FP64 values); a
+ * group holds the particles of one line, 1..GS of them.  A wavefront walks
+ * its groups in order and packs whole groups into chunks of at most 64
+ * lanes, so no line is shared by two chunks (a line split across chunks is
+ * read twice when it leaves the L2 in between).  Group entry: cell | line
+ * << 8 | size << 16. */
+template <int TX, int TY, int TZ, int GS>
+__device__ __forceinline__ void build_groups(const int *sCnt, const int *sG, unsigned *sGrp,
+                                             int tid, int nthreads) {
+    constexpr int C = TX * TY * TZ;
+    static_assert(C <= 256, "group entries hold the cell in 8 bits");
+    for (int w = tid; w < 4 * C; w += nthreads) {
+        const int c = w >> 2, part = w & 3, n = sCnt[c];
+        for (int line = part; line < sG[c + 1] - sG[c]; line += 4) {
+            const int left = n - line * GS, sz = left < GS ? left : GS;
+            sGrp[sG[c] + line] = (unsigned)c | ((unsigned)line << 8) | ((unsigned)sz << 16);
+        }
+    }
+}
+
+/* first cell c with sOff[c] >= w*T/nw: the wavefronts own whole cells */
+template <int C>
+__device__ __forceinline__ int wave_first_cell(const int *sOff, int w, int nw) {
+    const int T = sOff[C];
+    const int f = (int)(((long)w * T) / nw);
+    if (w >= nw) return C;
+    int lo = -1, hi = C;    /* sOff[lo] < f <= sOff[hi] */
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sOff[mid] >= f) hi = mid; else lo = mid;
+    }
+    return hi;
+}
+
+/* chunk composition (wavefront-uniform): whole groups from g0 while they
+ * fit 64 lanes; lane start of each group in `starts`; returns the next
+ * group */
+__device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int gend,
+                                             unsigned long long &starts, int &lanes) {
+    int g = g0, n = 0;
+    unsigned long long m = 0;
+    while (g < gend) {
+        const int sz = (int)(__builtin_amdgcn_readfirstlane(sGrp[g]) >> 16);
+        if (n + sz > 64) break;
+        m |= 1ull << n;
+        n += sz;
+        g++;
+    }
+    starts = m;
+    lanes = n;
+    return g;
+}
+
+/* bytes of the cell table: what is left of a workgroup's LDS share after
+ * the static arrays and the leaver mask, at least 1 KiB, at most one entry
+ * per particle slot of the tile */
+inline int cell_table_bytes(int share, int static_bytes, int mask_bytes, int slots) {
+    int tab = (share - 64 - static_bytes - mask_bytes) / 256 * 256;
+    if (tab < 1024) tab = 1024;
+    return tab < slots ? tab : slots;
+}
+
 struct SpeciesTable {
     double m[PIC_MAX_SLOTS];
     double q_m[PIC_MAX_SLOTS];
