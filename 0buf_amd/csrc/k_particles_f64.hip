@@ -4,15 +4,16 @@
  * One workgroup per tile of TX x TY x TZ cells:
  *   1. stage E and H^{n+1/2} of the tile's node box ((TX+2)(TY+2)(TZ+2)
  *      nodes: a cell's stencil reaches global nodes owner+0..owner+2) in LDS;
- *   2. every lane takes particles of the tile in flat order (consecutive
- *      lanes -> consecutive slots of a cell bucket, so loads coalesce),
- *      gathers, pushes, and deposits its charge-conserving current into an
- *      LDS copy of the tile's J box (ds_add_f64);
+ *   2. each wavefront owns whole cells of the tile and walks their particles
+ *      in flat order, in chunks of whole 128-byte line groups (consecutive
+ *      lanes -> consecutive slots of a cell bucket, so loads coalesce and no
+ *      line is shared by two chunks); a lane gathers, pushes, and deposits
+ *      its charge-conserving current into an LDS copy of the tile's J box
+ *      (ds_add_f64);
  *   3. the particle is wrapped, its fly direction and receiving cells are
- *      computed with the reference predicates; stayers are written back in
- *      place, leavers go to the tile's departure region and leave a hole;
- *   4. holes are filled from the tail of each bucket (one lane per cell);
- *   5. the J box is added to the global arrays with FP64 atomics.
+ *      computed with the reference predicates; leavers go to the tile's
+ *      departure region, stayers are compacted in place in flat order;
+ *   4. the J box is added to the global arrays with FP64 atomics.
  * The arithmetic mirrors cell.h / particle.h in the reference's evaluation
  * order and is built with -ffp-contract=off.
  */
@@ -32,6 +33,7 @@ namespace {
 #endif
 
 constexpr int TXC = 2, TYC = 4, TZC = 8;
+constexpr int GS64 = 16;    /* slots of an FP64 array in one 128-byte line */
 constexpr int JCOPIES = PIC_JCOPIES64;
 
 struct PushArgs {
@@ -46,7 +48,6 @@ struct PushArgs {
     DevStatus *st;
     DiffArgs d;        /* differentiable form (DIFF instantiation only) */
     int snap_on;
-    int tab;           /* bytes of the particle-to-cell table in LDS */
 };
 
 /* one node contribution to the LDS J box: FP64 atomic add, or in the
@@ -103,24 +104,6 @@ __device__ __forceinline__ void pqr_lds(double *sJ, const Geo &c, const DivConst
     jadd<DIFF>(Jz, n + BY * BZ + BZ, sw * (dx * dy + s3), bad);
 }
 
-/* smallest pos in [from,to) whose bit (base+pos) equals `want`, else `to` */
-__device__ __forceinline__ int next_bit(const unsigned *mask, int base, int from, int to,
-                                        bool want) {
-    int pos = from;
-    while (pos < to) {
-        int gbit = base + pos;
-        unsigned w = mask[gbit >> 5];
-        if (!want) w = ~w;
-        w >>= (gbit & 31);
-        if (w) {
-            int p = pos + __builtin_ctz(w);
-            return p < to ? p : to;
-        }
-        pos += 32 - (gbit & 31);
-    }
-    return to;
-}
-
 /* 4 waves per SIMD (at most 128 VGPRs): the FP64 chains of the push are
  * latency-bound at 3 */
 template <int TX, int TY, int TZ, bool DIFF>
@@ -133,13 +116,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
     __shared__ int sOut[C];
-    __shared__ int sDel[C];
+    __shared__ int sBase[C];     /* first slot of each cell, relative to the tile's first cell */
+    __shared__ int sG[C + 1];    /* first line group of each cell */
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     __shared__ unsigned long long sDrop, sDup;
-    /* dynamic LDS: the leaver mask (one bit per particle slot of the tile),
-     * then the particle-to-cell table */
-    extern __shared__ unsigned sMask[];
+    extern __shared__ unsigned sGrp[];    /* line groups of the tile */
     double *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
     const Grid &g = a.g;
@@ -158,7 +140,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     const int tz = (int)(tile % a.t.ntz);
     const int ty = (int)((tile / a.t.ntz) % a.t.nty);
     const int tx = (int)(tile / ((long)a.t.ntz * a.t.nty));
-    const int I0 = tx * TX, L0 = ty * TY, K0 = tz * TZ;
+    /* workgroup-uniform (the 64-bit divisions above run on the vector
+     * unit; readfirstlane keeps the tile's base addresses scalar) */
+    const int I0 = __builtin_amdgcn_readfirstlane(tx * TX);
+    const int L0 = __builtin_amdgcn_readfirstlane(ty * TY);
+    const int K0 = __builtin_amdgcn_readfirstlane(tz * TZ);
     const int cap = a.p.cap;
 
     for (int c = tid; c < C; c += PIC_BLOCK) {
@@ -174,6 +160,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         sCnt[c] = cnt;
         sLeave[c] = 0;
         sOut[c] = 0;
+        sBase[c] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
     }
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
         int fi = e / NB, nn = e % NB;
@@ -193,19 +180,18 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     }
     __syncthreads();
     if (tid == 0) {
-        int s = 0;
+        int s = 0, ng = 0;
         for (int c = 0; c < C; c++) {
             sOff[c] = s;
+            sG[c] = ng;
             s += sCnt[c];
+            ng += (sCnt[c] + GS64 - 1) / GS64;
         }
         sOff[C] = s;
+        sG[C] = ng;
     }
     __syncthreads();
-    const int T = sOff[C];
-    unsigned char *const sTab = (unsigned char *)(sMask + (C * cap + 31) / 32);
-    for (int w = tid; w < (T + 31) / 32; w += PIC_BLOCK) sMask[w] = 0u;
-    const int sh = build_cell_table<TX, TY, TZ>(sOff, sDel, sTab, a.tab, g.Ny, g.Nz, cap, tid,
-                                                PIC_BLOCK);
+    build_groups<TX, TY, TZ, GS64>(sCnt, sG, sGrp, tid, PIC_BLOCK);
     __syncthreads();
 
     /* per-tile bases of the particle arrays: slot rel of the tile is
@@ -219,193 +205,216 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
 
     const double hx = g.hx, hy = g.hy, hz = g.hz, tau = g.tau;
     unsigned bad = 0;
-    for (int fidx = tid; fidx < T; fidx += PIC_BLOCK) {
-        const int c = tile_cell(sTab, sOff, sh, fidx);
-        const unsigned rel = (unsigned)(fidx + sDel[c]);
-        Geo G;
-        G.ca = c / (TY * TZ);
-        G.cb = (c / TZ) % TY;
-        G.cc = c % TZ;
-        const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
-        G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
-        G.x0 = (double)gi * hx;
-        G.y0 = (double)gl * hy;
-        G.z0 = (double)gk * hz;
+    /* each wavefront owns whole cells and walks their line groups in chunks
+     * of whole groups; stayers are compacted in place in flat order */
+    const int lane = tid & 63, wv = tid >> 6;
+    const int ghi = __builtin_amdgcn_readfirstlane(
+        sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
+    int g0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
+    int carry_cell = -1, carry_cnt = 0;
+    while (g0 < ghi) {
+        unsigned long long starts = 0;
+        int lanes = 0;
+        const int gnext = compose_chunk(sGrp, g0, ghi, starts, lanes);
+        const bool valid = lane < lanes;
+        int c = 0, j = 0;
+        bool stays = false;
+        double xn = 0, yn = 0, zn = 0, pu = 0, pv = 0, pw = 0;
+        unsigned char s = 0;
+        int pid = 0;
+        if (valid) {
+            const unsigned long long msk = starts & (~0ull >> (63 - lane));
+            const unsigned ge = sGrp[g0 + __popcll(msk) - 1];
+            c = (int)(ge & 255u);
+            j = (int)((ge >> 8) & 255u) * GS64 + (lane - (63 - __clzll(msk)));
+            const unsigned rel = (unsigned)(sBase[c] + j);
+            Geo G;
+            G.ca = c / (TY * TZ);
+            G.cb = (c / TZ) % TY;
+            G.cc = c % TZ;
+            const int gi = I0 + G.ca, gl = L0 + G.cb, gk = K0 + G.cc;
+            G.hx = hx; G.hy = hy; G.hz = hz; G.tau = tau;
+            G.x0 = (double)gi * hx;
+            G.y0 = (double)gl * hy;
+            G.z0 = (double)gk * hz;
 
-        const double x = slot(P0, rel), y = slot(P1, rel), z = slot(P2, rel);
-        double pu = slot(P3, rel), pv = slot(P4, rel), pw = slot(P5, rel);
-        const unsigned char s = slot(SP, rel);
-        const int pid = DIFF ? slot(ID, rel) : 0;
-        const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
-        const long r = snap ? a.d.snap_off[tile] + fidx : 0;
-        if (snap && r < a.d.snap_room) {
-            a.d.snap.p[0][r] = x;
-            a.d.snap.p[1][r] = y;
-            a.d.snap.p[2][r] = z;
-            a.d.snap.p[3][r] = pu;
-            a.d.snap.p[4][r] = pv;
-            a.d.snap.p[5][r] = pw;
-            a.d.snap.sp[r] = s;
-            a.d.snap.id[r] = pid;
-        }
-
-        const double frx = divc(x - G.x0, hx, a.dc.rhx);
-        const double fry = divc(y - G.y0, hy, a.dc.rhy);
-        const double frz = divc(z - G.z0, hz, a.dc.rhz);
-        double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
-        if (!(x < 0 || y < 0 || z < 0)) {
-            const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
-            const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
-            const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
-                         s51 = 1. - s5, s61 = 1. - s6;
-            if (!W.ok) {
-                bad = 1;
-            } else {
-                const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
-                Ex = interp<BY, BZ>(sF + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, s1, s11, s4, s41, s6, s61);
-                Ey = interp<BY, BZ>(sF + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, s2, s21, s3, s31, s6, s61);
-                Ez = interp<BY, BZ>(sF + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, s2, s21, s4, s41, s5, s51);
-                Hx = interp<BY, BZ>(sF + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, s2, s21, s3, s31, s5, s51);
-                Hy = interp<BY, BZ>(sF + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, s1, s11, s4, s41, s5, s51);
-                Hz = interp<BY, BZ>(sF + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, s1, s11, s3, s31, s6, s61);
+            const double x = slot(P0, rel), y = slot(P1, rel), z = slot(P2, rel);
+            pu = slot(P3, rel);
+            pv = slot(P4, rel);
+            pw = slot(P5, rel);
+            s = slot(SP, rel);
+            pid = DIFF ? slot(ID, rel) : 0;
+            const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
+            const long r = snap ? a.d.snap_off[tile] + sOff[c] + j : 0;
+            if (snap && r < a.d.snap_room) {
+                a.d.snap.p[0][r] = x;
+                a.d.snap.p[1][r] = y;
+                a.d.snap.p[2][r] = z;
+                a.d.snap.p[3][r] = pu;
+                a.d.snap.p[4][r] = pv;
+                a.d.snap.p[5][r] = pw;
+                a.d.snap.sp[r] = s;
+                a.d.snap.id[r] = pid;
             }
-        }
 
-        /* Particle::Move (particle.h:160-202) */
-        const Boris B = boris_move(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, q_m * tau * 0.5);
-        pu = B.p1[0];
-        pv = B.p1[1];
-        pw = B.p1[2];
-        const double x1 = x + tau * (B.g2 * pu);
-        const double y1 = y + tau * (B.g2 * pv);
-        const double z1 = z + tau * (B.g2 * pw);
-
-        /* CurrentToMesh (cell.h:1306-1422): one code path for every lane;
-         * without a crossing the first segment is the whole path (x2 = x1)
-         * and the second is skipped */
-        {
-            const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
-            const int m = R.m;
-            const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
-            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y, e1z,
-                                      mass, bad, true);
-            pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i2x, R.i2y, R.i2z, R.x2x, R.x2y, R.x2z, x1,
-                                      y1, z1, mass, bad, m != 0);
-        }
-
-        /* x <- x1, Reflect (cell.h:1410-1478) */
-        const double xn = reflect1(x1, g.Lx), yn = reflect1(y1, g.Ly), zn = reflect1(z1, g.Lz);
-
-        /* flyDirection (cell.h:694-701) with the owner cell's geometry */
-        int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
-                      ? 2
-                      : (((xn < G.x0) || ((xn > g.Lx - hx) && (gi == 0))) ? 0 : 1);
-        int ddy = (((yn > G.y0 + hy) && (yn < G.y0 + 2 * hy)) || ((yn < hy) && (gl == g.Ny - 1)))
-                      ? 2
-                      : (((yn < G.y0) || ((yn > g.Ly - hy) && (gl == 0))) ? 0 : 1);
-        int ddz = (((zn > G.z0 + hz) && (zn < G.z0 + 2 * hz)) || ((zn < hz) && (gk == g.Nz - 1)))
-                      ? 2
-                      : (((zn < G.z0) || ((zn > g.Lz - hz) && (gk == 0))) ? 0 : 1);
-        const int dir = ddx * 9 + ddy * 3 + ddz;
-        const bool inside = (G.x0 <= xn) && (xn < G.x0 + hx) && (G.y0 <= yn) &&
-                            (yn < G.y0 + hy) && (G.z0 <= zn) && (zn < G.z0 + hz);
-        if (!inside) atomicAdd(&sOut[c], 1);
-
-        if (dir == 13) {
-            slot(P0, rel) = xn;
-            slot(P1, rel) = yn;
-            slot(P2, rel) = zn;
-            slot(P3, rel) = pu;
-            slot(P4, rel) = pv;
-            slot(P5, rel) = pw;
-        } else {
-            atomicAdd(&sLeave[c], 1);
-            atomicOr(&sMask[fidx >> 5], 1u << (fidx & 31));
-            /* GPU_ArrangeFlights (kernels.cu:316-370): every bulk cell of the
-             * 27-neighbourhood (getWrapCellNumber) whose isPointInCell
-             * (cell.h:717) holds receives a copy; per axis the test is
-             * separable, so the receivers are a product of per-axis sets. */
-            /* per axis, bit d+1 set when neighbour d in {-1,0,1} accepts */
-            unsigned bx = 0, by = 0, bz = 0;
-            for (int d = -1; d <= 1; d++) {
-                int r = gi + d;
-                if (r >= g.Nx) r -= g.Nx; else if (r < 0) r += g.Nx;
-                double r0 = (double)r * hx;
-                if (r0 <= xn && xn < r0 + hx) bx |= 1u << (d + 1);
-                r = gl + d;
-                if (r >= g.Ny) r -= g.Ny; else if (r < 0) r += g.Ny;
-                r0 = (double)r * hy;
-                if (r0 <= yn && yn < r0 + hy) by |= 1u << (d + 1);
-                r = gk + d;
-                if (r >= g.Nz) r -= g.Nz; else if (r < 0) r += g.Nz;
-                r0 = (double)r * hz;
-                if (r0 <= zn && zn < r0 + hz) bz |= 1u << (d + 1);
+            const double frx = divc(x - G.x0, hx, a.dc.rhx);
+            const double fry = divc(y - G.y0, hy, a.dc.rhy);
+            const double frz = divc(z - G.z0, hz, a.dc.rhz);
+            double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
+            if (!(x < 0 || y < 0 || z < 0)) {
+                const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
+                const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
+                const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
+                             s51 = 1. - s5, s61 = 1. - s6;
+                if (!W.ok) {
+                    bad = 1;
+                } else {
+                    const int ua = G.ca - 2, va = G.cb - 2, wa = G.cc - 2;
+                    Ex = interp<BY, BZ>(sF + 0 * NB, ua + W.ix, va + W.i1y, wa + W.i1z, s1, s11, s4, s41, s6, s61);
+                    Ey = interp<BY, BZ>(sF + 1 * NB, ua + W.i1x, va + W.iy, wa + W.i1z, s2, s21, s3, s31, s6, s61);
+                    Ez = interp<BY, BZ>(sF + 2 * NB, ua + W.i1x, va + W.i1y, wa + W.iz, s2, s21, s4, s41, s5, s51);
+                    Hx = interp<BY, BZ>(sF + 3 * NB, ua + W.i1x, va + W.iy, wa + W.iz, s2, s21, s3, s31, s5, s51);
+                    Hy = interp<BY, BZ>(sF + 4 * NB, ua + W.ix, va + W.i1y, wa + W.iz, s1, s11, s4, s41, s5, s51);
+                    Hz = interp<BY, BZ>(sF + 5 * NB, ua + W.ix, va + W.iy, wa + W.i1z, s1, s11, s3, s31, s6, s61);
+                }
             }
-            const int ncopy = __popc(bx) * __popc(by) * __popc(bz);
-            if (ncopy == 0) atomicAdd(&sDrop, 1ull);
-            if (ncopy > 1) atomicAdd(&sDup, (unsigned long long)(ncopy - 1));
-            if (ncopy > 0) {
-                int slot = atomicAdd(&sQ, ncopy);
-                const long qb = tile * (long)a.q.qcap;
-                int kq = 0;
-                /* receivers in the reference's scan order (x, then y, then z) */
-                for (int ax = 0; ax < 3; ax++) {
-                    if (!((bx >> ax) & 1u)) continue;
-                    int rx = gi + ax - 1;
-                    if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
-                    for (int ay = 0; ay < 3; ay++) {
-                        if (!((by >> ay) & 1u)) continue;
-                        int ry = gl + ay - 1;
-                        if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
-                        for (int az = 0; az < 3; az++) {
-                            if (!((bz >> az) & 1u)) continue;
-                            int rz = gk + az - 1;
-                            if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
-                            const int idx = slot + kq++;
-                            if (idx >= a.q.qcap) continue;
-                            const long e = qb + idx;
-                            a.q.q[0][e] = xn;
-                            a.q.q[1][e] = yn;
-                            a.q.q[2][e] = zn;
-                            a.q.q[3][e] = pu;
-                            a.q.q[4][e] = pv;
-                            a.q.q[5][e] = pw;
-                            a.q.sp[e] = s;
-                            if (DIFF) a.q.id[e] = pid;
-                            a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
-                            a.q.halo[e] = (unsigned char)(((G.ca + ax) * BY + (G.cb + ay)) * BZ +
-                                                          (G.cc + az));
+
+            /* Particle::Move (particle.h:160-202) */
+            const Boris B = boris_move(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, q_m * tau * 0.5);
+            pu = B.p1[0];
+            pv = B.p1[1];
+            pw = B.p1[2];
+            const double x1 = x + tau * (B.g2 * pu);
+            const double y1 = y + tau * (B.g2 * pv);
+            const double z1 = z + tau * (B.g2 * pw);
+
+            /* CurrentToMesh (cell.h:1306-1422): one code path for every lane;
+             * without a crossing the first segment is the whole path (x2 = x1)
+             * and the second is skipped */
+            {
+                const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
+                const int m = R.m;
+                const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
+                pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y, e1z,
+                                          mass, bad, true);
+                pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i2x, R.i2y, R.i2z, R.x2x, R.x2y, R.x2z, x1,
+                                          y1, z1, mass, bad, m != 0);
+            }
+
+            /* x <- x1, Reflect (cell.h:1410-1478) */
+            xn = reflect1(x1, g.Lx);
+            yn = reflect1(y1, g.Ly);
+            zn = reflect1(z1, g.Lz);
+
+            /* flyDirection (cell.h:694-701) with the owner cell's geometry */
+            int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
+                          ? 2
+                          : (((xn < G.x0) || ((xn > g.Lx - hx) && (gi == 0))) ? 0 : 1);
+            int ddy = (((yn > G.y0 + hy) && (yn < G.y0 + 2 * hy)) || ((yn < hy) && (gl == g.Ny - 1)))
+                          ? 2
+                          : (((yn < G.y0) || ((yn > g.Ly - hy) && (gl == 0))) ? 0 : 1);
+            int ddz = (((zn > G.z0 + hz) && (zn < G.z0 + 2 * hz)) || ((zn < hz) && (gk == g.Nz - 1)))
+                          ? 2
+                          : (((zn < G.z0) || ((zn > g.Lz - hz) && (gk == 0))) ? 0 : 1);
+            const int dir = ddx * 9 + ddy * 3 + ddz;
+            const bool inside = (G.x0 <= xn) && (xn < G.x0 + hx) && (G.y0 <= yn) &&
+                                (yn < G.y0 + hy) && (G.z0 <= zn) && (zn < G.z0 + hz);
+            if (!inside) atomicAdd(&sOut[c], 1);
+
+            stays = dir == 13;
+            if (!stays) {
+                atomicAdd(&sLeave[c], 1);
+                /* GPU_ArrangeFlights (kernels.cu:316-370): every bulk cell of the
+                 * 27-neighbourhood (getWrapCellNumber) whose isPointInCell
+                 * (cell.h:717) holds receives a copy; per axis the test is
+                 * separable, so the receivers are a product of per-axis sets. */
+                /* per axis, bit d+1 set when neighbour d in {-1,0,1} accepts */
+                unsigned bx = 0, by = 0, bz = 0;
+                for (int d = -1; d <= 1; d++) {
+                    int r = gi + d;
+                    if (r >= g.Nx) r -= g.Nx; else if (r < 0) r += g.Nx;
+                    double r0 = (double)r * hx;
+                    if (r0 <= xn && xn < r0 + hx) bx |= 1u << (d + 1);
+                    r = gl + d;
+                    if (r >= g.Ny) r -= g.Ny; else if (r < 0) r += g.Ny;
+                    r0 = (double)r * hy;
+                    if (r0 <= yn && yn < r0 + hy) by |= 1u << (d + 1);
+                    r = gk + d;
+                    if (r >= g.Nz) r -= g.Nz; else if (r < 0) r += g.Nz;
+                    r0 = (double)r * hz;
+                    if (r0 <= zn && zn < r0 + hz) bz |= 1u << (d + 1);
+                }
+                const int ncopy = __popc(bx) * __popc(by) * __popc(bz);
+                if (ncopy == 0) atomicAdd(&sDrop, 1ull);
+                if (ncopy > 1) atomicAdd(&sDup, (unsigned long long)(ncopy - 1));
+                if (ncopy > 0) {
+                    int slot = atomicAdd(&sQ, ncopy);
+                    const long qb = tile * (long)a.q.qcap;
+                    int kq = 0;
+                    /* receivers in the reference's scan order (x, then y, then z) */
+                    for (int ax = 0; ax < 3; ax++) {
+                        if (!((bx >> ax) & 1u)) continue;
+                        int rx = gi + ax - 1;
+                        if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
+                        for (int ay = 0; ay < 3; ay++) {
+                            if (!((by >> ay) & 1u)) continue;
+                            int ry = gl + ay - 1;
+                            if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
+                            for (int az = 0; az < 3; az++) {
+                                if (!((bz >> az) & 1u)) continue;
+                                int rz = gk + az - 1;
+                                if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
+                                const int idx = slot + kq++;
+                                if (idx >= a.q.qcap) continue;
+                                const long e = qb + idx;
+                                a.q.q[0][e] = xn;
+                                a.q.q[1][e] = yn;
+                                a.q.q[2][e] = zn;
+                                a.q.q[3][e] = pu;
+                                a.q.q[4][e] = pv;
+                                a.q.q[5][e] = pw;
+                                a.q.sp[e] = s;
+                                if (DIFF) a.q.id[e] = pid;
+                                a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
+                                a.q.halo[e] = (unsigned char)(((G.ca + ax) * BY + (G.cb + ay)) * BZ +
+                                                              (G.cc + az));
+                            }
                         }
                     }
                 }
             }
         }
+
+        /* stayers compacted in flat order (the reference removes leavers by
+         * swap-with-last, removeParticleFromSurfaceDevice, cell.h:391-432;
+         * the order inside a cell changes no result but the rounding of the
+         * FP64 atomic sums, which have no fixed order either) */
+        const int rank = stable_rank(c, valid, stays, carry_cell, carry_cnt);
+        if (valid && stays) {
+            const unsigned dst = (unsigned)(sBase[c] + rank);
+            slot(P0, dst) = xn;
+            slot(P1, dst) = yn;
+            slot(P2, dst) = zn;
+            slot(P3, dst) = pu;
+            slot(P4, dst) = pv;
+            slot(P5, dst) = pw;
+            if (rank != j) {
+                slot(SP, dst) = s;
+                if (DIFF) slot(ID, dst) = pid;
+            }
+        }
+        g0 = gnext;
     }
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();
 
-    /* holes -> tail stayers, one lane per cell; the swap-with-last idea of
-     * removeParticleFromSurfaceDevice (cell.h:391-432) */
     int maxout = 0;
     for (int c = tid; c < C; c += PIC_BLOCK) {
         const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
         const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
         if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
         const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-        const int n = sCnt[c], m = sLeave[c], np = n - m, base = sOff[c];
-        if (m > 0) {
-            int h = next_bit(sMask, base, 0, np, true);
-            int sidx = next_bit(sMask, base, np, n, false);
-            while (h < np && sidx < n) {
-                const long from = cell * cap + sidx, to = cell * cap + h;
-                for (int q = 0; q < 6; q++) a.p.p[q][to] = a.p.p[q][from];
-                a.p.sp[to] = a.p.sp[from];
-                if (DIFF) a.p.id[to] = a.p.id[from];
-                h = next_bit(sMask, base, h + 1, np, true);
-                sidx = next_bit(sMask, base, sidx + 1, n, false);
-            }
-        }
-        a.p.count[cell] = np;
+        a.p.count[cell] = sCnt[c] - sLeave[c];
         if (sOut[c] > maxout) maxout = sOut[c];
     }
 
@@ -611,19 +620,6 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 
 }  // namespace
 
-size_t push_lds_bytes(const TileGeom &t, int cap) {
-    long bits = (long)t.TX * t.TY * t.TZ * cap;
-    return (size_t)((bits + 31) / 32) * 4;
-}
-
-namespace {
-/* LDS of k_push_deposit: the static arrays, the leaver mask and the cell
- * table, which gets what is left of the share that keeps 4 workgroups
- * (4 waves per SIMD) on a CU */
-constexpr int PUSH64_STATIC_LDS = 6 * ((TXC + 2) * (TYC + 2) * (TZC + 2)) * 8 +
-                                  JCOPIES * 3 * ((TXC + 2) * (TYC + 2) * (TZC + 2)) * 8 +
-                                  (TXC * TYC * TZC + 1) * 4 + 4 * TXC * TYC * TZC * 4 + 8 + 16;
-}  // namespace
 
 void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
@@ -640,9 +636,8 @@ void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, Part
     a.st = st;
     a.d = diff ? *diff : DiffArgs{};
     a.snap_on = diff && diff->snap.p[0] != nullptr;
-    const int mask = (int)push_lds_bytes(t, p.cap);
-    a.tab = cell_table_bytes(160 * 1024 / 4, PUSH64_STATIC_LDS, mask, TXC * TYC * TZC * p.cap);
-    const size_t lds = (size_t)mask + a.tab;
+    /* dynamic LDS: one 4-byte entry per line group */
+    const size_t lds = (size_t)4 * TXC * TYC * TZC * ((p.cap + GS64 - 1) / GS64);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), lds, s, a);

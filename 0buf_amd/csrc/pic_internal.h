@@ -110,72 +110,30 @@ struct TileGeom {
     long ntiles;
 };
 
-/* The push kernels walk a tile's particles in flat order f = 0..T-1 (cell
- * by cell, sOff[c] = first f of local cell c).  A particle's bucket slot is
- * addressed by a 32-bit byte offset from per-tile base pointers (the host
- * keeps a tile's bucket span below 2^32 bytes): rel(f) = f + sDel[c], with
- * sDel[c] = the first slot of cell c relative to the tile's first cell,
- * minus sOff[c]. */
+/* The push kernels number a tile's particles in flat order f = 0..T-1
+ * (cell by cell, sOff[c] = first f of local cell c; the step records use
+ * this order).  A particle's bucket slot is addressed by a 32-bit byte offset
+ * from per-tile base pointers (the host keeps a tile's bucket span below
+ * 2^32 bytes): slot j of local cell c is rel = sBase[c] + j, sBase[c] = the
+ * first slot of cell c relative to the tile's first cell. */
 template <class T>
 __device__ __forceinline__ T &slot(T *base, unsigned rel) {
     return *(T *)((char *)base + rel * (unsigned)sizeof(T));
 }
 
-/* Builds sDel and the particle-to-cell table sTab of a tile (tabn bytes of
- * LDS, local cells numbered (a, b, c) -> (a*TY + b)*TZ + c < 256): entry k
- * is the cell of particle k << sh, with sh the smallest shift that fits T
- * particles into the table.  Returns sh; the caller synchronises. */
-template <int TX, int TY, int TZ>
-__device__ __forceinline__ int build_cell_table(const int *sOff, int *sDel, unsigned char *sTab,
-                                                int tabn, int Ny, int Nz, int cap, int tid,
-                                                int nthreads) {
-    constexpr int C = TX * TY * TZ;
-    static_assert(C <= 256, "cell table entries are bytes");
-    const int T = sOff[C];
-    int sh = 0;
-    while (sh < 24 && ((T + (1 << sh) - 1) >> sh) > tabn) sh++;
-    for (int c = tid; c < C; c += nthreads) {
-        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        sDel[c] = ((ca * Ny + cb) * Nz + cc) * cap - sOff[c];
-    }
-    for (int w = tid; w < 4 * C; w += nthreads) {
-        const int c = w >> 2, part = w & 3;
-        const int k1 = (sOff[c + 1] + (1 << sh) - 1) >> sh;
-        for (int k = ((sOff[c] + (1 << sh) - 1) >> sh) + part; k < k1; k += 4)
-            sTab[k] = (unsigned char)c;
-    }
-    return sh;
-}
-
-/* local cell of flat particle f */
-__device__ __forceinline__ int tile_cell(const unsigned char *sTab, const int *sOff, int sh,
-                                         int f) {
-    int c = sTab[f >> sh];
-    if (sh)
-        while (sOff[c + 1] <= f) c++;
-    return c;
-}
-
 /* Stable in-place compaction by wavefronts.  Each wavefront of a push
- * workgroup owns whole cells of the tile: the flat range [split(w),
- * split(w+1)), split at the first cell boundary at or after w*T/nw.  It walks
- * the range in chunks of 64 consecutive particles; a stayer is stored at
- * slot rank of its cell, rank = the stayers of its cell before it in flat
- * order.  Only slots of particles already loaded by the wavefront are
- * overwritten (rank <= its old slot), so no other wavefront and no later
- * chunk is disturbed. */
-__device__ __forceinline__ int wave_split(const unsigned char *sTab, const int *sOff, int sh,
-                                          int T, int w, int nw) {
-    const int f = (int)(((long)w * T) / nw);
-    if (f >= T) return T;
-    const int c = tile_cell(sTab, sOff, sh, f);
-    return sOff[c] == f ? f : sOff[c + 1];
-}
-
-/* rank of this lane's particle among the stayers of its cell (valid lanes
- * form a prefix of the wavefront and hold consecutive flat indices); the
+ * workgroup owns whole cells of the tile (wave_first_cell) and walks them in
+ * flat order, one chunk of up to 64 consecutive particles at a time
+ * (compose_chunk).  A stayer is stored at slot rank of its cell, rank = the
+ * stayers of its cell before it in flat order.  Only slots of particles the
+ * wavefront has already loaded are overwritten (rank <= the particle's own
+ * slot), so no other wavefront and no later chunk is disturbed, and the
+ * buckets need no second pass.
+ *
+ * stable_rank: the rank of this lane's particle (valid lanes form a prefix
+ * of the wavefront and hold consecutive flat indices); the
  * wavefront-uniform carry holds the cell of the previous chunk's last
- * particle and that cell's stayers so far */
+ * particle and that cell's stayers so far. */
 __device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &carry_cell,
                                            int &carry_cnt) {
     const int lane = (int)(threadIdx.x & 63);
@@ -220,12 +178,12 @@ __device__ __forceinline__ void build_groups(const int *sCnt, const int *sG, uns
     }
 }
 
-/* first cell c with sOff[c] >= w*T/nw: the wavefronts own whole cells */
+/* first cell c with sOff[c] >= w*T/nw (C for w = nw): wavefront w owns
+ * cells [first(w), first(w+1)) */
 template <int C>
 __device__ __forceinline__ int wave_first_cell(const int *sOff, int w, int nw) {
-    const int T = sOff[C];
-    const int f = (int)(((long)w * T) / nw);
     if (w >= nw) return C;
+    const int f = (int)(((long)w * sOff[C]) / nw);
     int lo = -1, hi = C;    /* sOff[lo] < f <= sOff[hi] */
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
@@ -251,15 +209,6 @@ __device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int g
     starts = m;
     lanes = n;
     return g;
-}
-
-/* bytes of the cell table: what is left of a workgroup's LDS share after
- * the static arrays and the leaver mask, at least 1 KiB, at most one entry
- * per particle slot of the tile */
-inline int cell_table_bytes(int share, int static_bytes, int mask_bytes, int slots) {
-    int tab = (share - 64 - static_bytes - mask_bytes) / 256 * 256;
-    if (tab < 1024) tab = 1024;
-    return tab < slots ? tab : slots;
 }
 
 struct SpeciesTable {
@@ -300,7 +249,6 @@ struct DiffArgs {
 };
 
 /* particle launchers (k_particles_f64.hip) */
-size_t push_lds_bytes(const TileGeom &t, int cap);
 void launch_push_deposit_f64(const Grid &g, const TileGeom &t, FieldPtrs f, PartPtrs p,
                              QueuePtrs q, SpeciesTable sp, int dep_cap, DevStatus *st,
                              const DiffArgs *diff, hipStream_t s);
