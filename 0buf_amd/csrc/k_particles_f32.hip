@@ -110,6 +110,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sLeave[C];
     __shared__ int sBase[C];     /* first slot of each cell, relative to the tile's first cell */
     __shared__ int sG[C + 1];    /* first line group of each cell */
+    __shared__ int sHalo[NB];    /* leavers bound for each cell of the halo box */
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     extern __shared__ unsigned sGrp[];    /* line groups of the tile */
@@ -159,6 +160,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         sF[e] = val;
     }
     for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
+    for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
@@ -282,24 +284,15 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
             stays = (shx | shy | shz) == 0;
             if (!stays) {
                 atomicAdd(&sLeave[c], 1);
-                int rx = gi + shx, ry = gl + shy, rz = gk + shz;
-                if (rx >= g.Nx) rx -= g.Nx; else if (rx < 0) rx += g.Nx;
-                if (ry >= g.Ny) ry -= g.Ny; else if (ry < 0) ry += g.Ny;
-                if (rz >= g.Nz) rz -= g.Nz; else if (rz < 0) rz += g.Nz;
-                const int slot = atomicAdd(&sQ, 1);
-                if (slot < a.q.qcap) {
-                    const long e = tile * (long)a.q.qcap + slot;
-                    a.q.q[0][e] = nxo;
-                    a.q.q[1][e] = nyo;
-                    a.q.q[2][e] = nzo;
-                    a.q.q[3][e] = pu;
-                    a.q.q[4][e] = pv;
-                    a.q.q[5][e] = pw;
-                    a.q.sp[e] = s;
-                    if (DIFF) a.q.id[e] = pid;
-                    a.q.dest[e] = (int)(((long)rx * g.Ny + ry) * g.Nz + rz);
-                    a.q.halo[e] = (unsigned char)(((ca + shx + 1) * BY + (cb + shy + 1)) * BZ +
-                                                  (cc + shz + 1));
+                const int qs_ = atomicAdd(&sQ, 1);
+                if (qs_ < a.q.qcap) {
+                    /* destination in the halo box (|shift| <= 1 per axis) */
+                    const int h = ((ca + shx + 1) * BY + (cb + shy + 1)) * BZ + (cc + shz + 1);
+                    const int hr = atomicAdd(&sHalo[h], 1);
+                    const unsigned key = (unsigned)s | ((unsigned)h << 8) | ((unsigned)hr << 16);
+                    float4 *q = a.q.e + 2 * (tile * (long)a.q.qcap + qs_);
+                    q[0] = make_float4(nxo, nyo, nzo, pu);
+                    q[1] = make_float4(pv, pw, __uint_as_float(key), __int_as_float(pid));
                 }
             }
         }
@@ -348,6 +341,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         if (a.dep_cap > 0 && maxout > a.dep_cap) atomicOr(&sFlags, (unsigned)PIC_F_TOO_MANY);
         if (maxout > a.st->max_dep) atomicMax(&a.st->max_dep, maxout);
     }
+    for (int h = tid; h < NB; h += PIC_BLOCK) a.q.hcnt[tile * NB + h] = sHalo[h];
     __syncthreads();
     if (tid == 0) {
         int nq = sQ;
@@ -399,19 +393,96 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_repack32(long ncell, PartPtrs32 f
     if (to.id) to.id[dst] = from.id[e];
 }
 
-__global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(QueuePtrs32 q, int oldcap,
-                                                             PartPtrs32 to) {
+/* bulk cell of halo-box cell h of a tile (periodic wrap) */
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ long halo_cell(const Grid &g, const TileGeom &t, long tile, int h) {
+    constexpr int BY = TY + 2, BZ = TZ + 2;
+    const int tz = (int)(tile % t.ntz), ty = (int)((tile / t.ntz) % t.nty),
+              tx = (int)(tile / ((long)t.ntz * t.nty));
+    int gx = tx * TX - 1 + h / (BY * BZ), gy = ty * TY - 1 + (h / BZ) % BY,
+        gz = tz * TZ - 1 + h % BZ;
+    if (gx < 0) gx += g.Nx; else if (gx >= g.Nx) gx -= g.Nx;
+    if (gy < 0) gy += g.Ny; else if (gy >= g.Ny) gy -= g.Ny;
+    if (gz < 0) gz += g.Nz; else if (gz >= g.Nz) gz -= g.Nz;
+    return ((long)gx * g.Ny + gy) * g.Nz + gz;
+}
+
+/* one queue record into its bucket slot */
+__device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, float4 A, float4 B,
+                                        unsigned key) {
+    const long to = cell * p.cap + slot;
+    p.p[0][to] = A.x;
+    p.p[1][to] = A.y;
+    p.p[2][to] = A.z;
+    p.p[3][to] = A.w;
+    p.p[4][to] = B.x;
+    p.p[5][to] = B.y;
+    p.sp[to] = (unsigned char)(key & 255u);
+    if (p.id) p.id[to] = __float_as_int(B.w);
+}
+
+/* Arrival of the FP32 departures (GPU_ArrangeFlights / Cell::Insert,
+ * kernels.cu:316-370), one workgroup per source tile: one claim per
+ * destination cell on its bucket count (the push counted the tile's leavers
+ * per halo cell and ranked them), then each record goes to slot base + rank.
+ * A slot past the capacity is kept in q.slot for the regrow path and raises
+ * PIC_F_BUCKET_OVERFLOW. */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, PartPtrs32 p,
+                                                       QueuePtrs32 q, DevStatus *st) {
+    constexpr int NH = (TX + 2) * (TY + 2) * (TZ + 2);
+    __shared__ int hbase[NH];
     const long tile = blockIdx.x;
-    const int n = q.count[tile];
+    const int tid = threadIdx.x;
+    int n = q.count[tile];
+    n = n < q.qcap ? n : q.qcap;
+    const long qb = tile * (long)q.qcap;
+    /* the first record of each thread is loaded before the claims */
+    float4 A0 = make_float4(0.f, 0.f, 0.f, 0.f), B0 = A0;
+    if (tid < n) {
+        A0 = q.e[2 * (qb + tid)];
+        B0 = q.e[2 * (qb + tid) + 1];
+    }
+    for (int h = tid; h < NH; h += PIC_BLOCK) {
+        const int c = q.hcnt[tile * NH + h];
+        if (c > 0) hbase[h] = atomicAdd(&p.count[halo_cell<TX, TY, TZ>(g, t, tile, h)], c);
+    }
+    __syncthreads();
+    bool over = false;
+    for (int j = tid; j < n; j += PIC_BLOCK) {
+        float4 A = A0, B = B0;
+        if (j != tid) {
+            A = q.e[2 * (qb + j)];
+            B = q.e[2 * (qb + j) + 1];
+        }
+        const unsigned key = __float_as_uint(B.z);
+        const int h = (int)((key >> 8) & 255u);
+        const int slot = hbase[h] + (int)(key >> 16);
+        q.slot[qb + j] = slot;
+        if (slot >= p.cap) {
+            over = true;
+            continue;
+        }
+        place32(p, halo_cell<TX, TY, TZ>(g, t, tile, h), slot, A, B, key);
+    }
+    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+}
+
+/* regrow: the arrivals whose claimed slot lay past the old capacity */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(Grid g, TileGeom t, QueuePtrs32 q,
+                                                             int oldcap, PartPtrs32 to) {
+    const long tile = blockIdx.x;
+    int n = q.count[tile];
+    n = n < q.qcap ? n : q.qcap;
     const long qb = tile * (long)q.qcap;
     for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
-        const long e = qb + j;
-        const int slot = q.slot[e];
+        const int slot = q.slot[qb + j];
         if (slot < oldcap || slot >= to.cap) continue;
-        const long dst = (long)q.dest[e] * to.cap + slot;
-        for (int k = 0; k < 6; k++) to.p[k][dst] = q.q[k][e];
-        to.sp[dst] = q.sp[e];
-        if (to.id) to.id[dst] = q.id[e];
+        const float4 A = q.e[2 * (qb + j)], B = q.e[2 * (qb + j) + 1];
+        const unsigned key = __float_as_uint(B.z);
+        place32(to, halo_cell<TX, TY, TZ>(g, t, tile, (int)((key >> 8) & 255u)), slot, A, B,
+                key);
     }
 }
 
@@ -566,8 +637,8 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
 
 void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
                        DevStatus *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_arrive_tile<TXC, TYC, TZC, PartPtrs32, QueuePtrs32>),
-                       dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, g, t, p, q, st);
+    hipLaunchKernelGGL((k_arrive32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0,
+                       s, g, t, p, q, st);
 }
 
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
@@ -581,10 +652,10 @@ void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s) 
                        ncell, from, to);
 }
 
-void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtrs32 to,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_fix_arrivals32, dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0, s, q,
-                       oldcap, to);
+void launch_fix_arrivals32(const Grid &g, const TileGeom &t, QueuePtrs32 q, int oldcap,
+                           PartPtrs32 to, hipStream_t s) {
+    hipLaunchKernelGGL((k_fix_arrivals32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                       dim3(PIC_BLOCK), 0, s, g, t, q, oldcap, to);
 }
 
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,

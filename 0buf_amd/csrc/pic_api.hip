@@ -88,6 +88,27 @@ int alloc_queue(Q &q, const TileGeom &t, int qcap, bool with_id) {
     return PIC_SUCCESS;
 }
 
+void free_queue(QueuePtrs32 &q) {
+    if (q.e) (void)hipFree(q.e);
+    if (q.slot) (void)hipFree(q.slot);
+    if (q.count) (void)hipFree(q.count);
+    if (q.hcnt) (void)hipFree(q.hcnt);
+    q = QueuePtrs32{};
+}
+
+int alloc_queue(QueuePtrs32 &q, const TileGeom &t, int qcap, bool) {
+    q = QueuePtrs32{};
+    q.qcap = qcap;
+    const size_t n = (size_t)t.ntiles * qcap;
+    const size_t nh = (size_t)(t.TX + 2) * (t.TY + 2) * (t.TZ + 2);
+    HIPCHK(hipMalloc(&q.e, 2 * n * sizeof(float4)));
+    HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
+    HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
+    HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
+    HIPCHK(hipMalloc(&q.hcnt, t.ntiles * nh * sizeof(int)));
+    return PIC_SUCCESS;
+}
+
 int queue_capacity(const TileGeom &t, int cap) {
     long c = (long)t.TX * t.TY * t.TZ * cap / 2;
     c = (c + 63) / 64 * 64;
@@ -110,7 +131,9 @@ bool push_span_ok(const pic_ctx *c, int cap) {
     const TileGeom &t = c->t;
     const int elem = c->fp32 ? 4 : 8;
     const long span = ((long)(t.TX - 1) * c->g.Ny * c->g.Nz + (long)(t.TY - 1) * c->g.Nz + t.TZ) * cap;
-    return span * elem < (1L << 32) && (long)cap * elem <= 256L * 128;
+    /* the FP32 queue records rank a leaver among its tile's leavers in 16 bits */
+    const bool ranks_ok = !c->fp32 || queue_capacity(t, cap) < (1 << 16);
+    return span * elem < (1L << 32) && (long)cap * elem <= 256L * 128 && ranks_ok;
 }
 
 int auto_capacity(int maxc) {
@@ -235,7 +258,7 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     np.count = parts.count;
     if constexpr (std::is_same<P, PartPtrs32>::value) {
         launch_repack32(NC, parts, np, c->stream);
-        launch_fix_arrivals32(c->t, queue, oldcap, np, c->stream);
+        launch_fix_arrivals32(c->g, c->t, queue, oldcap, np, c->stream);
     } else {
         launch_repack(NC, parts, np, c->stream);
         launch_fix_arrivals(c->t, queue, oldcap, np, c->stream);

@@ -82,15 +82,18 @@ struct PartPtrs32 {
     int *id;                 /* differentiable form only */
 };
 
+/* FP32 departure queue: one 32-byte record per leaver, two float4
+ * (ox, oy, oz, pu) and (pv, pw, key, id), with key = species | h << 8 |
+ * rank << 16 (h = the destination's index in the source tile's
+ * (TX+2)(TY+2)(TZ+2) halo box, rank = the leaver's rank among the tile's
+ * leavers bound for h) and id as int bits (differentiable form); hcnt holds
+ * per tile the leavers bound for each halo cell */
 struct QueuePtrs32 {
-    float *q[6];
-    unsigned char *sp;
-    int *dest;
-    int *slot;
-    int *count;
-    int qcap;
-    int *id;                 /* differentiable form only */
-    unsigned char *halo;     /* destination in the source tile's halo box */
+    float4 *e;
+    int *slot;               /* slot claimed by the arrival kernel */
+    int *count;              /* per tile */
+    int *hcnt;               /* per tile and halo cell */
+    int qcap;                /* entries per tile */
 };
 
 struct QueuePtrs {
@@ -284,8 +287,8 @@ void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
 void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);
-void launch_fix_arrivals32(const TileGeom &t, QueuePtrs32 q, int oldcap, PartPtrs32 to,
-                           hipStream_t s);
+void launch_fix_arrivals32(const Grid &g, const TileGeom &t, QueuePtrs32 q, int oldcap,
+                           PartPtrs32 to, hipStream_t s);
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
                                double *dst[6], unsigned char *dsp, int *dcell, int *did,
                                hipStream_t s);

@@ -120,3 +120,34 @@ def test_fp32_matches_fp64_gpu_at_larger_grid(picmod):
     rel = np.abs(e[picmod.FP32] - e[picmod.FP64]) / np.abs(e[picmod.FP64])
     print(f"64x32x32 FP32 vs FP64 energy history: max rel {rel.max():.3e}")
     assert rel.max() <= 1e-4
+
+
+def test_fp32_bucket_regrow_bit_identical(picmod, cfg0):
+    """Buckets sized to the initial maximum overflow on the first arrivals;
+    the library regrows them and re-places the overflowed arrivals from the
+    FP32 departure records.  With order-free integer current the run equals
+    one with ample buckets bit for bit (energies, fields, counts, particles)."""
+    from helpers import sorted_particles
+    params, parts = cfg0
+    runs = {}
+    for cap in (0, 48):
+        p = picmod.make_params((params.nx, params.ny, params.nz),
+                               (params.lx, params.ly, params.lz), params.tau,
+                               precision=picmod.FP32, bucket_capacity=cap)
+        g = picmod.Pic(p)
+        load_workload(g, parts)
+        e = np.array([g.step(nt) for nt in range(1, 21)])
+        st = g.status()
+        assert st["flags"] == 0
+        runs[cap] = (e, [g.get_field(i) for i in range(9)], g.cell_counts(),
+                     sorted_particles(g.particles()), st)
+        g.close()
+    assert runs[48][4]["bucket_regrows"] >= 1
+    assert runs[48][4]["bucket_capacity"] > 48
+    assert runs[0][4]["bucket_regrows"] == 0
+    assert np.array_equal(runs[0][0], runs[48][0])
+    for a, b in zip(runs[0][1], runs[48][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(runs[0][2], runs[48][2])
+    for k in runs[0][3]:
+        assert np.array_equal(runs[0][3][k], runs[48][3][k]), k
