@@ -311,14 +311,18 @@ def main():
 
     P, ncell = r64["P"], r64["ncell"]
 
-    def measured_traffic(name):
+    def measured_traffic(name, alg):
         """HBM bytes per launch of a push kernel from the committed counter
-        summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None"""
+        summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None when that
+        summary was measured on another workload (other algorithmic bytes)"""
         tf = os.path.join(ROOT, "profiles", name)
         try:
-            return json.load(open(tf)).get("bytes_per_launch")
+            rec = json.load(open(tf))
         except (OSError, ValueError):
             return None
+        if rec.get("algorithmic_bytes") != alg:
+            return None
+        return rec.get("bytes_per_launch")
 
     def form_result(res, width, kernel, traffic_file):
         push_ms, alg, achieved = push_roofline(res, width)
@@ -329,7 +333,7 @@ def main():
                 "ms_per_step": res["dt"] * 1e3 / args.steps,
                 "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                             "traffic": measured_traffic(traffic_file), "kernel": kernel,
+                             "traffic": measured_traffic(traffic_file, alg), "kernel": kernel,
                              "algorithmic_bytes_per_launch": alg, "mean_launch_ms": push_ms},
                 "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
                 "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,

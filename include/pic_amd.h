@@ -96,7 +96,14 @@ typedef struct pic_params {
     int precision;           /* enum pic_precision */
     int departure_cap;       /* PARTICLES_FLYING_ONE_DIRECTION (cell.h:25) = 50;
                                 <= 0 disables the check */
-    int bucket_capacity;     /* particle slots per cell; 0 = automatic */
+    int bucket_capacity;     /* particle slots per cell; 0 = automatic.  Loading
+                                or a regrow fails with PIC_ERR_INVALID when a
+                                capacity leaves the push kernels' addressing:
+                                a tile's buckets (TX-1 planes + TY-1 rows +
+                                TZ cells) over 4 GiB, a bucket over 256 lines
+                                of 128 bytes (FP64: 4096 slots), or in the
+                                FP32 form 2048 slots or more (its departure
+                                records rank leavers in 16 bits) */
     int device;              /* HIP device ordinal */
     int differentiable;      /* 1: differentiable form: particles carry their
                                 load-order id, J is accumulated in int64 fixed
