@@ -98,6 +98,42 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, in
     add64(&Jz[n + BY * BZ + BZ], fix64(qz * (mx * my + sxy)));
 }
 
+/* groups of a cell of the live FP32 buckets, in slot order: bucket lines of
+ * [0, bc), inbox records [bc, bc + nin) by 4 (one 128-byte line of
+ * records), bucket slots [bc + PIC_INBOX, cnt) by line (arrivals past a full
+ * inbox) */
+__device__ __forceinline__ int cell_groups32(int cnt, int bc) {
+    const int nin = min(PIC_INBOX, cnt - bc), t0 = bc + PIC_INBOX;
+    return (bc + GS32 - 1) / GS32 + (nin + 3) / 4 + (cnt > t0 ? (cnt - 1) / GS32 - t0 / GS32 + 1 : 0);
+}
+
+template <int C>
+__device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, const int *sG,
+                                               unsigned *sGrp, int tid) {
+    for (int w = tid; w < 4 * C; w += PIC_BLOCK) {
+        const int c = w >> 2, part = w & 3;
+        const int cnt = sCnt[c], bc = sBc[c];
+        const int nsoa = (bc + GS32 - 1) / GS32, nin = min(PIC_INBOX, cnt - bc);
+        const int ninb = (nin + 3) / 4, t0 = bc + PIC_INBOX, ng = sG[c + 1] - sG[c];
+        for (int k = part; k < ng; k += 4) {
+            int kind = 0, start, size;
+            if (k < nsoa) {
+                start = GS32 * k;
+                size = min(GS32, bc - start);
+            } else if (k < nsoa + ninb) {
+                kind = 1;
+                start = bc + 4 * (k - nsoa);
+                size = min(4, bc + nin - start);
+            } else {
+                const int line = t0 / GS32 + (k - nsoa - ninb);
+                start = max(t0, GS32 * line);
+                size = min(cnt, GS32 * (line + 1)) - start;
+            }
+            sGrp[sG[c] + k] = group_entry(c, kind, start, size);
+        }
+    }
+}
+
 /* 8 waves per SIMD (at most 64 VGPRs, no spills) */
 template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
@@ -109,6 +145,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
     __shared__ int sBase[C];     /* first slot of each cell, relative to the tile's first cell */
+    __shared__ int sBc[C];       /* slots of each cell held by the bucket arrays */
+    __shared__ int sIB[C];       /* first inbox record of each cell, relative to the tile's */
     __shared__ int sG[C + 1];    /* first line group of each cell */
     __shared__ int sHalo[NB];    /* leavers bound for each cell of the halo box */
     __shared__ int sQ;
@@ -138,16 +176,19 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     for (int c = tid; c < C; c += PIC_BLOCK) {
         int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
         int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        int cnt = 0;
+        int cnt = 0, bc = 0;
         if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
             const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
             cnt = a.p.count[cell];
             cnt = cnt < cap ? cnt : cap;
+            bc = max(0, min(a.p.bcount[cell], cnt));
             if (snap) a.snap.count[cell] = cnt;
         }
         sCnt[c] = cnt;
+        sBc[c] = bc;
         sLeave[c] = 0;
         sBase[c] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
+        sIB[c] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
     const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
@@ -172,13 +213,13 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
             sOff[c] = s;
             sG[c] = ng;
             s += sCnt[c];
-            ng += (sCnt[c] + GS32 - 1) / GS32;
+            ng += cell_groups32(sCnt[c], sBc[c]);
         }
         sOff[C] = s;
         sG[C] = ng;
     }
     __syncthreads();
-    build_groups<TX, TY, TZ, GS32>(sCnt, sG, sGrp, tid, PIC_BLOCK);
+    build_groups32<C>(sCnt, sBc, sG, sGrp, tid);
     __syncthreads();
 
     /* per-tile bases of the particle arrays: slot rel of the tile is
@@ -189,6 +230,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
                  *const P4 = a.p.p[4] + pbase, *const P5 = a.p.p[5] + pbase;
     unsigned char *const SP = a.p.sp + pbase;
     int *const ID = DIFF ? a.p.id + pbase : nullptr;
+    const float4 *const IB = a.p.inbox + 2 * ((((long)I0 * g.Ny + L0) * g.Nz + K0) * PIC_INBOX);
 
     /* the wavefront's next chunk is composed and its particles loaded one
      * chunk ahead, so the memory latency overlaps this chunk's arithmetic */
@@ -200,14 +242,22 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
     auto fetch = [&](int g0, unsigned long long starts) {
         const unsigned long long m = starts & (~0ull >> (63 - lane));
         const unsigned e = sGrp[g0 + __popcll(m) - 1];
-        const int cf = (int)(e & 255u);
+        const int cf = group_cell(e);
         nfc = cf;
-        nslot = (int)((e >> 8) & 255u) * GS32 + (lane - (63 - __clzll(m)));
-        const unsigned nrel = (unsigned)(sBase[cf] + nslot);
-        qox = slot(P0, nrel); qoy = slot(P1, nrel); qoz = slot(P2, nrel);
-        qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
-        qs = slot(SP, nrel);
-        if (DIFF) qid = slot(ID, nrel);
+        nslot = group_start(e) + (lane - (63 - __clzll(m)));
+        if (group_kind(e) == 0) {
+            const unsigned nrel = (unsigned)(sBase[cf] + nslot);
+            qox = slot(P0, nrel); qoy = slot(P1, nrel); qoz = slot(P2, nrel);
+            qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
+            qs = slot(SP, nrel);
+            if (DIFF) qid = slot(ID, nrel);
+        } else {
+            const unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
+            const float4 A = slot(IB, ir), B = slot(IB, ir + 1u);
+            qox = A.x; qoy = A.y; qoz = A.z; qpu = A.w; qpv = B.x; qpw = B.y;
+            qs = (unsigned char)(__float_as_uint(B.z) & 255u);
+            if (DIFF) qid = __float_as_int(B.w);
+        }
     };
     const int ghi = __builtin_amdgcn_readfirstlane(
         sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
@@ -307,7 +357,9 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
             slot(P3, dst) = pu;
             slot(P4, dst) = pv;
             slot(P5, dst) = pw;
-            if (rank != j) {
+            /* species and id only move with the particle (or come from an
+             * inbox record) */
+            if (rank != j || (j >= sBc[c] && j < sBc[c] + PIC_INBOX)) {
                 slot(SP, dst) = s;
                 if (DIFF) slot(ID, dst) = pid;
             }
@@ -323,6 +375,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 8) void k_push_deposit32(PushArgs32 a) {
         const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
         const int m = sLeave[c];
         a.p.count[cell] = sCnt[c] - m;
+        a.p.bcount[cell] = sCnt[c] - m;
         if (m > maxout) maxout = m;
     }
 
@@ -366,7 +419,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load32(const long *off, l
     const long c = e / p.cap;
     const int j = (int)(e % p.cap);
     const long n = off[c + 1] - off[c];
-    if (j == 0) p.count[c] = (int)n;
+    if (j == 0) {
+        p.count[c] = (int)n;
+        if (p.bcount) p.bcount[c] = (int)n;
+    }
     if (j >= n) return;
     const long src = off[c] + j;
     p.p[0][e] = s0[src];
@@ -421,17 +477,36 @@ __device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, floa
     if (p.id) p.id[to] = __float_as_int(B.w);
 }
 
+/* inbox records into the bucket arrays: PIC_INBOX lanes per cell, all in
+ * one wavefront (they read bcount before lane 0 rewrites it) */
+static_assert(64 % PIC_INBOX == 0, "a cell's settle lanes share a wavefront");
+__global__ __launch_bounds__(PIC_BLOCK) void k_settle32(long ncell, PartPtrs32 p) {
+    const long cell = ((long)blockIdx.x * PIC_BLOCK + threadIdx.x) / PIC_INBOX;
+    const int r = (int)(threadIdx.x % PIC_INBOX);
+    if (cell >= ncell) return;
+    const int cnt = min(p.count[cell], p.cap), bc = p.bcount[cell];
+    if (r < min(PIC_INBOX, cnt - bc)) {
+        const float4 A = p.inbox[2 * (cell * PIC_INBOX + r)];
+        const float4 B = p.inbox[2 * (cell * PIC_INBOX + r) + 1];
+        place32(p, cell, bc + r, A, B, __float_as_uint(B.z));
+    }
+    if (r == 0) p.bcount[cell] = cnt;
+}
+
 /* Arrival of the FP32 departures (GPU_ArrangeFlights / Cell::Insert,
  * kernels.cu:316-370), one workgroup per source tile: one claim per
- * destination cell on its bucket count (the push counted the tile's leavers
- * per halo cell and ranked them), then each record goes to slot base + rank.
- * A slot past the capacity is kept in q.slot for the regrow path and raises
- * PIC_F_BUCKET_OVERFLOW. */
+ * destination cell on its count (the push counted the tile's leavers per
+ * halo cell and ranked them), then each record takes slot s = base + rank:
+ * inbox record s - bcount while that is below PIC_INBOX (one 32-byte
+ * write), else bucket slot s.  A slot past the capacity is kept in q.slot
+ * for the regrow path and raises PIC_F_BUCKET_OVERFLOW. */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, PartPtrs32 p,
                                                        QueuePtrs32 q, DevStatus *st) {
     constexpr int NH = (TX + 2) * (TY + 2) * (TZ + 2);
     __shared__ int hbase[NH];
+    __shared__ int hbc[NH];
+    __shared__ long hcell[NH];
     const long tile = blockIdx.x;
     const int tid = threadIdx.x;
     int n = q.count[tile];
@@ -445,7 +520,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
     }
     for (int h = tid; h < NH; h += PIC_BLOCK) {
         const int c = q.hcnt[tile * NH + h];
-        if (c > 0) hbase[h] = atomicAdd(&p.count[halo_cell<TX, TY, TZ>(g, t, tile, h)], c);
+        if (c > 0) {
+            const long cell = halo_cell<TX, TY, TZ>(g, t, tile, h);
+            hcell[h] = cell;
+            hbc[h] = p.bcount[cell];
+            hbase[h] = atomicAdd(&p.count[cell], c);
+        }
     }
     __syncthreads();
     bool over = false;
@@ -463,7 +543,14 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
             over = true;
             continue;
         }
-        place32(p, halo_cell<TX, TY, TZ>(g, t, tile, h), slot, A, B, key);
+        const long cell = hcell[h];
+        const int r = slot - hbc[h];
+        if (r < PIC_INBOX) {
+            p.inbox[2 * (cell * PIC_INBOX + r)] = A;
+            p.inbox[2 * (cell * PIC_INBOX + r) + 1] = B;
+        } else {
+            place32(p, cell, slot, A, B, key);
+        }
     }
     if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
 }
@@ -625,14 +712,20 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.snap_on = diff && diff->p[0] != nullptr;
     a.snap_off = snap_off;
     a.snap_room = snap_room;
-    /* dynamic LDS: one 4-byte entry per line group */
-    const size_t lds = (size_t)4 * TXC * TYC * TZC * ((p.cap + GS32 - 1) / GS32);
+    /* dynamic LDS: one 4-byte entry per group (bucket lines before and after
+     * the inbox, inbox lines) */
+    const size_t lds = (size_t)4 * TXC * TYC * TZC * (2 * ((p.cap + GS32 - 1) / GS32) + 6);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), lds, s, a);
     else
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), lds, s, a);
+}
+
+void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s) {
+    hipLaunchKernelGGL(k_settle32, dim3(blocks_for(ncell * PIC_INBOX)), dim3(PIC_BLOCK), 0, s,
+                       ncell, p);
 }
 
 void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,

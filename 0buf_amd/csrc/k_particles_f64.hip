@@ -225,8 +225,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         if (valid) {
             const unsigned long long msk = starts & (~0ull >> (63 - lane));
             const unsigned ge = sGrp[g0 + __popcll(msk) - 1];
-            c = (int)(ge & 255u);
-            j = (int)((ge >> 8) & 255u) * GS64 + (lane - (63 - __clzll(msk)));
+            c = group_cell(ge);
+            j = group_start(ge) + (lane - (63 - __clzll(msk)));
             const unsigned rel = (unsigned)(sBase[c] + j);
             Geo G;
             G.ca = c / (TY * TZ);

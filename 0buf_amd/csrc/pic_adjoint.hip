@@ -147,6 +147,8 @@ int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     const long NC = c->NC;
     const size_t es = elem(c);
     std::vector<int> cnt(NC);
+    int rc0 = settle(c);
+    if (rc0) return rc0;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(cnt.data(), live_count(c), NC * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<long> off(NC + 1, 0);
@@ -178,8 +180,9 @@ int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
     if (c->fp32) {
         const float *v[6];
         for (int j = 0; j < 6; j++) v[j] = (const float *)src.v[j];
-        launch_scatter_load32(src.off, c->NC, v, src.sp, c->p32, c->stream);
+        launch_scatter_load32(src.off, c->NC, v, src.sp, c->p32, c->stream);   /* bcount = count */
         launch_scatter_ids32(src.off, c->NC, src.id, c->p32, c->stream);
+        c->unsettled = false;
     } else {
         const double *v[6];
         for (int j = 0; j < 6; j++) v[j] = (const double *)src.v[j];
@@ -287,6 +290,7 @@ int pic_get_particle_ids(pic_ctx *c, int *id, long cap, long *n_out) {
         return PIC_SUCCESS;
     }
     HIPCHK(hipSetDevice(c->dev));
+    if ((rc = settle(c))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     const long NC = c->NC;
     std::vector<int> cnt(NC);

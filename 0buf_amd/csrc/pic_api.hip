@@ -36,7 +36,19 @@ void free_parts(P &p) {
     if (p.sp) (void)hipFree(p.sp);
     if (p.count) (void)hipFree(p.count);
     if (p.id) (void)hipFree(p.id);
+    if constexpr (std::is_same<P, PartPtrs32>::value) {
+        if (p.bcount) (void)hipFree(p.bcount);
+        if (p.inbox) (void)hipFree(p.inbox);
+    }
     p = P{};
+}
+
+/* inbox of the live FP32 buckets; bcount is set equal to count by whoever
+ * fills the buckets (scatter load, initializer, regrow) */
+int alloc_inbox(PartPtrs32 &p, long ncell) {
+    HIPCHK(hipMalloc(&p.bcount, ncell * sizeof(int)));
+    HIPCHK(hipMalloc(&p.inbox, (size_t)ncell * PIC_INBOX * 2 * sizeof(float4)));
+    return PIC_SUCCESS;
 }
 
 /* buckets of ncell x cap slots; ids only for the differentiable form */
@@ -123,17 +135,20 @@ int queue_capacity(const TileGeom &t, int cap) {
 #ifndef PIC_CAP_ALIGN
 #define PIC_CAP_ALIGN 32
 #endif
-/* the push kernels address a tile's particle slots by 32-bit byte offsets
- * from the bucket of its first cell: the tile's bucket span (from that cell
- * to the end of its last cell's bucket) must stay below 2^32 bytes; their
- * line groups (128 bytes of one array) number at most 256 per bucket */
+/* the push kernels address a tile's particle slots (and FP32 inbox records)
+ * by 32-bit byte offsets from the bucket of its first cell: the tile's span
+ * (from that cell to the end of its last cell's bucket) must stay below
+ * 2^32 bytes; a bucket holds at most 256 lines of 128 bytes (the group
+ * entries keep a slot in 15 bits) */
 bool push_span_ok(const pic_ctx *c, int cap) {
     const TileGeom &t = c->t;
     const int elem = c->fp32 ? 4 : 8;
-    const long span = ((long)(t.TX - 1) * c->g.Ny * c->g.Nz + (long)(t.TY - 1) * c->g.Nz + t.TZ) * cap;
+    const long cells = (long)(t.TX - 1) * c->g.Ny * c->g.Nz + (long)(t.TY - 1) * c->g.Nz + t.TZ;
+    const long span = cells * cap * elem;
+    const long inbox = c->fp32 ? cells * PIC_INBOX * 2 * (long)sizeof(float4) : 0;
     /* the FP32 queue records rank a leaver among its tile's leavers in 16 bits */
     const bool ranks_ok = !c->fp32 || queue_capacity(t, cap) < (1 << 16);
-    return span * elem < (1L << 32) && (long)cap * elem <= 256L * 128 && ranks_ok;
+    return span < (1L << 32) && inbox < (1L << 32) && (long)cap * elem <= 256L * 128 && ranks_ok;
 }
 
 int auto_capacity(int maxc) {
@@ -179,6 +194,9 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
     }
     int rc = alloc_parts(parts, NC, cap, true, c->diff);
     if (rc) return rc;
+    if constexpr (std::is_same<P, PartPtrs32>::value) {
+        if ((rc = alloc_inbox(parts, NC))) return rc;
+    }
     rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
     long *d_off = nullptr;
@@ -257,8 +275,15 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     if (rc) return rc;
     np.count = parts.count;
     if constexpr (std::is_same<P, PartPtrs32>::value) {
+        /* the inbox records (all below the old capacity) into the old
+         * bucket arrays, then the bucket arrays into the grown ones */
+        launch_settle32(NC, parts, c->stream);
+        if ((rc = alloc_inbox(np, NC))) return rc;
         launch_repack32(NC, parts, np, c->stream);
         launch_fix_arrivals32(c->g, c->t, queue, oldcap, np, c->stream);
+        HIPCHK(hipMemcpyAsync(np.bcount, np.count, NC * sizeof(int), hipMemcpyDeviceToDevice,
+                              c->stream));
+        c->unsettled = false;
     } else {
         launch_repack(NC, parts, np, c->stream);
         launch_fix_arrivals(c->t, queue, oldcap, np, c->stream);
@@ -285,6 +310,14 @@ int regrow(pic_ctx *c) {
 
 int *count_ptr(pic_ctx *c) { return c->fp32 ? c->p32.count : c->p.count; }
 int bucket_cap(pic_ctx *c) { return c->fp32 ? c->p32.cap : c->p.cap; }
+
+int settle(pic_ctx *c) {
+    if (!c->fp32 || !c->unsettled || !c->p32.inbox) return PIC_SUCCESS;
+    launch_settle32(c->NC, c->p32, c->stream);
+    HIPCHK(hipGetLastError());
+    c->unsettled = false;
+    return PIC_SUCCESS;
+}
 int *qcount_ptr(pic_ctx *c) { return c->fp32 ? c->q32.count : c->q.count; }
 
 /* stage launchers dispatching on the form */
@@ -312,7 +345,10 @@ void enq_push(pic_ctx *c) {
     }
 }
 void enq_arrive(pic_ctx *c) {
-    if (c->fp32) launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, c->stream);
+    if (c->fp32) {
+        launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, c->stream);
+        c->unsettled = true;
+    }
     else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, c->stream);
 }
 void enq_fold(pic_ctx *c) {
@@ -667,6 +703,7 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, c->diff)
                      : alloc_parts(c->p, NC, cap, true, c->diff);
     if (rc) return rc;
+    if (c->fp32 && (rc = alloc_inbox(c->p32, NC))) return rc;
     rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), c->diff)
                  : alloc_queue(c->q, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
@@ -709,6 +746,8 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
         total += plane * g.Nx;
     }
     HIPCHK(hipMemcpy(c->d_slot_sort, c->slot_sort, sizeof(c->slot_sort), hipMemcpyHostToDevice));
+    if (c->fp32)
+        HIPCHK(hipMemcpy(c->p32.bcount, c->p32.count, NC * sizeof(int), hipMemcpyDeviceToDevice));
     unsigned long long bad = 0;
     int fail = 0;
     HIPCHK(hipMemcpy(&bad, d_bad, sizeof(bad), hipMemcpyDeviceToHost));
@@ -925,6 +964,7 @@ int pic_get_particles(pic_ctx *c, double *x, double *y, double *z, double *px, d
                       long *n_out) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = settle(c))) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     const long NC = c->NC;
     std::vector<int> cnt(NC);
@@ -972,6 +1012,7 @@ int pic_moments(pic_ctx *c, double *out27) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!out27) return PIC_ERR_INVALID;
+    if ((rc = settle(c))) return rc;
     if (c->fp32)
         launch_moments32(c->g, c->NC, c->p32, c->sp, c->d_slot_sort, c->d_part, c->npart,
                          c->d_red, c->stream);
@@ -989,6 +1030,7 @@ int pic_cell_moments(pic_ctx *c, int sort, int op, double *out) {
     int rc = check_ready(c);
     if (rc) return rc;
     if (!out || sort < 0 || sort > 2 || op < PIC_DENSITY || op > PIC_CURRENT) return PIC_ERR_INVALID;
+    if ((rc = settle(c))) return rc;
     const long n = (op >= PIC_VELOCITY ? 3 : 1) * c->NC;
     double *d = nullptr;
     HIPCHK(hipMalloc(&d, n * sizeof(double)));
@@ -1007,6 +1049,7 @@ int pic_cell_moments(pic_ctx *c, int sort, int op, double *out) {
 int pic_gauss_residual(pic_ctx *c, double *res, double *max_drift, int set_reference) {
     int rc = check_ready(c);
     if (rc) return rc;
+    if ((rc = settle(c))) return rc;
     const Grid &g = c->g;
     if (!c->d_rho) HIPCHK(hipMalloc(&c->d_rho, g.G * sizeof(long long)));
     if (!c->d_gres) HIPCHK(hipMalloc(&c->d_gres, c->NC * sizeof(double)));

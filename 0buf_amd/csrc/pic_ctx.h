@@ -64,6 +64,7 @@ struct pic_ctx {
     double *h_red = nullptr;
     Staged staged;
     bool committed = false;
+    bool unsettled = false;   /* FP32 form: inbox records not yet in the bucket arrays */
     int regrows = 0;
     pic_step_status last{};
     /* stage timing */
@@ -80,6 +81,9 @@ int commit(pic_ctx *c);
 int check_ready(pic_ctx *c);
 int enq_step(pic_ctx *c);
 int finish_step(pic_ctx *c);
+/* FP32 form: move the inbox records into the bucket arrays before anything
+ * other than the push reads the buckets (no-op otherwise) */
+int settle(pic_ctx *c);
 template <class P>
 int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id);
 template <class P>

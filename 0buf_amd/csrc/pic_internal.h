@@ -73,13 +73,27 @@ struct PartPtrs {
 };
 
 /* particle storage of the FP32 form: x, y, z are offsets in [0,1) inside the
- * owning cell (cell units), px, py, pz the momenta */
+ * owning cell (cell units), px, py, pz the momenta.
+ *
+ * The live buckets also have an inbox per cell: PIC_INBOX records of 32
+ * bytes (the departure-queue record format) where the arrival kernel puts
+ * the particles that joined the cell this step, one 32-byte write each
+ * instead of seven scattered ones.  count is the cell's total; slots
+ * [0, bcount) are in the bucket arrays, slots bcount + r for r < PIC_INBOX
+ * are inbox record r, and slots from bcount + PIC_INBOX on (arrivals past a
+ * full inbox) are in the bucket arrays again.  The next push reads the
+ * cell in slot order and compacts it into the bucket arrays; the settle
+ * kernel does the same for every other reader.  Checkpoints and staging
+ * copies have no inbox (bcount, inbox null). */
+constexpr int PIC_INBOX = 16;
 struct PartPtrs32 {
     float *p[6];
     unsigned char *sp;
     int *count;
     int cap;
     int *id;                 /* differentiable form only */
+    int *bcount;             /* live buckets: slots held by the bucket arrays */
+    float4 *inbox;           /* live buckets: PIC_INBOX records (2 float4) per cell */
 };
 
 /* FP32 departure queue: one 32-byte record per leaver, two float4
@@ -165,8 +179,17 @@ __device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &c
  * group holds the particles of one line, 1..GS of them.  A wavefront walks
  * its groups in order and packs whole groups into chunks of at most 64
  * lanes, so no line is shared by two chunks (a line split across chunks is
- * read twice when it leaves the L2 in between).  Group entry: cell | line
- * << 8 | size << 16. */
+ * read twice when it leaves the L2 in between).  Group entry: cell (8 bits)
+ * | kind (1: FP32 inbox records, else bucket slots) | first slot (15 bits)
+ * | size (8 bits). */
+__device__ __forceinline__ unsigned group_entry(int cell, int kind, int start, int size) {
+    return (unsigned)cell | ((unsigned)kind << 8) | ((unsigned)start << 9) | ((unsigned)size << 24);
+}
+__device__ __forceinline__ int group_cell(unsigned e) { return (int)(e & 255u); }
+__device__ __forceinline__ int group_kind(unsigned e) { return (int)((e >> 8) & 1u); }
+__device__ __forceinline__ int group_start(unsigned e) { return (int)((e >> 9) & 0x7fffu); }
+__device__ __forceinline__ int group_size(unsigned e) { return (int)(e >> 24); }
+
 template <int TX, int TY, int TZ, int GS>
 __device__ __forceinline__ void build_groups(const int *sCnt, const int *sG, unsigned *sGrp,
                                              int tid, int nthreads) {
@@ -176,7 +199,7 @@ __device__ __forceinline__ void build_groups(const int *sCnt, const int *sG, uns
         const int c = w >> 2, part = w & 3, n = sCnt[c];
         for (int line = part; line < sG[c + 1] - sG[c]; line += 4) {
             const int left = n - line * GS, sz = left < GS ? left : GS;
-            sGrp[sG[c] + line] = (unsigned)c | ((unsigned)line << 8) | ((unsigned)sz << 16);
+            sGrp[sG[c] + line] = group_entry(c, 0, line * GS, sz);
         }
     }
 }
@@ -203,7 +226,7 @@ __device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int g
     int g = g0, n = 0;
     unsigned long long m = 0;
     while (g < gend) {
-        const int sz = (int)(__builtin_amdgcn_readfirstlane(sGrp[g]) >> 16);
+        const int sz = group_size(__builtin_amdgcn_readfirstlane(sGrp[g]));
         if (n + sz > 64) break;
         m |= 1ull << n;
         n += sz;
@@ -282,6 +305,8 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                              QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
                              const PartPtrs32 *diff, const long *snap_off, long snap_room,
                              hipStream_t s);
+/* inbox records into the bucket arrays (bcount = count, at most cap) */
+void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s);
 void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
                        DevStatus *st, hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
