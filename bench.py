@@ -150,28 +150,37 @@ def yee_roofline(ncell, yee_ms, width):
 def adjoint_ratio(pic):
     """Config 3 (64x16x16 cells, 32 ppc, 200 steps, checkpoints every 20):
     wall time of a gradient (forward with checkpoints + recompute + adjoint
-    sweep) against a plain forward run of the same form."""
+    sweep) against a plain forward run of the same form.  Each call builds
+    its own context (host particles loaded, checkpoint memory allocated), so
+    both are the best of three runs after a warm-up of each."""
     import torch
     ag = importlib.import_module("0buf_amd.autograd")
     bp = ag.BeamPlasma(3)
     theta = torch.tensor([0.3] + [1e-3] * 8, dtype=torch.float64, requires_grad=True)
     bp.loss_value(theta.detach())                    # warm-up
-    t0 = time.perf_counter()
-    bp.loss_value(theta.detach())
-    t_fwd = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    L = bp.loss(theta)
-    t1 = time.perf_counter()
-    L.backward()
-    t2 = time.perf_counter()
-    dev = bp.run.last.adjoint_timing()
+    t_fwd = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        bp.loss_value(theta.detach())
+        t_fwd = min(t_fwd, time.perf_counter() - t0)
+    runs = []
+    for k in range(4):                               # the first is a warm-up
+        theta.grad = None
+        t0 = time.perf_counter()
+        L = bp.loss(theta)
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        if k:
+            runs.append((t2 - t0, t1 - t0, t2 - t1, tuple(bp.run.last.adjoint_timing())))
+    grad_s, rec_s, back_s, dev = min(runs)
     bp.run.last.close()
     return {"workload": "config3: beam-plasma 64x16x16 cells, 32 ppc (ions 8, electrons 16, "
                         "beam 8), 200 steps, FP64 differentiable form, checkpoint every 20 "
                         "steps, loss = sum of field energy over steps 150..200, parameters "
                         "= beam drift + 8 displacement amplitudes",
-            "forward_s": t_fwd, "gradient_s": t2 - t0, "ratio": (t2 - t0) / t_fwd,
-            "forward_record_s": t1 - t0, "backward_s": t2 - t1,
+            "forward_s": t_fwd, "gradient_s": grad_s, "ratio": grad_s / t_fwd,
+            "forward_record_s": rec_s, "backward_s": back_s,
             "device_ms": {"forward_with_checkpoints": dev[0], "recompute": dev[1],
                           "adjoint_steps": dev[2]},
             "grad_norm": float(theta.grad.norm())}
