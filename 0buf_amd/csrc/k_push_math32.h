@@ -109,28 +109,21 @@ __device__ __forceinline__ Relay32 relay32(float ox, float oy, float oz, float o
     R.r1[0] = o1x >= 0.5f;
     R.r1[1] = o1y >= 0.5f;
     R.r1[2] = o1z >= 0.5f;
-    const int m = 4 * (rd[0] != R.r1[0]) + 2 * (rd[1] != R.r1[1]) + (rd[2] != R.r1[2]);
+    const bool cx = rd[0] != R.r1[0], cy = rd[1] != R.r1[1], cz = rd[2] != R.r1[2];
+    const int m = 4 * cx + 2 * cy + cz;
     R.m = m;
     const float dux = du[0], duy = du[1], duz = du[2];
-    float num = 0.0f, den = 1.0f;
-    switch (m) {
-    case 1: num = 0.5f - oz; den = duz; break;
-    case 2: num = 0.5f - oy; den = duy; break;
-    case 4: num = 0.5f - ox; den = dux; break;
-    case 3:
-        num = (0.5f - oz) * duz * hh[2] + (0.5f - oy) * duy * hh[1];
-        den = duz * duz * hh[2] + duy * duy * hh[1];
-        break;
-    case 5:
-        num = (0.5f - oz) * duz * hh[2] + (0.5f - ox) * dux * hh[0];
-        den = duz * duz * hh[2] + dux * dux * hh[0];
-        break;
-    case 6:
-        num = (0.5f - oy) * duy * hh[1] + (0.5f - ox) * dux * hh[0];
-        den = duy * duy * hh[1] + dux * dux * hh[0];
-        break;
-    default: break;
-    }
+    /* one formula for one and two crossed faces, without branches: the
+     * h^2-weighted projection of the crossed faces' parameters; with one
+     * crossed face d it is (1/2 - o_d) / du_d exactly in exact arithmetic.
+     * No face (m = 0) or three (m = 7, every coordinate of x2 is 1/2) leave
+     * sr unused. */
+    const float wx = cx ? hh[0] : 0.0f, wy = cy ? hh[1] : 0.0f, wz = cz ? hh[2] : 0.0f;
+    const bool used = m != 0 && m != 7;
+    const float num = used ? (0.5f - oz) * duz * wz + (0.5f - oy) * duy * wy +
+                                 (0.5f - ox) * dux * wx
+                           : 0.0f;
+    const float den = used ? duz * duz * wz + duy * duy * wy + dux * dux * wx : 1.0f;
     R.num = num;
     R.den = den;
     const float sr = num * __builtin_amdgcn_rcpf(den);
