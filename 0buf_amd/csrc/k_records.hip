@@ -63,7 +63,34 @@ __global__ __launch_bounds__(SCAN_T) void k_scan_tiles(const long *sum, long n, 
     }
 }
 
+/* six arrays of n 4-byte words in one launch (the fields of a step
+ * record), blockIdx.y = array */
+struct Six {
+    const unsigned *src[6];
+    unsigned *dst[6];
+};
+__global__ __launch_bounds__(PIC_BLOCK) void k_copy6(Six a, long n) {
+    const unsigned *src = a.src[blockIdx.y];
+    unsigned *dst = a.dst[blockIdx.y];
+    for (long i = (long)blockIdx.x * PIC_BLOCK + threadIdx.x; i < n;
+         i += (long)gridDim.x * PIC_BLOCK)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy6(void *const dst[6], const void *const src[6], size_t bytes, hipStream_t s) {
+    Six a;
+    for (int k = 0; k < 6; k++) {
+        a.src[k] = (const unsigned *)src[k];
+        a.dst[k] = (unsigned *)dst[k];
+    }
+    const long n = (long)(bytes / 4);
+    long blocks = (n + PIC_BLOCK - 1) / PIC_BLOCK;
+    if (blocks > 2048) blocks = 2048;
+    if (blocks < 1) blocks = 1;
+    hipLaunchKernelGGL(k_copy6, dim3((unsigned)blocks, 6), dim3(PIC_BLOCK), 0, s, a, n);
+}
 
 void launch_record_offsets(const Grid &g, const TileGeom &t, const int *count, int cap,
                            long *tile_sum, long *off, long room, DevStatus *st, hipStream_t s) {

@@ -198,15 +198,18 @@ int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
 }
 
 /* one forward step that records the pre-push particle state into record r
- * of the segment (through the push kernel) and E^n, H^{n+1/2} into `f` */
+ * of the segment (through the push kernel) and E^n, H^{n+1/2} into `f`.
+ * The recompute repeats steps the first forward has already run with the
+ * same counts and capacity, so it enqueues its steps without waiting: the
+ * status accumulates over the segment and is checked once at its end
+ * (check_segment). */
 int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
     FieldSet &f = s->recf[r];
     const size_t bytes = c->g.G * elem(c);
-    launch_reset_status(c->d_st, c->stream);
     enq_h_half(c);
-    for (int k = 0; k < 6; k++)
-        HIPCHK(hipMemcpyAsync(f.f[k], live_field(c, k), bytes, hipMemcpyDeviceToDevice,
-                              c->stream));
+    const void *src[6];
+    for (int k = 0; k < 6; k++) src[k] = live_field(c, k);
+    launch_copy6(f.f, src, bytes, c->stream);
     launch_record_offsets(c->g, c->t, live_count(c), live_cap(c), s->tile_sum, s->recoff[r],
                           s->room, c->d_st, c->stream);
     if (c->fp32) c->rec_snap32 = s->rec32[r];
@@ -224,12 +227,21 @@ int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
     enq_e(c);
     enq_e_periodic(c);
     HIPCHK(hipGetLastError());
-    const int cap = live_cap(c);
-    int rc = finish_step(c);
-    if (rc) return rc;
-    /* the recompute sees the forward's own counts, which the live capacity
-     * (the largest of the run) already holds */
-    if (live_cap(c) != cap) return PIC_ERR_STATE;
+    return PIC_SUCCESS;
+}
+
+/* end of a recomputed segment: any flag of its steps fails the sweep (the
+ * first forward ran the same steps without one; the recompute sees the
+ * forward's own counts, which the live capacity, the largest of the run,
+ * already holds) */
+int check_segment(pic_ctx *c) {
+    HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->h_st->flags) {
+        snprintf(g_hip_err, sizeof(g_hip_err), "recomputed segment raised status flags 0x%x",
+                 c->h_st->flags);
+        return PIC_ERR_STATE;
+    }
     return PIC_SUCCESS;
 }
 
@@ -475,8 +487,10 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
         const int t0 = seg * K, t1 = std::min(t0 + K, s->nsteps);
         HIPCHK(hipEventRecord(s->ev[0], st));
         if ((rc = restore_state(c, s->ck[seg], s->ckf[seg]))) return rc;
+        launch_reset_status(c->d_st, st);
         for (int t = t0 + 1; t <= t1; t++)
             if ((rc = recorded_step(c, s, t - t0 - 1))) return rc;
+        if ((rc = check_segment(c))) return rc;
         for (int k = 0; k < 3; k++)
             HIPCHK(hipMemcpyAsync(s->Eend[k], live_field(c, k), G * elem(c),
                                   hipMemcpyDeviceToDevice, st));

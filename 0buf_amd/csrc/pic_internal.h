@@ -282,6 +282,8 @@ void launch_jfix_to_f64(const Grid &g, FieldPtrs f, DiffArgs d, hipStream_t s);
 /* compact record offsets (k_records.hip): off[t] = particles of the tiles
  * before t (counts capped at cap), off[ntiles] the total; a total beyond
  * `room` raises PIC_F_RECORD (the push writes no record entry past room) */
+/* six device arrays of `bytes` (a multiple of 4) each, in one launch */
+void launch_copy6(void *const dst[6], const void *const src[6], size_t bytes, hipStream_t s);
 void launch_record_offsets(const Grid &g, const TileGeom &t, const int *count, int cap,
                            long *tile_sum, long *off, long room, DevStatus *st, hipStream_t s);
 void launch_arrive_f64(const Grid &g, const TileGeom &t, PartPtrs p, QueuePtrs q, DevStatus *st,
