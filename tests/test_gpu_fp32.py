@@ -13,7 +13,10 @@ the reference step) or against an FP64 invariant:
     are exact integer sums, so no result depends on atomic order);
   * per-cell counts: >= 99% of cells equal to the oracle's at step 100
     (FP32 offsets round differently near faces; measured 100% on config 0);
-  * particle moments: sums of positions and momenta within 1e-5 relative.
+  * particle moments: sums of positions and momenta within 1e-5 relative;
+  * bucket regrow and a full inbox (arrivals past the 16 inbox records of a
+    cell): bit-identical to ample buckets, and within FP32 rounding of the
+    FP64 form, respectively.
 """
 import numpy as np
 import pytest
@@ -151,3 +154,50 @@ def test_fp32_bucket_regrow_bit_identical(picmod, cfg0):
     assert np.array_equal(runs[0][2], runs[48][2])
     for k in runs[0][3]:
         assert np.array_equal(runs[0][3][k], runs[48][3][k]), k
+
+
+def test_fp32_full_inbox_matches_fp64(picmod):
+    """A cold beam crossing cell faces in bulk: every few steps the 200
+    particles of a cell all move into the next one, more than the 16-record
+    inbox of the FP32 buckets holds, so the arrival also writes into the
+    bucket arrays past the inbox and the next push reads all three parts of
+    the cell.  Against the FP64 form: cell counts equal, energy history within
+    1e-4, particles within FP32 rounding (1e-5 of a cell for positions)."""
+    from helpers import sorted_particles
+    n = (16, 8, 8)
+    rng = np.random.default_rng(7)
+    xs, ys, zs = [], [], []
+    for i in range(2, 6):                       # 4 beam cells along x at (i, 3, 4)
+        k = 200
+        xs.append((i + rng.uniform(0.85, 1.0, k)) * 0.1)
+        ys.append((3 + rng.uniform(0.2, 0.8, k)) * 0.1)
+        zs.append((4 + rng.uniform(0.2, 0.8, k)) * 0.1)
+    x, y, z = (np.concatenate(a) for a in (xs, ys, zs))
+    px = np.full(x.size, 0.75)                  # 0.6 c: 0.3 cells per step
+    zero = np.zeros(x.size)
+    runs = {}
+    for prec in (picmod.FP64, picmod.FP32):
+        p = picmod.make_params(n, tuple(0.1 * k for k in n), 0.05, precision=prec,
+                               departure_cap=0)
+        g = picmod.Pic(p)
+        g.load_particles(1, x, y, z, px, zero, zero, -1e-3, -1.0)
+        e, maxdep = [], 0
+        for nt in range(1, 31):
+            e.append(g.step(nt))
+            st = g.status()
+            assert st["flags"] == 0
+            maxdep = max(maxdep, st["max_cell_departures"])
+        runs[prec] = (np.array(e), g.cell_counts(), sorted_particles(g.particles()), maxdep)
+        g.close()
+    e64, c64, p64, _ = runs[picmod.FP64]
+    e32, c32, p32, maxdep32 = runs[picmod.FP32]
+    assert maxdep32 > 16                             # the inbox overflowed
+    assert np.array_equal(c64, c32)
+    rel = np.abs(e32 - e64) / np.abs(e64)
+    print(f"full-inbox beam: FP32 vs FP64 energy history max rel {rel.max():.3e}")
+    assert rel.max() <= 1e-4
+    assert len(p32["x"]) == len(p64["x"]) == x.size
+    for k in ("x", "y", "z"):
+        assert np.abs(p32[k] - p64[k]).max() <= 1e-5 * 0.1, k
+    for k in ("px", "py", "pz"):
+        assert np.abs(p32[k] - p64[k]).max() <= 1e-5, k
