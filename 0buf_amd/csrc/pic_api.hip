@@ -344,13 +344,14 @@ void enq_push(pic_ctx *c) {
                                 c->d_st, nullptr, c->stream);
     }
 }
-void enq_arrive(pic_ctx *c) {
+void enq_arrive(pic_ctx *c, hipStream_t st) {
     if (c->fp32) {
-        launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, c->stream);
+        launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, st);
         c->unsettled = true;
     }
-    else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, c->stream);
+    else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, st);
 }
+void enq_arrive(pic_ctx *c) { enq_arrive(c, c->stream); }
 void enq_fold(pic_ctx *c) {
     if (c->fp32) launch_fold_j32(c->g, c->f32, c->stream);
     else launch_fold_j(c->g, c->f, c->stream);
@@ -386,7 +387,10 @@ void accumulate_profile(pic_ctx *c) {
     if (!c->prof) return;
     for (int k = 0; k < PIC_NSTAGES; k++) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+        /* the arrival runs on the side stream: from the end of the push to
+         * its own end */
+        hipEvent_t end = (k == PIC_STAGE_MIGRATE && c->side) ? c->ev_join : c->ev[k + 1];
+        if (hipEventElapsedTime(&ms, c->ev[k], end) == hipSuccess) {
             c->prof_ms[k] += ms;
             c->prof_n[k] += 1;
         }
@@ -402,7 +406,15 @@ int enq_step(pic_ctx *c) {
     mark(c, PIC_STAGE_PUSH_DEPOSIT);
     enq_push(c);
     mark(c, PIC_STAGE_MIGRATE);
-    enq_arrive(c);
+    if (c->side) {
+        /* the arrival (particles only) beside the field stages (fields only) */
+        HIPCHK(hipEventRecord(c->ev_dep, c->stream));
+        HIPCHK(hipStreamWaitEvent(c->side, c->ev_dep, 0));
+        enq_arrive(c, c->side);
+        HIPCHK(hipEventRecord(c->ev_join, c->side));
+    } else {
+        enq_arrive(c, c->stream);
+    }
     mark(c, PIC_STAGE_FOLD_J);
     enq_fold(c);
     mark(c, PIC_STAGE_H2);
@@ -414,6 +426,7 @@ int enq_step(pic_ctx *c) {
     mark(c, PIC_STAGE_ENERGY);
     enq_energy(c);
     mark(c, PIC_NSTAGES);
+    if (c->side) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return PIC_SUCCESS;
@@ -538,6 +551,13 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     };
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e != hipSuccess) return fail(record_hip(e, "hipStreamCreate"));
+    /* both streams at the default priority */
+    e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+    if (e != hipSuccess) return fail(record_hip(e, "hipStreamCreate"));
+    e = hipEventCreateWithFlags(&c->ev_dep, hipEventDisableTiming);
+    if (e != hipSuccess) return fail(record_hip(e, "hipEventCreate"));
+    e = hipEventCreate(&c->ev_join);
+    if (e != hipSuccess) return fail(record_hip(e, "hipEventCreate"));
     c->fp32 = prm->precision == PIC_FP32;
     for (int k = 0; k < PIC_NFIELDS; k++) {
         void **ptr;
@@ -607,6 +627,9 @@ int pic_destroy(pic_ctx *c) {
     if (c->d_slot_sort) hipFree(c->d_slot_sort);
     for (int k = 0; k <= PIC_NSTAGES; k++)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
+    if (c->ev_dep) hipEventDestroy(c->ev_dep);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->side) hipStreamDestroy(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return PIC_SUCCESS;

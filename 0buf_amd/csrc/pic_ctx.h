@@ -32,6 +32,11 @@ struct pic_ctx {
     long NC = 0;
     int dev = 0;
     hipStream_t stream = nullptr;
+    /* a step runs the arrival on `side` while the field stages run on
+     * `stream` (the two touch disjoint data); ev_dep orders it after the
+     * push, ev_join joins it before the step's end */
+    hipStream_t side = nullptr;
+    hipEvent_t ev_dep = nullptr, ev_join = nullptr;
     bool fp32 = false;
     FieldPtrs f{};          /* FP64 plain form */
     PartPtrs p{};
