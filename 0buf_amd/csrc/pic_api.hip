@@ -387,10 +387,7 @@ void accumulate_profile(pic_ctx *c) {
     if (!c->prof) return;
     for (int k = 0; k < PIC_NSTAGES; k++) {
         float ms = 0.f;
-        /* the arrival runs on the side stream: from the end of the push to
-         * its own end */
-        hipEvent_t end = (k == PIC_STAGE_MIGRATE && c->side) ? c->ev_join : c->ev[k + 1];
-        if (hipEventElapsedTime(&ms, c->ev[k], end) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
             c->prof_ms[k] += ms;
             c->prof_n[k] += 1;
         }
@@ -406,8 +403,11 @@ int enq_step(pic_ctx *c) {
     mark(c, PIC_STAGE_PUSH_DEPOSIT);
     enq_push(c);
     mark(c, PIC_STAGE_MIGRATE);
-    if (c->side) {
-        /* the arrival (particles only) beside the field stages (fields only) */
+    /* the arrival (particles only) beside the field stages (fields only);
+     * with stage timing on, one after the other so that each stage's events
+     * bracket its own kernels */
+    const bool overlap = c->side && !c->prof;
+    if (overlap) {
         HIPCHK(hipEventRecord(c->ev_dep, c->stream));
         HIPCHK(hipStreamWaitEvent(c->side, c->ev_dep, 0));
         enq_arrive(c, c->side);
@@ -426,7 +426,7 @@ int enq_step(pic_ctx *c) {
     mark(c, PIC_STAGE_ENERGY);
     enq_energy(c);
     mark(c, PIC_NSTAGES);
-    if (c->side) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+    if (overlap) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_red, c->d_red, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     return PIC_SUCCESS;

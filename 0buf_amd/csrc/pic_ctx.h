@@ -34,7 +34,8 @@ struct pic_ctx {
     hipStream_t stream = nullptr;
     /* a step runs the arrival on `side` while the field stages run on
      * `stream` (the two touch disjoint data); ev_dep orders it after the
-     * push, ev_join joins it before the step's end */
+     * push, ev_join joins it before the step's end.  With stage timing on,
+     * the step stays on `stream` alone. */
     hipStream_t side = nullptr;
     hipEvent_t ev_dep = nullptr, ev_join = nullptr;
     bool fp32 = false;
