@@ -55,9 +55,10 @@ struct PushArgs32 {
     int snap_on;
 };
 
-/* round-to-nearest int64 of v (|v| < 2^51) by the 1.5*2^52 shifter */
-__device__ __forceinline__ long long fix64(float v) {
-    double d = (double)v + 6755399441055744.0;
+/* round-to-nearest int64 of a*b (|a*b| < 2^51) by the 1.5*2^52 shifter: the
+ * product of two FP32 values is exact in FP64, so the fma rounds once */
+__device__ __forceinline__ long long fixmul(double a, float b) {
+    const double d = fma(a, (double)b, 6755399441055744.0);
     return (long long)(__double_as_longlong(d) - 0x4338000000000000LL);
 }
 
@@ -81,21 +82,21 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, in
     const float lx = 1.0f - mx, ly = 1.0f - my, lz = 1.0f - mz;
     const float syz = dy * dz * (1.0f / 12.0f), sxz = dx * dz * (1.0f / 12.0f),
                 sxy = dx * dy * (1.0f / 12.0f);
-    const float qx = cj[0] * dx, qy = cj[1] * dy, qz = cj[2] * dz;
+    const double qx = (double)(cj[0] * dx), qy = (double)(cj[1] * dy), qz = (double)(cj[2] * dz);
     const int n = ((ca + rdx) * BY + (cb + rdy)) * BZ + (cc + rdz);
     long long *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
-    add64(&Jx[n], fix64(qx * (ly * lz + syz)));
-    add64(&Jx[n + 1], fix64(qx * (ly * mz - syz)));
-    add64(&Jx[n + BZ], fix64(qx * (my * lz - syz)));
-    add64(&Jx[n + BZ + 1], fix64(qx * (my * mz + syz)));
-    add64(&Jy[n], fix64(qy * (lx * lz + sxz)));
-    add64(&Jy[n + 1], fix64(qy * (lx * mz - sxz)));
-    add64(&Jy[n + BY * BZ], fix64(qy * (mx * lz - sxz)));
-    add64(&Jy[n + BY * BZ + 1], fix64(qy * (mx * mz + sxz)));
-    add64(&Jz[n], fix64(qz * (lx * ly + sxy)));
-    add64(&Jz[n + BZ], fix64(qz * (lx * my - sxy)));
-    add64(&Jz[n + BY * BZ], fix64(qz * (mx * ly - sxy)));
-    add64(&Jz[n + BY * BZ + BZ], fix64(qz * (mx * my + sxy)));
+    add64(&Jx[n], fixmul(qx, ly * lz + syz));
+    add64(&Jx[n + 1], fixmul(qx, ly * mz - syz));
+    add64(&Jx[n + BZ], fixmul(qx, my * lz - syz));
+    add64(&Jx[n + BZ + 1], fixmul(qx, my * mz + syz));
+    add64(&Jy[n], fixmul(qy, lx * lz + sxz));
+    add64(&Jy[n + 1], fixmul(qy, lx * mz - sxz));
+    add64(&Jy[n + BY * BZ], fixmul(qy, mx * lz - sxz));
+    add64(&Jy[n + BY * BZ + 1], fixmul(qy, mx * mz + sxz));
+    add64(&Jz[n], fixmul(qz, lx * ly + sxy));
+    add64(&Jz[n + BZ], fixmul(qz, lx * my - sxy));
+    add64(&Jz[n + BY * BZ], fixmul(qz, mx * ly - sxy));
+    add64(&Jz[n + BY * BZ + BZ], fixmul(qz, mx * my + sxy));
 }
 
 /* groups of a cell of the live FP32 buckets, in slot order: bucket lines of
