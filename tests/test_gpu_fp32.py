@@ -16,7 +16,9 @@ the reference step) or against an FP64 invariant:
   * particle moments: sums of positions and momenta within 1e-5 relative;
   * bucket regrow and a full inbox (arrivals past the 16 inbox records of a
     cell): bit-identical to ample buckets, and within FP32 rounding of the
-    FP64 form, respectively.
+    FP64 form, respectively;
+  * the two-stream step, the one-stream step (stage timing on) and the stage
+    entry points: bit-identical.
 """
 import numpy as np
 import pytest
@@ -201,3 +203,38 @@ def test_fp32_full_inbox_matches_fp64(picmod):
         assert np.abs(p32[k] - p64[k]).max() <= 1e-5 * 0.1, k
     for k in ("px", "py", "pz"):
         assert np.abs(p32[k] - p64[k]).max() <= 1e-5, k
+
+
+def test_fp32_step_paths_bit_identical(picmod, cfg0):
+    """The step runs its arrival on a second stream beside the field stages
+    unless stage timing is on; stage-by-stage calls run everything on one
+    stream.  With the order-free integer current all three give the same
+    energies, fields and particles, bit for bit."""
+    from helpers import sorted_particles
+    params, parts = cfg0
+    names = ("yee_h_half", "push_deposit", "migrate", "fold_j", "yee_h2", "yee_e", "e_periodic")
+    runs = []
+    for mode in ("overlap", "timed", "stages"):
+        g = fp32_pic(picmod, params)
+        load_workload(g, parts)
+        if mode == "timed":
+            g.profile(True)
+        e = []
+        for nt in range(1, 21):
+            if mode == "stages":
+                for nm in names:
+                    g.stage(nm)
+                e.append(g.energy())
+            else:
+                e.append(g.step(nt))
+        runs.append((np.array(e), [g.get_field(i) for i in range(9)], g.cell_counts(),
+                     sorted_particles(g.particles())))
+        assert g.status()["flags"] == 0
+        g.close()
+    for other in runs[1:]:
+        assert np.array_equal(runs[0][0], other[0])
+        for a, b in zip(runs[0][1], other[1]):
+            assert np.array_equal(a, b)
+        assert np.array_equal(runs[0][2], other[2])
+        for k in runs[0][3]:
+            assert np.array_equal(runs[0][3][k], other[3][k]), k
