@@ -46,6 +46,7 @@ def load():
         "orc_compare": (C.c_double, [dp, dp, C.c_int, C.c_double]),
         "orc_rms": (C.c_double, [dp, dp, C.c_int]),
         "orc_boris": (None, [dp, dp, C.c_double, C.c_double, dp, dp, dp]),
+        "orc_relay": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -113,6 +114,16 @@ class Oracle:
 
     def particle_count(self):
         return int(self.lib.orc_total_particles(self.h))
+
+    def relay(self, cell, x, x1):
+        """cell.h:1306-1403: crossing code m and relay point of a particle of
+        cell (i, l, k) moving x -> x1 (x1 itself when m = 0)."""
+        a = np.array(x, dtype=np.float64)
+        b = np.array(x1, dtype=np.float64)
+        r = np.empty(3)
+        m = self.lib.orc_relay(self.h, int(cell[0]), int(cell[1]), int(cell[2]), _d(a), _d(b),
+                               _d(r))
+        return int(m), r
 
     def cell_counts(self):
         a = np.empty(self.ncell, dtype=np.int32)

@@ -258,13 +258,13 @@ inline double reflect1(double x, double xm) {
     return (x > xm) * (x - xm) + (x < 0.0) * (xm + x) + (x > 0 && x < xm) * x;
 }
 
-/* cell.h:1306-1422 CurrentToMesh: zig-zag relay + pqr; then x=x1 and Reflect */
-inline int current_to_mesh(const CellGeom &c, OParticle &p, TTensor &t1, TTensor &t2) {
-    D3 x = {p.x, p.y, p.z};
-    D3 x1 = {p.x1, p.y1, p.z1};
-    double mass = p.m;
-    int cells = 1;
-    I3 i1, i2;
+/* cell.h:1306-1403 the zig-zag relay point of CurrentToMesh: dual cells i1
+ * of x and i2 of x1 (getCellNumberCenterCurrent), crossing code
+ * m = 4 di + 2 dl + dk, and the relay point x2 on the crossed dual faces;
+ * for two crossed faces the third coordinate follows the path at the
+ * parameter s of the least-squares projection onto the crossed edge
+ * (cell.h:1374-1403). Returns m (0 when no face is crossed). */
+inline int relay_point(const CellGeom &c, const D3 &x, const D3 &x1, I3 &i1, I3 &i2, D3 &x2) {
     i1.x = cell_number_center_current(x.x, c.x0, c.hx);
     i1.y = cell_number_center_current(x.y, c.y0, c.hy);
     i1.z = cell_number_center_current(x.z, c.z0, c.hz);
@@ -273,7 +273,6 @@ inline int current_to_mesh(const CellGeom &c, OParticle &p, TTensor &t1, TTensor
     i2.z = cell_number_center_current(x1.z, c.z0, c.hz);
     int di = std::abs(i2.x - i1.x), dl = std::abs(i2.y - i1.y), dk = std::abs(i2.z - i1.z);
     int m = 4 * di + 2 * dl + dk;
-    D3 x2;
     double s;
     switch (m) {
     case 1:
@@ -324,6 +323,18 @@ inline int current_to_mesh(const CellGeom &c, OParticle &p, TTensor &t1, TTensor
         m = 0;
         break;
     }
+    return m;
+}
+
+/* cell.h:1306-1422 CurrentToMesh: zig-zag relay + pqr; then x=x1 and Reflect */
+inline int current_to_mesh(const CellGeom &c, OParticle &p, TTensor &t1, TTensor &t2) {
+    D3 x = {p.x, p.y, p.z};
+    D3 x1 = {p.x1, p.y1, p.z1};
+    double mass = p.m;
+    int cells = 1;
+    I3 i1, i2;
+    D3 x2;
+    int m = relay_point(c, x, x1, i1, i2, x2);
     if (m == 0) {
         pqr(c, i1, x, x1, mass, c.tau, t1);
     } else {
@@ -807,6 +818,19 @@ double orc_rms(const double *a, const double *b, int size) {
     double diff = 0.0;
     for (int n = 0; n < size; n++) diff += std::pow(a[n] - b[n], 2.0);
     return std::pow(diff / (size), 0.5);
+}
+
+int orc_relay(const orc_plasma *P, int i, int l, int k, const double x[3], const double x1[3],
+              double x2[3]) {
+    const CellGeom c = geom(P, i, l, k);
+    const D3 a = {x[0], x[1], x[2]}, b = {x1[0], x1[1], x1[2]};
+    I3 i1, i2;
+    D3 r = b;
+    const int m = relay_point(c, a, b, i1, i2, r);
+    x2[0] = m ? r.x : b.x;
+    x2[1] = m ? r.y : b.y;
+    x2[2] = m ? r.z : b.z;
+    return m;
 }
 
 void orc_boris(const double E[3], const double H[3], double tau, double q_m, double x[3],

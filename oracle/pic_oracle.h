@@ -67,6 +67,12 @@ void orc_moments(const orc_plasma *p, double *out);
 double orc_compare(const double *a, const double *b, int num, double tol);
 double orc_rms(const double *a, const double *b, int size);
 
+/* the zig-zag relay point of CurrentToMesh (cell.h:1306-1403) for a particle
+ * of cell (i, l, k) moving x -> x1: returns the crossing code m and writes
+ * the relay point (x1 itself when m = 0); for the relay-point tests */
+int orc_relay(const orc_plasma *P, int i, int l, int k, const double x[3], const double x1[3],
+              double x2[3]);
+
 /* one Boris push on a single particle (particle.h:160-202); for golden tests */
 void orc_boris(const double E[3], const double H[3], double tau, double q_m,
                double x[3], double p[3], double x1[3]);

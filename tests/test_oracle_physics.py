@@ -6,8 +6,10 @@ index, a dropped term, a wrong sign or staggering) fails it:
   gather         a field linear in each coordinate, on each component's own
                  staggering, interpolates exactly; H recovered from rotation
   deposit        textbook Villasenor-Buneman current of a straight segment
-                 (no crossing and single-face crossings), and the discrete
-                 continuity equation for every crossing case
+                 (no crossing and single-face crossings), the discrete
+                 continuity equation for every crossing case, and the relay
+                 points of every crossing case by geometry (faces found by
+                 search, the two-face parameter by a scalar minimiser)
   Yee            the leapfrog recurrence of a discrete eigenmode, the Gauss
                  law residual conserved, div H preserved in the interior
   migration      brute-force cell membership, exact per-cell counts
@@ -227,6 +229,68 @@ def test_deposit_vb_single_face_crossing_straight_split(orc, axis):
         for k in range(3):
             assert np.allclose(J[k], want[k], rtol=0, atol=1e-12), (axis, done, k)
         done += 1
+
+
+def test_relay_points_by_geometry(orc):
+    """cell.h:1306-1403 relay points on an anisotropic grid, against geometry
+    computed here without the restated formulas: the crossed dual faces
+    (cell-centre planes) are found by search; every crossed coordinate of the
+    relay point lies on its face; one face: the relay point is on the straight
+    path; two faces: the free coordinate follows the path at the parameter that
+    minimises the physical distance from the path to the crossed edge (found
+    by a bounded scalar minimiser); three faces: the corner. The grid is
+    anisotropic so that a projection in cell units (or any other weights)
+    misses, which the test also checks."""
+    from scipy.optimize import minimize_scalar
+
+    n = np.array([6, 5, 4])
+    h = np.array([0.1, 0.07, 0.13])
+    o = orc.Oracle(tuple(n), tuple(n * h), TAU)
+    rng = np.random.default_rng(1306)
+    seen = dict.fromkeys(range(8), 0)
+    tol = 1e-10
+    missed_cell_units = []
+    try:
+        while min(seen.values()) < 40:
+            cell = rng.integers(1, n - 1)
+            x = (cell + rng.uniform(0, 1, 3)) * h
+            d = rng.uniform(-0.95, 0.95, 3) * h
+            x1 = x + d
+            m, x2 = o.relay(cell, x, x1)
+            planes = []
+            for a in range(3):
+                lo, hi = min(x[a], x1[a]), max(x[a], x1[a])
+                js = [j for j in range(-1, n[a] + 1) if lo < (j + 0.5) * h[a] <= hi]
+                assert len(js) <= 1
+                planes.append((j_ + 0.5) * h[a] if (js and (j_ := js[0]) is not None) else None)
+            crossed = [a for a in range(3) if planes[a] is not None]
+            assert m == sum(4 >> a for a in crossed), (x, x1, m)
+            seen[m] += 1
+            for a in crossed:
+                assert abs(x2[a] - planes[a]) <= tol * h[a], (m, a)
+            if len(crossed) == 0:
+                assert np.array_equal(x2, x1)
+            elif len(crossed) == 1:
+                # collinear with the path and between its ends
+                assert np.linalg.norm(np.cross(x2 - x, d)) <= tol * float(h @ h)
+                assert np.all((x2 - x) * d >= -tol * h.max())
+                assert np.all((x1 - x2) * d >= -tol * h.max())
+            elif len(crossed) == 2:
+                a, b = crossed
+                f = 3 - a - b
+                dist = lambda s, w: (w[a] * (x[a] + s * d[a] - planes[a])) ** 2 + \
+                    (w[b] * (x[b] + s * d[b] - planes[b])) ** 2
+                opt = dict(method="bounded", bounds=(-2.0, 3.0), options={"xatol": 1e-13})
+                s_phys = minimize_scalar(dist, args=(np.ones(3),), **opt).x
+                # the minimiser stops at about sqrt(eps) |s| (1.5e-8 relative)
+                assert abs(x2[f] - (x[f] + s_phys * d[f])) <= 1e-8 * h[f], (m, s_phys)
+                s_cell = minimize_scalar(dist, args=(1.0 / h,), **opt).x
+                missed_cell_units.append(abs(x2[f] - (x[f] + s_cell * d[f])) > 1e-6 * h[f])
+            else:
+                assert np.allclose(x2, planes, rtol=0, atol=tol * h.max())
+    finally:
+        o.close()
+    assert np.mean(missed_cell_units) > 0.9
 
 
 def test_continuity_every_crossing_case(orc, picmod):
