@@ -9,6 +9,13 @@
  * the reference's per-cell AoS particle lists, 5x5x5 field tiles, 10-buffer
  * current deposit, 27-neighbour arrival scan and swap-with-last removal.
  * Every function in pic_oracle.cpp cites the reference file:line it follows.
+ *
+ * Pins (no function is left unpinned): the Boris push and the seeded-input
+ * RNG bit-exact against the reference's own particle.h and maxwell.cxx built
+ * unmodified (tests/test_oracle_golden.py); gather, deposit (straight VB
+ * segments, continuity for every crossing case, relay points by geometry),
+ * Yee, migration and energy by closed forms and invariants
+ * (tests/test_oracle_physics.py).
  */
 #ifndef PIC_ORACLE_H
 #define PIC_ORACLE_H
