@@ -11,6 +11,8 @@ the reference step) or against an FP64 invariant:
     2.5e-11);
   * determinism: two runs give bit-identical fields and energies (the J sums
     are exact integer sums, so no result depends on atomic order);
+  * J after one push from the same state, per node: within 1e-5 of max|J|
+    of the FP64 form (measured 2.2e-6), which checks the FP32 relay points;
   * per-cell counts: >= 99% of cells equal to the oracle's at step 100
     (FP32 offsets round differently near faces; measured 100% on config 0);
   * particle moments: sums of positions and momenta within 1e-5 relative;
@@ -106,6 +108,35 @@ def test_fp32_deterministic(picmod, cfg0):
     for a, b in zip(runs[0][1], runs[1][1]):
         assert np.array_equal(a, b)
     assert np.array_equal(runs[0][2], runs[1][2])
+
+
+def test_fp32_current_matches_fp64_per_node(picmod, cfg0):
+    """One H half step and one push/deposit from the config-0 state: every J
+    node of the FP32 form within 1e-5 of max|J| of the FP64 form. The FP32
+    form has its own relay points (cell units, h^2-weighted two-face
+    projection); the FP64 form's are the reference's, pinned to the oracle.
+    Every crossing case m = 0..7 occurs in this step. Measured: 1.0e-6, 1.1e-6
+    and 2.2e-6 of max|J| for Jx, Jy, Jz (FP32 offsets and momenta, and the
+    rare particle whose crossing FP32 rounding flips). A relay point off its
+    place moves the two segments of that particle by a sizeable part of its
+    contribution, about 1e-3 of max|J| (some 64 particles feed a node)."""
+    params, parts = cfg0
+    J = {}
+    for prec in (picmod.FP64, picmod.FP32):
+        p = picmod.make_params((params.nx, params.ny, params.nz),
+                               (params.lx, params.ly, params.lz), params.tau, precision=prec)
+        g = picmod.Pic(p)
+        load_workload(g, parts)
+        g.stage("yee_h_half")
+        g.stage("push_deposit")
+        g.sync()
+        J[prec] = [g.get_field(6 + k) for k in range(3)]
+        g.close()
+    for k in range(3):
+        ref = J[picmod.FP64][k]
+        err = np.abs(J[picmod.FP32][k] - ref).max() / np.abs(ref).max()
+        print(f"J{'xyz'[k]} FP32 vs FP64 after one push: max |diff| / max |J| = {err:.3e}")
+        assert err <= 1e-5, k
 
 
 def test_fp32_matches_fp64_gpu_at_larger_grid(picmod):
