@@ -186,6 +186,46 @@ def test_minimum_grid_corner_crossings(picmod, orc):
     g.close()
 
 
+@pytest.mark.parametrize("case", ["empty", "single"])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_empty_and_single_particle_domains(picmod, orc, case, prec):
+    """Random E, H on a grid whose z extent is not a whole number of tiles:
+    no particles at all (every tile empty), or one electron crossing faces
+    (one lane of one chunk, every other tile empty); 40 steps of either form
+    against the oracle. FP64: energies to 1e-10, counts exact, position to
+    1e-12 h. FP32: energies to 1e-4, position to 1e-5 h (FP32 fields, push
+    and offsets)."""
+    n = (10, 8, 12)
+    L = tuple(0.1 * k for k in n)
+    G = (n[0] + 2) * (n[1] + 2) * (n[2] + 2)
+    rng = np.random.default_rng(77)
+    fields = [0.01 * rng.standard_normal(G) for _ in range(6)]
+    parts = []
+    if case == "single":
+        parts = [(1, np.array([0.41]), np.array([0.37]), np.array([0.93]), np.array([0.3]),
+                  np.array([-0.2]), np.array([0.25]), -0.01, -1.0)]
+    params = picmod.make_params(n, L, 0.05)
+    o, g = pair(picmod, orc, params, parts, precision=prec)
+    for i, f in enumerate(fields):
+        o.load_field(i, f)
+        g.load_field(i, f)
+    for nt in range(1, 41):
+        e1, _ = o.step(nt)
+        e2 = g.step(nt)
+        assert e2 == pytest.approx(e1, rel=1e-10 if prec == 0 else 1e-4), nt
+        if prec == 0:
+            assert np.array_equal(g.cell_counts(), o.cell_counts()), nt
+    assert g.particle_count() == len(parts)
+    assert g.status()["flags"] == 0
+    if case == "single":
+        pg, po = g.particles(), o.particles()
+        # in units of the cell size h = 0.1: FP32 offsets carry about 1e-7 h
+        # of rounding per step
+        for k in ("x", "y", "z"):
+            assert abs(pg[k][0] - po[k][0]) <= (1e-12 if prec == 0 else 1e-5) * 0.1, k
+    g.close()
+
+
 def test_bucket_regrow_keeps_parity(picmod, orc, cfg0):
     """Buckets sized to the initial maximum overflow on the first arrivals;
     the library regrows them and replaces the overflowed arrivals."""
