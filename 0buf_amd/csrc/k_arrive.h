@@ -12,11 +12,13 @@
  * a skewed workload sends tens of particles per step into the same cells);
  * then every entry takes base + rank as its slot.  A slot past the bucket
  * capacity is kept in q.slot for the regrow path and raises
- * PIC_F_BUCKET_OVERFLOW.
+ * PIC_F_BUCKET_OVERFLOW.  An entry whose halo index or destination cell lies
+ * out of range is not placed (slot -1) and raises PIC_F_QUEUE_CORRUPT.
  */
 #ifndef K_ARRIVE_H
 #define K_ARRIVE_H
 
+#include "k_queue_key.h"
 #include "pic_internal.h"
 
 template <int TX, int TY, int TZ, class P, class Q>
@@ -35,8 +37,19 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive_tile(Grid g, TileGeom t, P
     const long qb = tile * (long)q.qcap;
     for (int h = tid; h < NH; h += PIC_BLOCK) hcnt[h] = 0;
     __syncthreads();
-    /* rank of each entry among the tile's entries for the same cell */
-    for (int j = tid; j < n; j += PIC_BLOCK) q.slot[qb + j] = atomicAdd(&hcnt[q.halo[qb + j]], 1);
+    /* rank of each entry among the tile's entries for the same cell; an
+     * entry whose halo index or cell lies out of range is refused (slot -1) */
+    const long ncell = (long)g.Nx * g.Ny * g.Nz;
+    bool bad = false;
+    for (int j = tid; j < n; j += PIC_BLOCK) {
+        const int h = q.halo[qb + j];
+        if (!arrival_entry_ok(h, NH, (long)q.dest[qb + j], ncell)) {
+            q.slot[qb + j] = -1;
+            bad = true;
+            continue;
+        }
+        q.slot[qb + j] = atomicAdd(&hcnt[h], 1);
+    }
     __syncthreads();
     /* one claim per destination cell on its bucket count */
     for (int h = tid; h < NH; h += PIC_BLOCK) {
@@ -53,6 +66,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive_tile(Grid g, TileGeom t, P
     bool over = false;
     for (int j = tid; j < n; j += PIC_BLOCK) {
         const long e = qb + j;
+        if (q.slot[e] < 0) continue;
         const int slot = hbase[q.halo[e]] + q.slot[e];
         q.slot[e] = slot;
         if (slot >= p.cap) {
@@ -65,6 +79,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive_tile(Grid g, TileGeom t, P
         if (p.id) p.id[to] = q.id[e];
     }
     if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+    if (bad) atomicOr(&st->flags, (unsigned)PIC_F_QUEUE_CORRUPT);
 }
 
 #endif

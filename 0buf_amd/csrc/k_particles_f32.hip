@@ -20,6 +20,7 @@
  */
 #include "k_arrive.h"
 #include "k_push_math32.h"
+#include "k_queue_key.h"
 
 /* copies of the LDS J box: lane l deposits into copy l % JCOPIES, which
  * divides the same-address conflicts of the 64-bit LDS atomics (consecutive
@@ -341,7 +342,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                     /* destination in the halo box (|shift| <= 1 per axis) */
                     const int h = ((ca + shx + 1) * BY + (cb + shy + 1)) * BZ + (cc + shz + 1);
                     const int hr = atomicAdd(&sHalo[h], 1);
-                    const unsigned key = (unsigned)s | ((unsigned)h << 8) | ((unsigned)hr << 16);
+                    const unsigned key = queue_key(s, (unsigned)h, (unsigned)hr);
                     float4 *q = a.q.e + 2 * (tile * (long)a.q.qcap + qs_);
                     q[0] = make_float4(nxo, nyo, nzo, pu);
                     q[1] = make_float4(pv, pw, __uint_as_float(key), __int_as_float(pid));
@@ -506,6 +507,8 @@ template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, PartPtrs32 p,
                                                        QueuePtrs32 q, DevStatus *st) {
     constexpr int NH = (TX + 2) * (TY + 2) * (TZ + 2);
+    static_assert(NH <= 256, "a record key holds the halo index in 8 bits");
+    __shared__ int hn[NH];
     __shared__ int hbase[NH];
     __shared__ int hbc[NH];
     __shared__ long hcell[NH];
@@ -522,6 +525,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
     }
     for (int h = tid; h < NH; h += PIC_BLOCK) {
         const int c = q.hcnt[tile * NH + h];
+        hn[h] = c > 0 ? c : 0;
+        hbase[h] = -1;
+        hbc[h] = 0;
+        hcell[h] = 0;
         if (c > 0) {
             const long cell = halo_cell<TX, TY, TZ>(g, t, tile, h);
             hcell[h] = cell;
@@ -530,7 +537,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
         }
     }
     __syncthreads();
-    bool over = false;
+    bool over = false, bad = false;
     for (int j = tid; j < n; j += PIC_BLOCK) {
         float4 A = A0, B = B0;
         if (j != tid) {
@@ -538,15 +545,15 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
             B = q.e[2 * (qb + j) + 1];
         }
         const unsigned key = __float_as_uint(B.z);
-        const int h = (int)((key >> 8) & 255u);
-        const int slot = hbase[h] + (int)(key >> 16);
+        int slot, r;
+        const int verdict = arrival_slot32(key, NH, hn, hbase, hbc, p.cap, &slot, &r);
         q.slot[qb + j] = slot;
-        if (slot >= p.cap) {
-            over = true;
+        if (verdict != PIC_ARR_OK) {
+            over |= verdict == PIC_ARR_OVER;
+            bad |= verdict == PIC_ARR_BAD;
             continue;
         }
-        const long cell = hcell[h];
-        const int r = slot - hbc[h];
+        const long cell = hcell[key_halo(key)];
         if (r < PIC_INBOX) {
             p.inbox[2 * (cell * PIC_INBOX + r)] = A;
             p.inbox[2 * (cell * PIC_INBOX + r) + 1] = B;
@@ -555,9 +562,11 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
         }
     }
     if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+    if (bad) atomicOr(&st->flags, (unsigned)PIC_F_QUEUE_CORRUPT);
 }
 
-/* regrow: the arrivals whose claimed slot lay past the old capacity */
+/* regrow: the arrivals whose claimed slot lay past the old capacity (a
+ * record the arrival refused carries slot -1) */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(Grid g, TileGeom t, QueuePtrs32 q,
                                                              int oldcap, PartPtrs32 to) {
@@ -570,8 +579,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(Grid g, TileGeom t
         if (slot < oldcap || slot >= to.cap) continue;
         const float4 A = q.e[2 * (qb + j)], B = q.e[2 * (qb + j) + 1];
         const unsigned key = __float_as_uint(B.z);
-        place32(to, halo_cell<TX, TY, TZ>(g, t, tile, (int)((key >> 8) & 255u)), slot, A, B,
-                key);
+        place32(to, halo_cell<TX, TY, TZ>(g, t, tile, key_halo(key)), slot, A, B, key);
     }
 }
 

@@ -442,6 +442,7 @@ int finish_step(pic_ctx *c) {
     c->last.flags = (int)st.flags;
     c->last.bucket_capacity = bucket_cap(c);
     c->last.bucket_regrows = c->regrows;
+    if (st.flags & PIC_F_QUEUE_CORRUPT) return PIC_ERR_QUEUE_CORRUPT;
     if (st.flags & PIC_F_BUCKET_OVERFLOW) {
         int rc = regrow(c);
         if (rc) return rc;
@@ -510,6 +511,7 @@ const char *pic_error_string(int s) {
     case PIC_ERR_NOMEM: return "device out of memory";
     case PIC_ERR_STATE: return "call out of order";
     case PIC_ERR_CELL_RANGE: return "particle stencil outside its tile";
+    case PIC_ERR_QUEUE_CORRUPT: return "departure record out of range (not placed)";
     default: return "unknown status";
     }
 }
@@ -523,6 +525,14 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
     if (prm->precision != PIC_FP64 && prm->precision != PIC_FP32) return PIC_ERR_INVALID;
     if (prm->ghost_mode != 0 && prm->ghost_mode != 1) return PIC_ERR_INVALID;
+    /* The FP32 push takes a particle's end to lie in the start's dual cell
+     * or the next one and its new cell to be the owner or a face neighbour:
+     * a displacement below half a cell per axis.  With |v| < c = 1 that holds
+     * for every particle when tau <= h/2 on every axis. */
+    if (prm->precision == PIC_FP32 &&
+        (prm->tau > 0.5 * (prm->lx / prm->nx) || prm->tau > 0.5 * (prm->ly / prm->ny) ||
+         prm->tau > 0.5 * (prm->lz / prm->nz)))
+        return PIC_ERR_INVALID;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
     if (prm->device < 0 || prm->device >= ndev) return PIC_ERR_INVALID;

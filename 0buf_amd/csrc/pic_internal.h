@@ -29,7 +29,8 @@ enum {
     PIC_F_BUCKET_OVERFLOW = 4,
     PIC_F_RANGE = 8,
     PIC_F_RECOMPUTE = 16,    /* differentiable form: recompute differs from the first run */
-    PIC_F_RECORD = 32        /* differentiable form: a step record ran out of room */
+    PIC_F_RECORD = 32,       /* differentiable form: a step record ran out of room */
+    PIC_F_QUEUE_CORRUPT = 64 /* an arrival refused a departure record out of range */
 };
 
 struct DevStatus {

@@ -67,7 +67,10 @@ enum pic_status {
     PIC_ERR_QUEUE_OVERFLOW = 5,     /* departure queue of a tile overflowed */
     PIC_ERR_NOMEM = 6,              /* device allocation failed */
     PIC_ERR_STATE = 7,              /* call out of order (e.g. no particles) */
-    PIC_ERR_CELL_RANGE = 8          /* a particle's stencil left its tile box */
+    PIC_ERR_CELL_RANGE = 8,         /* a particle's stencil left its tile box */
+    PIC_ERR_QUEUE_CORRUPT = 9       /* a departure record named a destination
+                                       outside its tile's halo box, or a slot
+                                       no claim covers; it was not placed */
 };
 
 /* field ids: the 12 ghosted arrays of init.cu:310-331 */
@@ -133,7 +136,8 @@ typedef struct pic_step_status {
     int bucket_regrows;       /* bucket capacity increases so far */
     int bucket_capacity;      /* current particle slots per cell */
     int flags;                /* bit0 too many, bit1 queue overflow,
-                                 bit2 bucket overflow (repaired), bit3 range */
+                                 bit2 bucket overflow (repaired), bit3 range,
+                                 bit6 departure record refused */
 } pic_step_status;
 
 void pic_params_default(pic_params *p);

@@ -1,7 +1,9 @@
 /* Check that the FMA-corrected quotient q = fma(x - q0*h, r, q0), q0 = x*r,
  * r = RN(1/h), equals the correctly rounded x/h (Markstein's theorem), for
  * the divisors the FP64 push uses and for random divisors.  Prints the
- * number of mismatches.  Build: gcc -O2 -ffp-contract=off divc_check.c -lm */
+ * number of mismatches.  Build: gcc -O2 -ffp-contract=off divc_check.c -lm
+ * (tests/test_divc.py builds and runs it; -DUNCORRECTED checks that the probe
+ * sees the mismatches of the uncorrected product x * r). */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -14,6 +16,9 @@ static double u01(void) {
 }
 static double divc(double x, double h, double r) {
     double q0 = x * r;
+#ifdef UNCORRECTED
+    return q0;   /* the product alone: must show mismatches */
+#endif
     double e = fma(-q0, h, x);
     return fma(e, r, q0);
 }

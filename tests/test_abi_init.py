@@ -169,6 +169,44 @@ def test_create_without_gpu_reports_no_device(picmod):
     assert lib.pic_create(C.byref(p), C.byref(ctx)) == 3
 
 
+def test_create_refuses_fp32_step_over_half_a_cell(picmod):
+    """The FP32 push moves a particle at most one cell and puts its end in the
+    start's dual cell or the next: |v| tau / h < 1/2 on every axis, which
+    |v| < 1 guarantees only for tau <= h/2.  A longer step is refused before
+    any device call (so the answer is the same with and without a GPU); at
+    tau = h/2 creation gets past the check."""
+    import torch
+    lib = picmod.load_library()
+    ok_code = 0 if torch.cuda.is_available() else 3
+    # default grid: 32 x 8 x 8 cells of h = 0.1, tau = 0.05 = h/2; shrink one
+    # axis's h (or lengthen tau) by 0.2% to cross the limit on that axis
+    for axis, want in (("tau", 1), ("x", 1), ("y", 1), ("z", 1), (None, ok_code)):
+        p = picmod.default_params()
+        p.precision = picmod.FP32
+        assert (p.lx / p.nx, p.ly / p.ny, p.lz / p.nz, p.tau) == (0.1, 0.1, 0.1, 0.05)
+        if axis == "tau":
+            p.tau = 0.0501
+        elif axis == "x":
+            p.lx *= 0.998
+        elif axis == "y":
+            p.ly *= 0.998
+        elif axis == "z":
+            p.lz *= 0.998
+        ctx = C.c_void_p()
+        rc = lib.pic_create(C.byref(p), C.byref(ctx))
+        if rc == 0:
+            lib.pic_destroy(ctx)
+        assert rc == want, (axis, rc)
+    p = picmod.default_params()
+    p.precision = picmod.FP64
+    p.tau = 0.0501          # the FP64 form checks ranges on the device instead
+    ctx = C.c_void_p()
+    rc = lib.pic_create(C.byref(p), C.byref(ctx))
+    if rc == 0:
+        lib.pic_destroy(ctx)
+    assert rc == ok_code
+
+
 def test_param_file_roundtrip_matches_reference_writer(picmod, tmp_path):
     """params_ref.dat was written by the reference's writeParamsFile
     (maxwell.cu:330-377); the reader must recover its values and the writer
