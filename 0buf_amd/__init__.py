@@ -106,6 +106,8 @@ EXPORTS = [
     # readback and timing
     "pic_profile", "pic_profile_read", "pic_get_status", "pic_particle_count", "pic_get_cell_counts",
     "pic_get_particles", "pic_moments", "pic_gauss_residual", "pic_cell_moments",
+    # moments without allocation (static scratch), for callers that cannot allocate
+    "pic_moments_noalloc",
     # state dumps (mumu files) and control points
     "pic_get_params", "pic_dump_file_name", "pic_dump_write", "pic_dump_read",
     "pic_save_state_dump", "pic_load_state_dump", "pic_control_point",
@@ -173,6 +175,7 @@ def load_library(path=LIB_PATH):
         "pic_get_particles": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp, dp, ip, ip,
                                         C.c_long, lp]),
         "pic_moments": (C.c_int, [vp, dp]),
+        "pic_moments_noalloc": (C.c_int, [vp, C.c_int, vp]),
         "pic_gauss_residual": (C.c_int, [vp, dp, dp, C.c_int]),
         "pic_get_params": (C.c_int, [vp, C.POINTER(PicParams)]),
         "pic_dump_file_name": (C.c_int, [C.c_int, C.c_int, C.c_char_p, C.c_size_t]),

@@ -40,6 +40,7 @@ HIP_SOURCES = {
     "k_adjoint32.hip": ["-ffp-contract=fast"],
     "pic_api.hip": ["-ffp-contract=off"],
     "pic_adjoint.hip": ["-ffp-contract=off"],
+    "pic_noalloc.hip": ["-ffp-contract=off"],
 }
 CXX_SOURCES = {"pic_init.cpp": ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread"],
                "pic_io.cpp": ["-O2", "-fPIC", "-std=c++17"]}

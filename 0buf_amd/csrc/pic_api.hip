@@ -537,6 +537,7 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PIC_ERR_NO_DEVICE;
     if (prm->device < 0 || prm->device >= ndev) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(prm->device));
+    if (int rc0 = noalloc_init(prm->device)) return rc0;
     pic_ctx *c = new pic_ctx();
     c->prm = *prm;
     c->dev = prm->device;

@@ -95,6 +95,9 @@ int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id);
 template <class P>
 void free_parts(P &p);
 void free_adjoint(pic_ctx *c);   /* pic_adjoint.hip */
+/* pic_noalloc.hip: device address of this device's scratch array, taken once
+ * per device with the device current */
+int noalloc_init(int dev);
 }  // namespace picx
 
 #endif

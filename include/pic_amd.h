@@ -216,6 +216,23 @@ int pic_gauss_residual(pic_ctx *ctx, double *res, double *max_drift, int set_ref
  * sum (m/q_m)(gamma-1), sum |px|+|py|+|pz| */
 int pic_moments(pic_ctx *ctx, double *out27);
 
+/* pic_moments without allocation or synchronisation.  No counterpart in the
+ * reference: this entry point is for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  It enqueues the same moment kernels on the context stream and
+ * writes the 27 moments to d_out27, a caller-owned DEVICE array of 27
+ * doubles, valid after pic_sync (or any wait on the context stream).  The
+ * reduction's per-block partial sums live in a static device array of the
+ * library, PIC_NOALLOC_MAX_BLOCKS x 27 doubles (432 KiB) per GPU, held until
+ * the process ends; nblocks (1..PIC_NOALLOC_MAX_BLOCKS, else
+ * PIC_ERR_INVALID) is the number of partial blocks, and with nblocks = 1024
+ * the result is bit for bit pic_moments'.  The context's particles must be
+ * committed already (a step, pic_moments or a readback has run; else
+ * PIC_ERR_STATE).  All calls on one GPU share the array: calls that overlap
+ * on one GPU (two contexts' streams, two host threads) race. */
+#define PIC_NOALLOC_MAX_BLOCKS 2048
+int pic_moments_noalloc(pic_ctx *ctx, int nblocks, double *d_out27);
+
 /* the parameters the context was created with */
 int pic_get_params(pic_ctx *ctx, pic_params *out);
 
