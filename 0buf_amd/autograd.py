@@ -195,16 +195,18 @@ def default_theta(cfg):
 
 
 def precision_compare(cfg=4, theta=None, first=150, last=200, ckpt_every=20, scale_n=None,
-                      nsteps=None):
+                      nsteps=None, log=None):
     """Gradient of the windowed energy loss by the FP32 form (FP32 forward and
     particle adjoint, FP64 field adjoints and accumulation) against the FP64
     adjoint on the same shape and seeded particles.  Returns, per precision,
     the loss, the gradient with respect to theta = (drift, a_1..a_8, Bx0) and
     the device times (ms) of the forward with checkpoints, the recompute and
-    the adjoint steps."""
+    the adjoint steps.  log (optional) receives a progress line per leg."""
     spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
     theta = default_theta(spec_cfg) if theta is None else theta
     workload = pic.generate_workload(spec_cfg, scale_n=scale_n)
+    if log:
+        log(f"generated {sum(len(p[1]) for p in workload[1])} particles")
     out = {}
     for prec in (pic.FP64, pic.FP32):
         bp = BeamPlasma(spec_cfg, precision=prec, nsteps=nsteps, ckpt_every=ckpt_every,
@@ -216,7 +218,70 @@ def precision_compare(cfg=4, theta=None, first=150, last=200, ckpt_every=20, sca
                      "ms": bp.run.last.adjoint_timing(), "n": int(bp.beam.numel())}
         bp.run.last.close()
         bp.run.last = None
+        if log:
+            log(f"{'FP32' if prec == pic.FP32 else 'FP64'} gradient done: loss "
+                f"{out[prec]['loss']:.12e}")
     return out
+
+
+def device_gradient(cfg=4, precision=pic.FP64, first=150, last=200, ckpt_every=20,
+                    seed=12345):
+    """Cost of a gradient of the differentiable form at full size, particles
+    drawn on the GPU by the device initializer (no displacement: the drawn
+    state is s_0): a plain forward of the workload's steps against
+    forward-with-checkpoints + the reverse sweep of the loss
+    sum_{t=first..last} W_t (steps after `last` have a zero adjoint and are
+    not recomputed), both from the same s_0 on one context whose gradient
+    memory a first, untimed gradient allocated.  Also reports whether the
+    recorded forward reproduced the plain forward's energy history bit for
+    bit, the step status (dropped particles, flags) after the sweep, and the
+    norm of the particle adjoint."""
+    import time
+    spec_cfg = pic.beam_plasma_config(cfg) if isinstance(cfg, int) else cfg
+    n = spec_cfg["n"]
+    T = spec_cfg["steps"]
+    params = pic.make_params(n, tuple(k * 0.1 for k in n), spec_cfg["tau"], precision=precision,
+                             differentiable=1, ghost_mode=1,
+                             departure_cap=spec_cfg.get("departure_cap", 50))
+    g = pic.Pic(params)
+    try:
+        g.generate_device(spec_cfg["species"], seed=seed)
+        if spec_cfg["bx0"]:
+            g.load_field(pic.HX, np.full(g.G, spec_cfg["bx0"]))
+        torch.cuda.synchronize()
+        P = g.particle_count()
+        w = np.zeros(T)
+        w[first - 1:last] = 1.0
+        g.forward_record(T, ckpt_every)
+        g.adjoint(weights=w)
+        g.rewind()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        W = g.run(1, T)
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        st_fwd = g.status()
+        g.rewind()
+        t0 = time.perf_counter()
+        Wr = g.forward_record(T, ckpt_every)
+        t1 = time.perf_counter()
+        gr = g.adjoint(weights=w)
+        t2 = time.perf_counter()
+        dev = g.adjoint_timing()
+        st = g.status()
+        P_end = g.particle_count()
+    finally:
+        g.close()
+    pg = math.sqrt(sum(float(np.dot(gr[k], gr[k])) for k in PART_KEYS))
+    return {"particles": P, "particles_end": P_end, "steps": T, "first": first, "last": last,
+            "ckpt_every": ckpt_every, "forward_s": t_fwd, "forward_record_s": t1 - t0,
+            "backward_s": t2 - t1, "gradient_s": t2 - t0, "ratio": (t2 - t0) / t_fwd,
+            "reversed_steps": last,
+            "backward_per_reversed_step_over_forward_step": (t2 - t1) / last / (t_fwd / T),
+            "device_ms": {"forward_with_checkpoints": dev[0], "recompute": dev[1],
+                          "adjoint_steps": dev[2]},
+            "record_matches_forward": bool(np.array_equal(W, Wr)),
+            "status_forward": st_fwd, "status": st, "particle_grad_norm": pg}
 
 
 def _main():

@@ -154,6 +154,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sHalo[NB];    /* leavers bound for each cell of the halo box */
     __shared__ int sQ;
     __shared__ unsigned sFlags;
+    __shared__ float4 sSpec[PIC_MAX_SLOTS];  /* (q/m tau/2, cj) per species slot */
     extern __shared__ unsigned sGrp[];    /* line groups of the tile */
     long long *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
@@ -205,6 +206,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     }
     for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
     for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
+    /* the per-species constants in LDS: a kernel-argument table indexed by a
+     * per-particle value would be a vector memory load in the loop, and its
+     * wait would also wait for the next chunk's loads issued ahead of it */
+    if (tid < PIC_MAX_SLOTS)
+        sSpec[tid] = make_float4(a.qm[tid] * a.tau * 0.5f, a.cj[tid][0], a.cj[tid][1],
+                                 a.cj[tid][2]);
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
@@ -239,6 +246,13 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
      * chunk ahead, so the memory latency overlaps this chunk's arithmetic */
     const int lane = tid & 63, wv = tid >> 6;
     int nfc = 0, nslot = 0;
+    /* the next chunk's particles, from the bucket arrays or from inbox
+     * records.  Both paths load into the same registers, so neither may
+     * compute anything into a register after the other path's loads were
+     * issued (that would wait for those loads): both addresses are formed
+     * before the branch (the empty asm statements keep the compiler from
+     * sinking them into it), and the inbox path loads the species byte of
+     * the record's key word by itself rather than masking the loaded word. */
     float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
     unsigned char qs = 0;
     int qid = 0;
@@ -248,18 +262,21 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         const int cf = group_cell(e);
         nfc = cf;
         nslot = group_start(e) + (lane - (63 - __clzll(m)));
+        unsigned nrel = (unsigned)(sBase[cf] + nslot);
+        unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
+        asm volatile("" : "+v"(nrel));
+        asm volatile("" : "+v"(ir));
         if (group_kind(e) == 0) {
-            const unsigned nrel = (unsigned)(sBase[cf] + nslot);
             qox = slot(P0, nrel); qoy = slot(P1, nrel); qoz = slot(P2, nrel);
             qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
             qs = slot(SP, nrel);
             if (DIFF) qid = slot(ID, nrel);
         } else {
-            const unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
-            const float4 A = slot(IB, ir), B = slot(IB, ir + 1u);
+            const float4 A = slot(IB, ir);
+            const float2 B = *(const float2 *)&slot(IB, ir + 1u);
             qox = A.x; qoy = A.y; qoz = A.z; qpu = A.w; qpv = B.x; qpw = B.y;
-            qs = (unsigned char)(__float_as_uint(B.z) & 255u);
-            if (DIFF) qid = __float_as_int(B.w);
+            qs = *((const unsigned char *)&slot(IB, ir + 1u) + 8);   /* key & 255 */
+            if (DIFF) qid = *((const int *)&slot(IB, ir + 1u) + 3);
         }
     };
     const int ghi = __builtin_amdgcn_readfirstlane(
@@ -310,7 +327,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
             const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
 
-            const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, a.qm[s] * a.tau * 0.5f, a.tdh);
+            const float4 spc = sSpec[s];
+            const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, spc.x, a.tdh);
             pu = B.p1[0];
             pv = B.p1[1];
             pw = B.p1[2];
@@ -318,7 +336,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             {
                 const Relay32 R = relay32(ox, oy, oz, o1x, o1y, o1z, B.du, W.rd, a.hh);
                 const int m = R.m;
-                const float *cj = a.cj[s];
+                const float cj[3] = {spc.y, spc.z, spc.w};
                 const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
                 vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
                 if (m)

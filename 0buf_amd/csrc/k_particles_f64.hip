@@ -121,6 +121,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     __shared__ int sQ;
     __shared__ unsigned sFlags;
     __shared__ unsigned long long sDrop, sDup;
+    __shared__ double2 sMQ[PIC_MAX_SLOTS];   /* (m, q_m) per species slot */
     extern __shared__ unsigned sGrp[];    /* line groups of the tile */
     double *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
 
@@ -172,6 +173,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         sF[e] = val;
     }
     for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0.0;
+    /* weight and q/m per species slot: a kernel-argument table indexed by a
+     * per-particle value would be a vector memory load in the loop, whose
+     * wait would also wait for the particle loads in flight */
+    if (tid < PIC_MAX_SLOTS) sMQ[tid] = make_double2(a.sp.m[tid], a.sp.q_m[tid]);
     if (tid == 0) {
         sQ = 0;
         sFlags = 0;
@@ -244,7 +249,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             pw = slot(P5, rel);
             s = slot(SP, rel);
             pid = DIFF ? slot(ID, rel) : 0;
-            const double mass = a.sp.m[s], q_m = a.sp.q_m[s];
+            const double2 mq = sMQ[s];
+            const double mass = mq.x, q_m = mq.y;
             const long r = snap ? a.d.snap_off[tile] + sOff[c] + j : 0;
             if (snap && r < a.d.snap_room) {
                 a.d.snap.p[0][r] = x;

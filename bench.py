@@ -1,26 +1,35 @@
 """Benchmark of the MI355X EM-PIC time step (one GPU).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--config 2]
 
 A step is one Plasma::Step of the beam-plasma workload: H half step, fused
 gather/Boris push/charge-conserving deposit/departures, arrivals, J fold,
 H second half, E update, periodic E, field energy.  Inputs are resident in
-HBM before the timed region.  Prints ONE JSON line:
+HBM before the timed region, which times K calls of pic_step exactly as a
+caller makes them.  The default workload is the FP32-vs-FP64 comparison:
+512x128x128 cells, 24 particles per cell per species (603,979,776
+particles), a uniform Bx0 = 0.5.  Prints ONE JSON line:
 
   value          particle pushes/s of the whole step (particles x steps / s)
-                 of the FP32 precision form, with error_vs_fp64 (its energy
-                 history against the FP64 run of the same particles) and
+                 of the FP32 precision form, with error_vs_fp64 (its
+                 field-energy history over every step run and its E field at
+                 the end, against the FP64 run of the same particles) and
                  speedup_vs_fp64
   roofline       its dominant kernel (the fused push/deposit): algorithmic
                  bytes per launch / its mean duration from HIP events recorded
-                 on the stream it runs on, against the 8 TB/s HBM peak
+                 on the stream it runs on (a few stage-timed steps after the
+                 timed region), against the 8 TB/s HBM peak
   fp64           the FP64 plain form (the parity form) on the same particles:
                  pushes/s, roofline, stage times
-  cpu_baseline   the FP64 CPU restatement (oracle/) timed on this host on a
-                 bounded slab of the same workload
-  adjoint        config 3 gradient (differentiable form): wall time of
-                 forward-with-checkpoints + recompute + adjoint sweep over a
-                 plain forward run (adjoint_forward_ratio)
+  secondary      the same pair on a second workload (default: 256x64x64
+                 cells, 64 particles per cell, 67,108,864 particles)
+  cpu_baseline   the FP64 CPU restatement (oracle/) timed on this host on an
+                 x-slab of the same workload (median step after 3 warm-ups)
+  adjoint        gradient of the differentiable form on 64x16x16 cells (200
+                 steps): wall time of forward-with-checkpoints + recompute +
+                 adjoint sweep over a plain forward run (adjoint_forward_ratio)
+  adjoint_config4  the same at 256x64x64 cells, 48 ppc, Gaussian beam, 400
+                 steps, FP64 and FP32 differentiable forms
 """
 import argparse
 import importlib
@@ -55,11 +64,13 @@ def algorithmic_bytes_push(P, ncell, width):
     return P * 2 * 6 * width + ncell * per_cell
 
 
-def cpu_baseline(pic, cfg, slab, steps=10, threads=1, all_threads=None):
-    """The FP64 CPU restatement (oracle/) on a slab of the workload: `steps`
-    full steps after one warm-up, single-threaded (about 10 s of CPU work at
-    the default 16x64x64 slab), then the same with the oracle's cell loops
-    spread over `all_threads` host threads."""
+def cpu_baseline(pic, cfg, slab, steps=7, warm=3, threads=1, all_threads=None):
+    """The FP64 CPU restatement (oracle/) on an x-slab of the workload (the
+    full y-z cross-section, `slab` planes of cells, the workload's species
+    and Bx0): `warm` untimed steps, then `steps` steps timed one by one;
+    the value is particles / median step time.  Single-threaded (the
+    reference's serial semantics), then the same with the oracle's cell
+    loops on `all_threads` host threads."""
     import oracle
     n = (slab, cfg["n"][1], cfg["n"][2])
     params, parts = pic.generate_workload(cfg, stream_mode=1, scale_n=n, nthreads=8)
@@ -68,35 +79,43 @@ def cpu_baseline(pic, cfg, slab, steps=10, threads=1, all_threads=None):
         o = oracle.Oracle(n, (params.lx, params.ly, params.lz), params.tau, threads=nthreads)
         for (sort, x, y, z, px, py, pz, m, qm) in parts:
             o.load_particles(sort, x, y, z, px, py, pz, m, qm)
+        if cfg["bx0"]:
+            o.load_field(pic.HX, np.full((n[0] + 2) * (n[1] + 2) * (n[2] + 2), cfg["bx0"]))
         P = o.particle_count()
-        o.step(1)
-        t0 = time.perf_counter()
-        for nt in range(2, 2 + steps):
+        for nt in range(1, warm + 1):
             o.step(nt)
-        dt = time.perf_counter() - t0
+        dts = []
+        for nt in range(warm + 1, warm + steps + 1):
+            t0 = time.perf_counter()
+            o.step(nt)
+            dts.append(time.perf_counter() - t0)
         o.close()
-        return P, dt
-    P, dt = timed(threads)
-    out = {"value": P * steps / dt, "unit": "particle pushes/s", "cores": threads,
+        return P, float(np.median(dts)), sum(dts)
+    P, med, tot = timed(threads)
+    out = {"value": P / med, "unit": "particle pushes/s", "cores": threads,
            "kind": "port",
            "sample": f"oracle FP64 restatement of Plasma::Step, full steps on a "
-                     f"{n[0]}x{n[1]}x{n[2]}-cell slab of the same workload "
-                     f"({P} particles), {steps} timed steps after 1 warm-up, "
-                     f"{dt:.1f} s, {threads} thread(s)"}
+                     f"{n[0]}x{n[1]}x{n[2]}-cell x-slab of the same workload "
+                     f"({P} particles{', Bx0 = %g' % cfg['bx0'] if cfg['bx0'] else ''}), "
+                     f"median of {steps} timed steps after {warm} warm-up steps "
+                     f"({med:.2f} s per step, {tot:.1f} s timed), {threads} thread(s)"}
     if all_threads and all_threads > threads:
-        P, dt = timed(all_threads)
-        out["all_cores"] = {"value": P * steps / dt, "cores": all_threads,
-                            "sample": f"the same slab and steps, cell loops over "
-                                      f"{all_threads} threads, {dt:.1f} s"}
+        P, med, tot = timed(all_threads)
+        out["all_cores"] = {"value": P / med, "cores": all_threads,
+                            "sample": f"the same slab and steps, cell loops on {all_threads} "
+                                      f"threads ({med:.3f} s per step)"}
     return out
 
 
-def run_form(pic, params, cfg, parts, bx0, precision, steps, warmup, seed=12345):
+def run_form(pic, params, cfg, parts, bx0, precision, steps, warmup, stage_steps=5,
+             keep_e=False, seed=12345):
     """Load the workload into a context of the given precision (drawn on the
-    GPU by the device initializer, or the host-generated `parts`), run warmup
-    untimed steps then `steps` timed steps (torch.cuda.synchronize on both
-    sides); per-stage mean launch times from HIP events on the context's
-    stream."""
+    GPU by the device initializer, or the host-generated `parts`); `warmup`
+    untimed steps; `steps` timed steps of pic_step as a caller runs it (stage
+    timing off, so the arrival runs on its second stream beside the field
+    stages), bracketed by torch.cuda.synchronize; then `stage_steps` more
+    steps with stage timing on (HIP events on the context's stream around
+    each stage) for the per-stage and push-kernel times."""
     import torch
     params.precision = precision
     g = pic.Pic(params)
@@ -112,15 +131,20 @@ def run_form(pic, params, cfg, parts, bx0, precision, steps, warmup, seed=12345)
     energies = []
     for nt in range(1, warmup + 1):
         energies.append(g.step(nt))
-    g.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for nt in range(warmup + 1, warmup + steps + 1):
         energies.append(g.step(nt))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    g.profile(True)
+    for nt in range(warmup + steps + 1, warmup + steps + stage_steps + 1):
+        energies.append(g.step(nt))
+    torch.cuda.synchronize()
     res = {"P": P, "ncell": g.ncell, "dt": dt, "prof": g.profile_read(), "status": g.status(),
            "P_end": g.particle_count(), "energies": np.array(energies)}
+    if keep_e:
+        res["E"] = [g.get_field(k) for k in (pic.EX, pic.EY, pic.EZ)]
     g.close()
     return res
 
@@ -186,63 +210,102 @@ def adjoint_ratio(pic):
             "grad_norm": float(theta.grad.norm())}
 
 
-def adjoint_device(pic, idx, precision, first=150, last=200, ckpt_every=20, seed=12345):
-    """Gradient cost of the differentiable form at full size with the particles
-    drawn on the GPU: a plain forward of the workload's steps, then
-    forward-with-checkpoints + the reverse sweep of the loss sum_{t=first..last}
-    W_t (steps after `last` have a zero adjoint and are not recomputed), both
-    from the same s_0 on one context whose gradient memory is allocated."""
-    import torch
+def adjoint_device(pic, idx, precision):
+    """Gradient cost of the differentiable form at full size, particles drawn
+    on the GPU (0buf_amd.autograd.device_gradient)"""
+    ag = importlib.import_module("0buf_amd.autograd")
+    r = ag.device_gradient(idx, precision)
     cfg = pic.beam_plasma_config(idx)
-    n = cfg["n"]
-    T = cfg["steps"]
-    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"], precision=precision,
-                             differentiable=1, ghost_mode=1,
-                             departure_cap=cfg.get("departure_cap", 50))
+    r["workload"] = (f"{workload_name(cfg, idx, tuple(s.ppc for s in cfg['species']))}, "
+                     f"{cfg['steps']} steps, differentiable form "
+                     f"{'FP32 (FP64 gradient accumulation)' if precision else 'FP64'}, "
+                     f"ghost_mode 1, checkpoint every {r['ckpt_every']}, loss = sum of field "
+                     f"energy over steps {r['first']}..{r['last']}")
+    return r
 
-    def ctx():
-        g = pic.Pic(params)
-        g.generate_device(cfg["species"], seed=seed)
-        if cfg["bx0"]:
-            g.load_field(pic.HX, np.full(g.G, cfg["bx0"]))
-        torch.cuda.synchronize()
-        return g
-    g = ctx()
-    P = g.particle_count()
-    w = np.zeros(T)
-    w[first - 1:last] = 1.0
-    # one untimed gradient allocates the checkpoint and record memory; the
-    # context is then rewound to s_0 for each timed leg
-    g.forward_record(T, ckpt_every)
-    g.adjoint(weights=w)
-    g.rewind()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    W = g.run(1, T)
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    g.rewind()
-    t0 = time.perf_counter()
-    Wr = g.forward_record(T, ckpt_every)
-    t1 = time.perf_counter()
-    gr = g.adjoint(weights=w)
-    t2 = time.perf_counter()
-    dev = g.adjoint_timing()
-    g.close()
-    pg = np.sqrt(sum(float(np.dot(gr[k], gr[k])) for k in ("x", "y", "z", "px", "py", "pz")))
-    return {"workload": f"{workload_name(cfg, idx, tuple(s.ppc for s in cfg['species']))}, "
-                        f"{T} steps, differentiable form "
-                        f"{'FP32 (FP64 gradient accumulation)' if precision else 'FP64'}, "
-                        f"ghost_mode 1, checkpoint every {ckpt_every}, loss = sum of field "
-                        f"energy over steps {first}..{last}",
-            "particles": P, "forward_s": t_fwd, "forward_record_s": t1 - t0,
-            "backward_s": t2 - t1, "gradient_s": t2 - t0, "ratio": (t2 - t0) / t_fwd,
-            "reversed_steps": last,
-            "backward_per_reversed_step_over_forward_step": (t2 - t1) / last / (t_fwd / T),
-            "device_ms": {"forward_with_checkpoints": dev[0], "recompute": dev[1],
-                          "adjoint_steps": dev[2]},
-            "record_matches_forward": bool(np.array_equal(W, Wr)),
-            "particle_grad_norm": pg}
+
+def measured_traffic(kernel, alg):
+    """HBM bytes per launch of a push kernel from a committed counter summary
+    (profiles/*traffic*.json: FETCH_SIZE + WRITE_SIZE, calibrated on a copy
+    kernel) measured on this workload (same kernel, same algorithmic bytes),
+    else None"""
+    import glob
+    for tf in sorted(glob.glob(os.path.join(ROOT, "profiles", "*traffic*.json"))):
+        try:
+            rec = json.load(open(tf))
+        except (OSError, ValueError):
+            continue
+        if rec.get("kernel") == kernel and rec.get("algorithmic_bytes") == alg:
+            return rec.get("bytes_per_launch")
+    return None
+
+
+def form_result(res, width, kernel, kernel_desc, steps):
+    P, ncell = res["P"], res["ncell"]
+    push_ms, alg, achieved = push_roofline(res, width)
+    prof = res["prof"]
+    nprof = max(prof["push_deposit"][1], 1)
+    yee_ms = sum(prof[k][0] for k in ("h_half", "h2", "e", "e_periodic", "fold_j")) / nprof
+    return {"value": P * steps / res["dt"], "unit": "particle pushes/s",
+            "ms_per_step": res["dt"] * 1e3 / steps,
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(kernel, alg),
+                         "kernel": f"{kernel} ({kernel_desc})",
+                         "algorithmic_bytes_per_launch": alg, "mean_launch_ms": push_ms,
+                         "launches_timed": nprof},
+            "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
+            "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,
+            "yee_roofline": yee_roofline(ncell, yee_ms, width),
+            "particles_end": res["P_end"], "status": res["status"]}
+
+
+K64 = ("k_push_deposit", "gather+Boris+VB deposit+departures, FP64")
+K32 = ("k_push_deposit32", "gather+Boris+VB deposit+departures, FP32 offsets, int64 current")
+
+
+def compare_forms(r64, r32):
+    """FP32 form against the FP64 form from the same particles: field-energy
+    history over every step run (warm-up, timed and stage-timed), and the
+    electric field at the end, max-norm over all nodes relative to the FP64
+    field's max-norm"""
+    rel = np.abs(r32["energies"] - r64["energies"]) / np.abs(r64["energies"])
+    out = {"energy_history_max_rel": float(rel.max()), "steps_compared": int(len(rel))}
+    if "E" in r64 and "E" in r32:
+        out["E_maxnorm_rel_at_end"] = {
+            nm: float(np.abs(a - b).max() / max(np.abs(a).max(), 1e-300))
+            for nm, a, b in zip(("Ex", "Ey", "Ez"), r64["E"], r32["E"])}
+    return out
+
+
+def run_config(pic, idx, args, fp32=True, keep_e=False):
+    cfg = pic.beam_plasma_config(idx)
+    ppc = tuple(s.ppc for s in cfg["species"])
+    log(f"workload {workload_name(cfg, idx, ppc)}")
+    n = cfg["n"]
+    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"],
+                             departure_cap=cfg.get("departure_cap", 50))
+    parts = None
+    if args.host_init:
+        t0 = time.perf_counter()
+        params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
+        log(f"generated {sum(len(p[1]) for p in parts)} particles on the host in "
+            f"{time.perf_counter() - t0:.1f} s")
+    r64 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP64, args.steps, args.warmup,
+                   keep_e=keep_e)
+    log(f"config {idx} FP64: {r64['P']} particles, {args.steps} steps in {r64['dt']:.3f} s")
+    out = {"workload": workload_name(cfg, idx, ppc), "particles": r64["P"],
+           "cells": r64["ncell"], "bx0": cfg["bx0"],
+           "fp64": form_result(r64, 8, *K64, args.steps)}
+    if fp32:
+        r32 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP32, args.steps, args.warmup,
+                       keep_e=keep_e)
+        log(f"config {idx} FP32: {args.steps} steps in {r32['dt']:.3f} s")
+        f32 = form_result(r32, 4, *K32, args.steps)
+        f32["speedup_vs_fp64"] = r64["dt"] / r32["dt"]
+        f32["error_vs_fp64"] = compare_forms(r64, r32)
+        out["fp32"] = f32
+    return cfg, ppc, out
 
 
 def main():
@@ -250,16 +313,22 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2,
+                    help="headline workload (default 2: 512x128x128 cells, 604M particles, "
+                         "Bx0 = 0.5, the FP32-vs-FP64 comparison)")
+    ap.add_argument("--secondary-config", type=int, default=1,
+                    help="second workload timed in both forms (default 1; -1 for none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
-    ap.add_argument("--no-adjoint", action="store_true")
-    ap.add_argument("--adjoint-config4", action="store_true",
-                    help="also time the config-4 gradient (FP64 and FP32 differentiable "
-                         "forms, particles drawn on the GPU)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip both gradient legs (config 3 and config 4)")
+    ap.add_argument("--no-adjoint-config4", action="store_true",
+                    help="skip the config-4 gradient (FP64 and FP32 differentiable forms)")
     ap.add_argument("--adjoint-config4-only", action="store_true",
                     help="only the config-4 gradient timing (for profiling its kernels)")
-    ap.add_argument("--cpu-slab", type=int, default=16)
+    ap.add_argument("--cpu-slab", type=int, default=0,
+                    help="x-planes of the CPU baseline's slab (0: about 2.4M particles, at "
+                         "least 3 planes)")
     ap.add_argument("--host-init", action="store_true",
                     help="generate the particles on the host and upload them instead of "
                          "drawing them on the GPU (same particles, bit for bit)")
@@ -297,93 +366,39 @@ def main():
             out[nm] = adjoint_device(pic, 4, prec)
         print(json.dumps({"adjoint_config4": out}))
         return
-    cfg = pic.beam_plasma_config(args.config)
-    ppc = tuple(s.ppc for s in cfg["species"])
-    log(f"workload {workload_name(cfg, args.config, ppc)}")
-    parts = None
-    n = cfg["n"]
-    params = pic.make_params(n, tuple(k * 0.1 for k in n), cfg["tau"],
-                             departure_cap=cfg.get("departure_cap", 50))
-    if args.host_init:
-        t0 = time.perf_counter()
-        params, parts = pic.generate_workload(cfg, stream_mode=1, nthreads=16)
-        log(f"generated {sum(len(p[1]) for p in parts)} particles on the host in "
-            f"{time.perf_counter() - t0:.1f} s")
 
-    r64 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP64, args.steps, args.warmup)
-    log(f"FP64: {r64['P']} particles, {args.steps} steps in {r64['dt']:.3f} s")
-    r32 = None
-    if not args.no_fp32:
-        r32 = run_form(pic, params, cfg, parts, cfg["bx0"], pic.FP32, args.steps, args.warmup)
-        log(f"FP32: {args.steps} steps in {r32['dt']:.3f} s")
-    del parts
-
-    P, ncell = r64["P"], r64["ncell"]
-
-    def measured_traffic(name, alg):
-        """HBM bytes per launch of a push kernel from the committed counter
-        summary (FETCH_SIZE + WRITE_SIZE, calibrated), or None when that
-        summary was measured on another workload (other algorithmic bytes)"""
-        tf = os.path.join(ROOT, "profiles", name)
-        try:
-            rec = json.load(open(tf))
-        except (OSError, ValueError):
-            return None
-        if rec.get("algorithmic_bytes") != alg:
-            return None
-        return rec.get("bytes_per_launch")
-
-    def form_result(res, width, kernel, traffic_file):
-        push_ms, alg, achieved = push_roofline(res, width)
-        prof = res["prof"]
-        yee_ms = sum(prof[k][0] for k in ("h_half", "h2", "e", "e_periodic",
-                                          "fold_j")) / args.steps
-        return {"value": P * args.steps / res["dt"], "unit": "particle pushes/s",
-                "ms_per_step": res["dt"] * 1e3 / args.steps,
-                "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                             "traffic": measured_traffic(traffic_file, alg), "kernel": kernel,
-                             "algorithmic_bytes_per_launch": alg, "mean_launch_ms": push_ms},
-                "stage_ms": {k: round(v[0] / max(v[1], 1), 4) for k, v in prof.items()},
-                "yee_cell_updates_per_s": ncell / (yee_ms * 1e-3) if yee_ms > 0 else None,
-                "yee_roofline": yee_roofline(ncell, yee_ms, width),
-                "particles_end": res["P_end"], "status": res["status"]}
-
-    fp64 = form_result(r64, 8, "k_push_deposit (gather+Boris+VB deposit+departures, FP64)",
-                       "push_deposit_f64_traffic.json")
-    fp32 = None
-    if r32 is not None:
-        fp32 = form_result(r32, 4, "k_push_deposit32 (gather+Boris+VB deposit+departures, "
-                                   "FP32 offsets, int64 current)", "push_deposit_f32_traffic.json")
-        rel = np.abs(r32["energies"] - r64["energies"]) / np.abs(r64["energies"])
-        fp32["speedup_vs_fp64"] = r64["dt"] / r32["dt"]
-        fp32["error_vs_fp64"] = {"energy_history_max_rel": float(rel.max()),
-                                 "steps_compared": int(len(rel))}
+    cfg, ppc, head_cfg = run_config(pic, args.config, args, fp32=not args.no_fp32, keep_e=True)
+    second = None
+    if args.secondary_config >= 0 and args.secondary_config != args.config:
+        _, _, second = run_config(pic, args.secondary_config, args, fp32=not args.no_fp32)
 
     adj = None
+    adj4 = None
     if not args.no_adjoint:
         log("timing the differentiable form (config 3 gradient)")
         adj = adjoint_ratio(pic)
-        log(f"adjoint/forward ratio {adj['ratio']:.2f}")
-
-    adj4 = None
-    if args.adjoint_config4:
-        adj4 = {}
-        for prec, nm in ((pic.FP64, "fp64"), (pic.FP32, "fp32")):
-            log(f"timing the config 4 gradient, {nm}")
-            adj4[nm] = adjoint_device(pic, 4, prec)
-            log(f"config 4 {nm}: gradient/forward {adj4[nm]['ratio']:.2f}")
+        log(f"config 3 adjoint/forward ratio {adj['ratio']:.2f}")
+        if not args.no_adjoint_config4:
+            adj4 = {}
+            for prec, nm in ((pic.FP64, "fp64"), (pic.FP32, "fp32")):
+                log(f"timing the config 4 gradient, {nm}")
+                adj4[nm] = adjoint_device(pic, 4, prec)
+                log(f"config 4 {nm}: gradient/forward {adj4[nm]['ratio']:.2f}")
 
     cpu = None
     if not args.no_cpu_baseline:
-        log("timing the CPU baseline (oracle) on a slab of the workload")
-        cpu = cpu_baseline(pic, cfg, args.cpu_slab,
-                           all_threads=min(16, len(os.sched_getaffinity(0))))
+        # at least 3 planes: the 27-neighbour arrival scan needs 3 distinct
+        # x-neighbours (as the step itself needs nx >= 3)
+        slab = args.cpu_slab or max(3, round(2.4e6 / (cfg["n"][1] * cfg["n"][2] * sum(ppc))))
+        log(f"timing the CPU baseline (oracle) on a {slab}-plane slab of the workload")
+        cpu = cpu_baseline(pic, cfg, slab, all_threads=min(16, len(os.sched_getaffinity(0))))
         log(f"cpu baseline {cpu['value']:.3e} pushes/s")
 
     # the headline is the FP32 precision form with its error against the FP64
-    # form stated; the FP64 plain form (the parity form) stands beside it
-    head, form = (fp32, "fp32") if fp32 is not None else (fp64, "fp64")
+    # form of the same particles stated; the FP64 plain form (the parity form)
+    # stands beside it
+    fp64 = head_cfg["fp64"]
+    head, form = (head_cfg["fp32"], "fp32") if "fp32" in head_cfg else (fp64, "fp64")
     out = {
         "metric": "particle pushes/s/GPU (FP32 vs FP64 at stated error) + adjoint/forward ratio",
         "value": head["value"],
@@ -398,8 +413,8 @@ def main():
         "dtype": "f32" if form == "fp32" else "f64",
         "data": "synthetic seeded beam-plasma (Wichmann-Hill per-cell streams, seed 12345, "
                 "drawn on the GPU by the device initializer)",
-        "config": {"workload": workload_name(cfg, args.config, ppc), "particles": P,
-                   "cells": ncell,
+        "config": {"workload": head_cfg["workload"], "particles": head_cfg["particles"],
+                   "cells": head_cfg["cells"], "bx0": head_cfg["bx0"],
                    "form": ("FP32 precision form: FP32 push, gather and Yee curls on cell-local "
                             "offsets, int64 fixed-point current, FP64 energy reduction")
                    if form == "fp32" else "FP64 plain form"},
@@ -409,6 +424,7 @@ def main():
         "adjoint_forward_ratio": adj["ratio"] if adj else None,
         "cpu_baseline": cpu,
         "fp64": fp64,
+        "secondary": second,
         "adjoint": adj,
         "adjoint_config4": adj4,
         "stage_ms": head["stage_ms"],

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <memory>
 #include <thread>
 
 namespace {
@@ -365,6 +366,19 @@ inline int fly_direction(const CellGeom &c, const OParticle &p) {
     return dx * 9 + dy * 3 + dz;
 }
 
+/* contiguous ranges [t*n/T, (t+1)*n/T) of 0..n-1 on T threads */
+template <class F>
+void parallel_ranges(int nthreads, long n, F fn) {
+    if (nthreads <= 1) {
+        fn(0, 0L, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nthreads; t++)
+        th.emplace_back([&, t]() { fn(t, n * t / nthreads, n * (t + 1) / nthreads); });
+    for (auto &x : th) x.join();
+}
+
 template <class F>
 void parallel_cells(int nthreads, long ncells, F fn) {
     if (nthreads <= 1 || ncells < 64) {
@@ -505,7 +519,9 @@ void orc_push(orc_plasma *P) {
  * GPU_WriteAllCurrents (kernels.cu:83-112): J[|n|] += tile. */
 void orc_deposit(orc_plasma *P) {
     long nb = (long)P->Nx * P->Ny * P->Nz;
-    std::vector<double> tiles((size_t)nb * 3 * 125, 0.0);
+    /* every tile of a cell with particles is written whole below; the
+     * others are never read (no zero fill of nb x 375 doubles) */
+    std::unique_ptr<double[]> tiles(new double[(size_t)nb * 3 * 125]);
     std::vector<char> has(nb, 0);
     parallel_cells(P->nthreads, nb, [&](long b) {
         int i = (int)(b / ((long)P->Ny * P->Nz));
@@ -541,22 +557,54 @@ void orc_deposit(orc_plasma *P) {
             }
         has[b] = 1;
     });
+    /* tiles into J in cell order.  One thread: J[n] += tile, cell after
+     * cell (the reference's serial order).  T threads: each adds the tiles
+     * of a contiguous range of cells, in cell order, into a private copy of
+     * J starting at zero, and the copies are summed in thread order -- a
+     * fixed summation order for a given T (J is zero before the deposit),
+     * so the result is deterministic but may differ from one thread's in
+     * the last bits.  The parity tests run one thread. */
+    const long G = P->G;
+    auto scatter = [&](double *const J[3], long b0, long b1) {
+        for (long b = b0; b < b1; b++) {
+            if (!has[b]) continue;
+            int i = (int)(b / ((long)P->Ny * P->Nz));
+            int l = (int)((b / P->Nz) % P->Ny);
+            int k = (int)(b % P->Nz);
+            const double *tile = &tiles[(size_t)b * 375];
+            for (int a = 0; a < 5; a++)
+                for (int bb = 0; bb < 5; bb++)
+                    for (int cc = 0; cc < 5; cc++) {
+                        long n = P->gcn(i + a - 2, l + bb - 2, k + cc - 2);
+                        if (n < 0) n = -n;
+                        int e = (a * 5 + bb) * 5 + cc;
+                        for (int comp = 0; comp < 3; comp++) J[comp][n] += tile[comp * 125 + e];
+                    }
+        }
+    };
     double *J[3] = {P->f[ORC_JX].data(), P->f[ORC_JY].data(), P->f[ORC_JZ].data()};
-    for (long b = 0; b < nb; b++) {
-        if (!has[b]) continue;
-        int i = (int)(b / ((long)P->Ny * P->Nz));
-        int l = (int)((b / P->Nz) % P->Ny);
-        int k = (int)(b % P->Nz);
-        const double *tile = &tiles[(size_t)b * 375];
-        for (int a = 0; a < 5; a++)
-            for (int bb = 0; bb < 5; bb++)
-                for (int cc = 0; cc < 5; cc++) {
-                    long n = P->gcn(i + a - 2, l + bb - 2, k + cc - 2);
-                    if (n < 0) n = -n;
-                    int e = (a * 5 + bb) * 5 + cc;
-                    for (int comp = 0; comp < 3; comp++) J[comp][n] += tile[comp * 125 + e];
-                }
+    const int T = P->nthreads;
+    if (T <= 1 || nb < 64) {
+        scatter(J, 0, nb);
+        return;
     }
+    std::vector<std::vector<double>> priv((size_t)T * 3);
+    parallel_ranges(T, nb, [&](int t, long b0, long b1) {
+        double *Jt[3];
+        for (int comp = 0; comp < 3; comp++) {
+            priv[(size_t)t * 3 + comp].assign(G, 0.0);
+            Jt[comp] = priv[(size_t)t * 3 + comp].data();
+        }
+        scatter(Jt, b0, b1);
+    });
+    parallel_ranges(T, G, [&](int, long n0, long n1) {
+        for (int comp = 0; comp < 3; comp++)
+            for (long n = n0; n < n1; n++) {
+                double d = J[comp][n];
+                for (int t = 0; t < T; t++) d += priv[(size_t)t * 3 + comp][n];
+                J[comp][n] = d;
+            }
+    });
 }
 
 /* gpu_plasma.h:2035-2046 reorder_particles:
@@ -568,11 +616,12 @@ void orc_deposit(orc_plasma *P) {
  *   GPU_MakeDepartureLists_rm (kernels.cu:219-242): removes direction != 13
  *     by swap-with-last (cell.h:391-432). */
 int orc_migrate(orc_plasma *P) {
-    int status = ORC_OK;
-    long G = P->G;
-    for (long n = 0; n < G; n++) {
+    const long G = P->G;
+    const int T = P->nthreads;
+    std::vector<char> too_many(G, 0);
+    parallel_cells(T, G, [&](long n) {
         std::vector<OParticle> &v = P->cells[n];
-        if (v.empty()) continue;
+        if (v.empty()) return;
         int i = (int)(n / ((long)(P->Ny + 2) * (P->Nz + 2)));
         int l = (int)((n / (P->Nz + 2)) % (P->Ny + 2));
         int k = (int)(n % (P->Nz + 2));
@@ -582,33 +631,52 @@ int orc_migrate(orc_plasma *P) {
             p.direction = fly_direction(c, p);
             if (!in_cell(c, p.x, p.y, p.z)) {
                 if (P->departure_cap > 0 && len == P->departure_cap) {
-                    status |= ORC_TOO_MANY_PARTICLES;
+                    too_many[n] = 1;
                     break;
                 }
                 len++;
             }
         }
-    }
-    for (int nx = 0; nx < P->Nx; nx++)
-        for (int ny = 0; ny < P->Ny; ny++)
-            for (int nz = 0; nz < P->Nz; nz++) {
-                std::vector<OParticle> &dst = P->cells[P->gcn(nx, ny, nz)];
-                CellGeom c = geom(P, nx, ny, nz);
-                for (int ix = 0; ix < 3; ix++)
-                    for (int iy = 0; iy < 3; iy++)
-                        for (int iz = 0; iz < 3; iz++) {
-                            long s = P->wrap_cell(nx + ix - 1, ny + iy - 1, nz + iz - 1);
-                            std::vector<OParticle> &src = P->cells[s];
-                            for (size_t j = 0; j < src.size(); j++) {
-                                OParticle p1 = src[j];
-                                if (p1.direction != 13) {
-                                    p1.direction = 13;
-                                    if (in_cell(c, p1.x, p1.y, p1.z)) dst.push_back(p1);
-                                }
-                            }
+    });
+    int status = ORC_OK;
+    for (long n = 0; n < G; n++)
+        if (too_many[n]) status |= ORC_TOO_MANY_PARTICLES;
+    /* GPU_ArrangeFlights.  The reference appends a cell's arrivals while
+     * later cells still scan it as a source; arrivals carry direction 13 and
+     * are skipped as sources, so every cell's arrivals can be collected
+     * first (sources unchanged) and appended afterwards: the same lists in
+     * the same order for any thread count. */
+    const long nb = (long)P->Nx * P->Ny * P->Nz;
+    std::vector<std::vector<OParticle>> arrivals(nb);
+    parallel_cells(T, nb, [&](long b) {
+        int nx = (int)(b / ((long)P->Ny * P->Nz));
+        int ny = (int)((b / P->Nz) % P->Ny);
+        int nz = (int)(b % P->Nz);
+        std::vector<OParticle> &dst = arrivals[b];
+        CellGeom c = geom(P, nx, ny, nz);
+        for (int ix = 0; ix < 3; ix++)
+            for (int iy = 0; iy < 3; iy++)
+                for (int iz = 0; iz < 3; iz++) {
+                    long s = P->wrap_cell(nx + ix - 1, ny + iy - 1, nz + iz - 1);
+                    const std::vector<OParticle> &src = P->cells[s];
+                    for (size_t j = 0; j < src.size(); j++) {
+                        OParticle p1 = src[j];
+                        if (p1.direction != 13) {
+                            p1.direction = 13;
+                            if (in_cell(c, p1.x, p1.y, p1.z)) dst.push_back(p1);
                         }
-            }
-    for (long n = 0; n < G; n++) {
+                    }
+                }
+    });
+    parallel_cells(T, nb, [&](long b) {
+        int nx = (int)(b / ((long)P->Ny * P->Nz));
+        int ny = (int)((b / P->Nz) % P->Ny);
+        int nz = (int)(b % P->Nz);
+        std::vector<OParticle> &dst = P->cells[P->gcn(nx, ny, nz)];
+        dst.insert(dst.end(), arrivals[b].begin(), arrivals[b].end());
+    });
+    /* GPU_MakeDepartureLists_rm: swap-with-last removal, cell by cell */
+    parallel_cells(T, G, [&](long n) {
         std::vector<OParticle> &v = P->cells[n];
         for (long num = 0; num < (long)v.size(); num++) {
             if (v[num].direction != 13) {
@@ -618,7 +686,7 @@ int orc_migrate(orc_plasma *P) {
                 num--;
             }
         }
-    }
+    });
     return status;
 }
 

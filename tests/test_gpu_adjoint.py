@@ -153,14 +153,33 @@ def test_recompute_is_bitwise_and_restores(picmod):
         h.close()
 
 
+@pytest.mark.xfail(strict=False, reason=(
+    "unmet bar: at step 1e-5 the free central difference differs from the adjoint by up to "
+    "2.7e-3 per parameter (a_8 2.7e-3, a_1 2.4e-3, a_6 1.7e-3; normwise 1.2e-4): the forward "
+    "is piecewise smooth and a 1e-5 window crosses kinks of the discrete cell and crossing "
+    "decisions that the adjoint holds fixed"))
 def test_config3_gradient_against_finite_differences(picmod):
-    """Config 3 (ghost_mode 1): the discrete loss is piecewise smooth with
-    kinks wherever a particle crosses an interpolation node or a dual face,
-    so a central difference of step d carries an O(d) error from the kinks
-    its window crosses, and below d ~ 1e-7 the round-off of the fixed-point
-    current takes over.  Each parameter's gradient must agree within 1e-4
-    with the central difference at one of the steps 1e-5..1e-8 (the adjoint
-    is their small-step limit), and within 3e-4 normwise at step 1e-5."""
+    """Config 3 (64x16x16 cells, 32 ppc, 200 steps, K = 20, ghost_mode 1,
+    loss = field energy over steps 150..200, theta = beam drift and
+    a_1..a_8): every parameter's adjoint gradient within 1e-4 of the central
+    difference at step 1e-5, the bar as stated."""
+    pytest.importorskip("torch")
+    ag = importlib.import_module("0buf_amd.autograd")
+    r = ag.fd_check(3, (1e-5,), ghost_mode=1)
+    g, fd = r["grad"], r["fd"][1e-5]
+    rel = np.abs(g - fd) / np.abs(fd)
+    print("step 1e-5: rel " + " ".join(f"{v:.1e}" for v in rel) +
+          f"  normwise {np.linalg.norm(g - fd) / np.linalg.norm(fd):.2e}")
+    assert rel.max() <= 1e-4, rel
+
+
+def test_config3_fd_best_step_diagnostic(picmod):
+    """Diagnostic, not the bar above: the adjoint is the small-step limit of
+    the central differences of the piecewise-smooth forward, so for each
+    parameter one of the steps 1e-5..1e-8 comes within 1e-4 (the kinks in
+    the window shrink with the step until the fixed-point round-off of the
+    current takes over), and normwise the step-1e-5 differences are within
+    3e-4."""
     pytest.importorskip("torch")
     ag = importlib.import_module("0buf_amd.autograd")
     deltas = (1e-5, 1e-6, 1e-7, 1e-8)

@@ -159,7 +159,8 @@ def test_config4_shape_fp32_gradient_against_fp64(picmod):
     on a 64x16x16 grid, 200 steps, loss = field energy over steps 150..200:
     the FP32 form's gradient with respect to (drift, a_1..a_8, Bx0) against
     the FP64 adjoint on the same seeded particles, normwise over the ten
-    parameters."""
+    parameters (a few times the measured 2.6e-5).  The full-size comparison is
+    tests/test_gpu_config4.py."""
     pytest.importorskip("torch")
     ag = importlib.import_module("0buf_amd.autograd")
     cfg = dict(picmod.beam_plasma_config(4))
@@ -172,8 +173,8 @@ def test_config4_shape_fp32_gradient_against_fp64(picmod):
     print("FP32", g32)
     print(f"normwise {rel:.2e}, loss FP64 {r[picmod.FP64]['loss']:.9e} "
           f"FP32 {r[picmod.FP32]['loss']:.9e}")
-    assert abs(r[picmod.FP32]["loss"] / r[picmod.FP64]["loss"] - 1) <= 1e-3
-    assert rel <= 1e-2
+    assert abs(r[picmod.FP32]["loss"] / r[picmod.FP64]["loss"] - 1) <= 1e-5
+    assert rel <= 1e-4      # measured 2.6e-5 on MI355X
 
 
 @pytest.mark.parametrize("prec", [0, 1])

@@ -456,3 +456,34 @@ def test_load_drops_outside_particles(orc):
     d = o.load_particles(0, x, np.full(4, 0.05), np.full(4, 0.05), np.zeros(4), np.zeros(4),
                          np.zeros(4), 0.1, 1.0 / 1836)
     assert d == 2 and o.particle_count() == 2
+
+
+def test_threaded_oracle_matches_one_thread(orc, picmod):
+    """The oracle's cell loops on several host threads (the CPU baseline's
+    all-cores leg): push and migration give the one-thread result bit for bit
+    (arrivals are collected per cell from unchanged sources, then appended
+    in the same order); the J scatter sums per-thread partial grids in a
+    fixed order, so J may differ from one thread's in the last bits only."""
+    params, parts = picmod.generate_workload(0)
+    n = (params.nx, params.ny, params.nz)
+    L = (params.lx, params.ly, params.lz)
+    runs = []
+    for threads in (1, 4):
+        o = orc.Oracle(n, L, params.tau, threads=1)
+        load_workload(o, parts)
+        o.step(1)
+        # the second step's stages one by one, from the same state in both
+        o.lib.orc_set_threads(o.h, threads)
+        o.stage("h_half")
+        o.stage("push")
+        o.stage("deposit")
+        J = [o.get_field(6 + c) for c in range(3)]
+        assert o.stage("migrate") == 0
+        runs.append((o.particles(), o.cell_counts(), J))
+        o.close()
+    (p1, c1, J1), (p4, c4, J4) = runs
+    assert np.array_equal(c1, c4)
+    for k in ("x", "y", "z", "px", "py", "pz", "sort", "cell"):
+        assert np.array_equal(p1[k], p4[k]), k
+    for a, b in zip(J1, J4):
+        assert np.abs(a - b).max() <= 1e-14 * np.abs(a).max()
