@@ -113,7 +113,7 @@ EXPORTS = [
     "pic_save_state_dump", "pic_load_state_dump", "pic_control_point",
     # differentiable form
     "pic_particle_id_count", "pic_get_particle_ids", "pic_forward_record", "pic_adjoint",
-    "pic_adjoint_timing", "pic_rewind",
+    "pic_adjoint_timing", "pic_rewind", "pic_tape", "pic_tape_get", "pic_tape_set",
     # seeded inputs
     "pic_rng_seed", "pic_rng_uniform", "pic_rng_normal", "pic_species_count",
     "pic_generate_species", "pic_generate_device",
@@ -193,6 +193,9 @@ def load_library(path=LIB_PATH):
         "pic_adjoint": (C.c_int, [vp, dp, C.POINTER(PicAdjointIO), C.POINTER(PicAdjointIO)]),
         "pic_adjoint_timing": (C.c_int, [vp, dp]),
         "pic_rewind": (C.c_int, [vp]),
+        "pic_tape": (C.c_int, [vp, C.c_int, C.c_int]),
+        "pic_tape_get": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip, C.c_long, C.c_int]),
+        "pic_tape_set": (C.c_int, [vp, C.POINTER(C.c_ulonglong), ip, C.c_long, C.c_int]),
         "pic_rng_seed": (None, [C.POINTER(PicRng), C.c_int, C.c_int, C.c_int]),
         "pic_rng_uniform": (C.c_double, [C.POINTER(PicRng)]),
         "pic_rng_normal": (C.c_double, [C.POINTER(PicRng), C.c_double, C.c_double]),
@@ -493,6 +496,31 @@ class Pic:
         a = np.zeros(3)
         _check(self.lib.pic_adjoint_timing(self._ctx, _dptr(a)), "pic_adjoint_timing")
         return tuple(float(v) for v in a)
+
+    TAPE_OFF, TAPE_RECORD, TAPE_REPLAY = 0, 1, 2
+
+    def tape(self, mode, nsteps=1):
+        """decision tape (FP64 differentiable form): record the decisions of
+        the next nsteps steps, replay a tape set with tape_set, or stop"""
+        _check(self.lib.pic_tape(self._ctx, mode, nsteps), "pic_tape")
+
+    def tape_get(self, nsteps):
+        """the recorded tape: words (nsteps, ids) uint64 and the load cells (ids,)"""
+        nids = self.id_count()
+        words = np.empty((nsteps, nids), dtype=np.uint64)
+        cells = np.empty(nids, dtype=np.int32)
+        _check(self.lib.pic_tape_get(self._ctx, words.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                     _iptr(cells), nids, nsteps), "pic_tape_get")
+        return words, cells
+
+    def tape_set(self, words, cells):
+        """impose a recorded tape (then tape(TAPE_REPLAY, nsteps) before the
+        first step)"""
+        words = np.ascontiguousarray(words, dtype=np.uint64)
+        cells = np.ascontiguousarray(cells, dtype=np.int32)
+        nsteps, nids = words.shape
+        _check(self.lib.pic_tape_set(self._ctx, words.ctypes.data_as(C.POINTER(C.c_ulonglong)),
+                                     _iptr(cells), nids, nsteps), "pic_tape_set")
 
     def rewind(self):
         """Back to s_0 of the last forward_record."""

@@ -223,7 +223,8 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         Hf[1] = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
         Hf[2] = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
         const Boris32 B = boris32(pu, pv, pw, Ef[0], Ef[1], Ef[2], Hf[0], Hf[1], Hf[2], tau1, a.tdh);
-        const float o1[3] = {ox + B.du[0], oy + B.du[1], oz + B.du[2]};
+        const float o1[3] = {end_offset(ox, B.du[0]), end_offset(oy, B.du[1]),
+                             end_offset(oz, B.du[2])};
         const Relay32 R = relay32(ox, oy, oz, o1[0], o1[1], o1[2], B.du, W.rd, a.hh);
         const int m = R.m;
 

@@ -27,7 +27,7 @@
  * lanes hold particles of one cell) by JCOPIES; the copies are summed before
  * the global flush (integer sums: the result does not depend on it) */
 #ifndef PIC_JCOPIES32
-#define PIC_JCOPIES32 1
+#define PIC_JCOPIES32 2
 #endif
 
 namespace {
@@ -69,13 +69,14 @@ __device__ __forceinline__ void add64(long long *p, long long v) {
 
 
 /* One straight segment a -> b inside dual cell rd (relative to the owner
- * cell; the dual cell spans centres rd-1/2 .. rd+1/2): the 12 VB node
- * contributions of pqr (cell.h:1511-1693) in cell units,
+ * cell; the dual cell spans centres rd-1/2 .. rd+1/2), n = its first node
+ * in the J box: the 12 VB node contributions of pqr (cell.h:1511-1693) in
+ * cell units,
  *   Jx = (m hx/tau) du (Wy Wz +- dv dw/12)  on (x dual, y/z centres), etc. */
 template <int BY, int BZ, int NB>
-__device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, int rdx, int rdy,
-                                       int rdz, float ax, float ay, float az, float bx, float by,
-                                       float bz, const float *cj) {
+__device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, int rdz, float ax,
+                                       float ay, float az, float bx, float by, float bz,
+                                       const float *cj) {
     const float dx = bx - ax, dy = by - ay, dz = bz - az;
     const float mx = 0.5f * (ax + bx) - ((float)rdx - 0.5f);
     const float my = 0.5f * (ay + by) - ((float)rdy - 0.5f);
@@ -84,7 +85,6 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int ca, int cb, int cc, in
     const float syz = dy * dz * (1.0f / 12.0f), sxz = dx * dz * (1.0f / 12.0f),
                 sxy = dx * dy * (1.0f / 12.0f);
     const double qx = (double)(cj[0] * dx), qy = (double)(cj[1] * dy), qz = (double)(cj[2] * dz);
-    const int n = ((ca + rdx) * BY + (cb + rdy)) * BZ + (cc + rdz);
     long long *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
     add64(&Jx[n], fixmul(qx, ly * lz + syz));
     add64(&Jx[n + 1], fixmul(qx, ly * mz - syz));
@@ -256,6 +256,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
     unsigned char qs = 0;
     int qid = 0;
+    bool qin = false;     /* the particle comes from an inbox record */
     auto fetch = [&](int g0, unsigned long long starts) {
         const unsigned long long m = starts & (~0ull >> (63 - lane));
         const unsigned e = sGrp[g0 + __popcll(m) - 1];
@@ -266,7 +267,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
         asm volatile("" : "+v"(nrel));
         asm volatile("" : "+v"(ir));
-        if (group_kind(e) == 0) {
+        qin = group_kind(e) != 0;
+        if (!qin) {
             qox = slot(P0, nrel); qoy = slot(P1, nrel); qoz = slot(P2, nrel);
             qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
             qs = slot(SP, nrel);
@@ -282,19 +284,21 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     const int ghi = __builtin_amdgcn_readfirstlane(
         sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
     int ng0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
-    unsigned long long nstarts = 0;
+    unsigned long long nstarts = 0, ncstarts = 0;
     int nlanes = 0;
-    int gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes);
+    int gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes, ncstarts);
     if (lane < nlanes) fetch(ng0, nstarts);
     int carry_cell = -1, carry_cnt = 0;
     while (nlanes > 0) {
         const bool valid = lane < nlanes;
+        const unsigned long long runs = ncstarts;
         const int c = nfc, j = nslot;
         const float ox0 = qox, oy0 = qoy, oz0 = qoz, pu0 = qpu, pv0 = qpv, pw0 = qpw;
         const unsigned char s = qs;
         const int pid = DIFF ? qid : 0;
+        const bool fromin = qin;
         ng0 = gnext;
-        gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes);
+        gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes, ncstarts);
         if (lane < nlanes) fetch(ng0, nstarts);
         bool stays = false;
         float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
@@ -320,28 +324,36 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             const float cx0 = W.c0[0], cx1 = 1.0f - cx0, cy0 = W.c0[1], cy1 = 1.0f - cy0,
                         cz0 = W.c0[2], cz1 = 1.0f - cz0;
             const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
-            const float Ex = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
-            const float Ey = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
-            const float Ez = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
-            const float Hx = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
-            const float Hy = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
-            const float Hz = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
+            /* node of the owner cell in the boxes (24-bit multiplies: full
+             * rate, where a 32-bit one is a quarter) and the dual offsets */
+            const int n0 = (int)__umul24(__umul24((unsigned)ca, BY) + (unsigned)cb, BZ) + cc;
+            const int dX = rdx ? BY * BZ : 0, dY = rdy ? BZ : 0, dZ = rdz;
+            const float Ex = tri_at<BY, BZ>(sF + 0 * NB + n0 + dY + dZ, nx0, ox, cy0, cy1, cz0, cz1);
+            const float Ey = tri_at<BY, BZ>(sF + 1 * NB + n0 + dX + dZ, cx0, cx1, ny0, oy, cz0, cz1);
+            const float Ez = tri_at<BY, BZ>(sF + 2 * NB + n0 + dX + dY, cx0, cx1, cy0, cy1, nz0, oz);
+            const float Hx = tri_at<BY, BZ>(sF + 3 * NB + n0 + dX, cx0, cx1, ny0, oy, nz0, oz);
+            const float Hy = tri_at<BY, BZ>(sF + 4 * NB + n0 + dY, nx0, ox, cy0, cy1, nz0, oz);
+            const float Hz = tri_at<BY, BZ>(sF + 5 * NB + n0 + dZ, nx0, ox, ny0, oy, cz0, cz1);
 
             const float4 spc = sSpec[s];
             const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, spc.x, a.tdh);
             pu = B.p1[0];
             pv = B.p1[1];
             pw = B.p1[2];
-            const float o1x = ox + B.du[0], o1y = oy + B.du[1], o1z = oz + B.du[2];
+            const float o1x = end_offset(ox, B.du[0]), o1y = end_offset(oy, B.du[1]),
+                        o1z = end_offset(oz, B.du[2]);
             {
                 const Relay32 R = relay32(ox, oy, oz, o1x, o1y, o1z, B.du, W.rd, a.hh);
                 const int m = R.m;
                 const float cj[3] = {spc.y, spc.z, spc.w};
                 const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
-                vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1x, e1y, e1z, cj);
+                vb_seg<BY, BZ, NB>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
+                                   e1z, cj);
                 if (m)
-                    vb_seg<BY, BZ, NB>(myJ, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
-                                       R.x2[2], o1x, o1y, o1z, cj);
+                    vb_seg<BY, BZ, NB>(myJ,
+                                       n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2],
+                                       R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1], R.x2[2], o1x,
+                                       o1y, o1z, cj);
             }
 
             /* migration by the cell-local offset */
@@ -355,11 +367,15 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             stays = (shx | shy | shz) == 0;
             if (!stays) {
                 atomicAdd(&sLeave[c], 1);
+                /* destination in the halo box (|shift| <= 1 per axis); its
+                 * rank and the queue slot are claimed together, so the two
+                 * LDS round trips overlap (after a queue overflow, which
+                 * fails the step, the halo counts include unqueued leavers) */
+                const int h = n0 + (BY * BZ + BZ + 1) + __mul24(shx, BY * BZ) +
+                              __mul24(shy, BZ) + shz;
+                const int hr = atomicAdd(&sHalo[h], 1);
                 const int qs_ = atomicAdd(&sQ, 1);
                 if (qs_ < a.q.qcap) {
-                    /* destination in the halo box (|shift| <= 1 per axis) */
-                    const int h = ((ca + shx + 1) * BY + (cb + shy + 1)) * BZ + (cc + shz + 1);
-                    const int hr = atomicAdd(&sHalo[h], 1);
                     const unsigned key = queue_key(s, (unsigned)h, (unsigned)hr);
                     float4 *q = a.q.e + 2 * (tile * (long)a.q.qcap + qs_);
                     q[0] = make_float4(nxo, nyo, nzo, pu);
@@ -369,7 +385,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         }
 
         /* stayers compacted in flat order: slot rank of the cell */
-        const int rank = stable_rank(c, valid, stays, carry_cell, carry_cnt);
+        const int rank = stable_rank_runs(runs, c, valid, stays, carry_cell, carry_cnt);
         if (valid && stays) {
             const unsigned dst = (unsigned)(sBase[c] + rank);
             slot(P0, dst) = nxo;
@@ -380,7 +396,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             slot(P5, dst) = pw;
             /* species and id only move with the particle (or come from an
              * inbox record) */
-            if (rank != j || (j >= sBc[c] && j < sBc[c] + PIC_INBOX)) {
+            if (rank != j || fromin) {
                 slot(SP, dst) = s;
                 if (DIFF) slot(ID, dst) = pid;
             }

@@ -18,6 +18,7 @@
  * order and is built with -ffp-contract=off.
  */
 #include "k_arrive.h"
+#include "k_tape.h"
 #include "k_push_math.h"
 
 #pragma clang fp contract(off)
@@ -227,6 +228,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         double xn = 0, yn = 0, zn = 0, pu = 0, pv = 0, pw = 0;
         unsigned char s = 0;
         int pid = 0;
+        tape_word tout = 0;    /* this step's decisions (decision tape, record mode) */
         if (valid) {
             const unsigned long long msk = starts & (~0ull >> (63 - lane));
             const unsigned ge = sGrp[g0 + __popcll(msk) - 1];
@@ -251,6 +253,16 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             pid = DIFF ? slot(ID, rel) : 0;
             const double2 mq = sMQ[s];
             const double mass = mq.x, q_m = mq.y;
+            /* decision tape (differentiable form): record this step's
+             * decisions, or replay a recorded run's */
+            const bool trec = DIFF && a.d.tape_mode == PIC_TAPE_RECORD;
+            bool trep = DIFF && a.d.tape_mode == PIC_TAPE_REPLAY;
+            const bool tin = (trec || trep) && pid >= 0 && pid < a.d.tape_n;
+            tape_word tw = 0;
+            if (trep) {
+                if (tin) tw = a.d.tape_row[pid];
+                if (!tin || !((tw >> TAPE_VALID) & 1ull) || ((tw >> TAPE_BAD) & 1ull)) bad = 1;
+            }
             const long r = snap ? a.d.snap_off[tile] + sOff[c] + j : 0;
             if (snap && r < a.d.snap_room) {
                 a.d.snap.p[0][r] = x;
@@ -267,8 +279,30 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             const double fry = divc(y - G.y0, hy, a.dc.rhy);
             const double frz = divc(z - G.z0, hz, a.dc.rhz);
             double Ex = 0, Ey = 0, Ez = 0, Hx = 0, Hy = 0, Hz = 0;
-            if (!(x < 0 || y < 0 || z < 0)) {
-                const GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
+            const bool neg = trep ? (tw & 1ull) != 0 : (x < 0 || y < 0 || z < 0);
+            if (trec && neg) tout |= 1ull;
+            if (!neg) {
+                GatherIdx W = gather_index(x, y, z, frx, fry, frz, g, a.dc);
+                const int gc[3] = {gi, gl, gk};
+                if (trep) {
+                    int *ii[6] = {&W.ix, &W.i1x, &W.iy, &W.i1y, &W.iz, &W.i1z};
+                    for (int q = 0; q < 6; q++) *ii[q] = tape_get(tw, TAPE_GIDX + q, 1, 2);
+                    for (int d = 0; d < 3; d++) {
+                        W.f1[d] = gc[d] + tape_get(tw, TAPE_FLOOR + 4 * d, 2, 0);
+                        W.f15[d] = gc[d] + tape_get(tw, TAPE_FLOOR + 4 * d + 2, 2, 0);
+                    }
+                    gather_weights(W, divc(x, g.hx, a.dc.rhx), divc(y, g.hy, a.dc.rhy),
+                                   divc(z, g.hz, a.dc.rhz));
+                    W.ok = gather_ok(W);
+                }
+                if (trec) {
+                    const int ii[6] = {W.ix, W.i1x, W.iy, W.i1y, W.iz, W.i1z};
+                    for (int q = 0; q < 6; q++) tape_put(tout, TAPE_GIDX + q, 1, ii[q], 2);
+                    for (int d = 0; d < 3; d++) {
+                        tape_put(tout, TAPE_FLOOR + 4 * d, 2, W.f1[d] - gc[d], 0);
+                        tape_put(tout, TAPE_FLOOR + 4 * d + 2, 2, W.f15[d] - gc[d], 0);
+                    }
+                }
                 const double s1 = W.s1, s2 = W.s2, s3 = W.s3, s4 = W.s4, s5 = W.s5, s6 = W.s6;
                 const double s11 = 1. - s1, s21 = 1. - s2, s31 = 1. - s3, s41 = 1. - s4,
                              s51 = 1. - s5, s61 = 1. - s6;
@@ -298,7 +332,21 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
              * without a crossing the first segment is the whole path (x2 = x1)
              * and the second is skipped */
             {
-                const Relay R = relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
+                const Relay R =
+                    trep ? relay_from(x, y, z, x1, y1, z1, G, tape_get(tw, TAPE_RI1, 1, 2),
+                                      tape_get(tw, TAPE_RI1 + 1, 1, 2),
+                                      tape_get(tw, TAPE_RI1 + 2, 1, 2),
+                                      tape_get(tw, TAPE_RI2, 2, 1), tape_get(tw, TAPE_RI2 + 2, 2, 1),
+                                      tape_get(tw, TAPE_RI2 + 4, 2, 1))
+                         : relay_point(x, y, z, x1, y1, z1, frx, fry, frz, G, a.dc);
+                if (trec) {
+                    tape_put(tout, TAPE_RI1, 1, R.i1x, 2);
+                    tape_put(tout, TAPE_RI1 + 1, 1, R.i1y, 2);
+                    tape_put(tout, TAPE_RI1 + 2, 1, R.i1z, 2);
+                    tape_put(tout, TAPE_RI2, 2, R.i2x, 1);
+                    tape_put(tout, TAPE_RI2 + 2, 2, R.i2y, 1);
+                    tape_put(tout, TAPE_RI2 + 4, 2, R.i2z, 1);
+                }
                 const int m = R.m;
                 const double e1x = m ? R.x2x : x1, e1y = m ? R.x2y : y1, e1z = m ? R.x2z : z1;
                 pqr_lds<BY, BZ, NB, DIFF>(myJ, G, a.dc, R.i1x, R.i1y, R.i1z, x, y, z, e1x, e1y, e1z,
@@ -308,9 +356,23 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             }
 
             /* x <- x1, Reflect (cell.h:1410-1478) */
-            xn = reflect1(x1, g.Lx);
-            yn = reflect1(y1, g.Ly);
-            zn = reflect1(z1, g.Lz);
+            if (DIFF && (trec || trep)) {
+                const double v1[3] = {x1, y1, z1}, Lm[3] = {g.Lx, g.Ly, g.Lz};
+                double vn[3];
+                for (int d = 0; d < 3; d++) {
+                    const int k = trep ? tape_get(tw, TAPE_REFL + 2 * d, 2, 0)
+                                       : reflect_case(v1[d], Lm[d]);
+                    if (trec) tape_put(tout, TAPE_REFL + 2 * d, 2, k, 0);
+                    vn[d] = reflect_apply(v1[d], Lm[d], k);
+                }
+                xn = vn[0];
+                yn = vn[1];
+                zn = vn[2];
+            } else {
+                xn = reflect1(x1, g.Lx);
+                yn = reflect1(y1, g.Ly);
+                zn = reflect1(z1, g.Lz);
+            }
 
             /* flyDirection (cell.h:694-701) with the owner cell's geometry */
             int ddx = (((xn > G.x0 + hx) && (xn < G.x0 + 2 * hx)) || ((xn < hx) && (gi == g.Nx - 1)))
@@ -328,6 +390,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             if (!inside) atomicAdd(&sOut[c], 1);
 
             stays = dir == 13;
+            /* a replayed word that cannot be trusted keeps its particle in
+             * place (the step reports PIC_F_RANGE) */
+            if (trep) stays = ((tw >> TAPE_STAY) & 1ull) || !tin ||
+                              !((tw >> TAPE_VALID) & 1ull) || ((tw >> TAPE_BAD) & 1ull);
+            if (trec && stays) tout |= 1ull << TAPE_STAY;
             if (!stays) {
                 atomicAdd(&sLeave[c], 1);
                 /* GPU_ArrangeFlights (kernels.cu:316-370): every bulk cell of the
@@ -349,6 +416,22 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
                     if (r >= g.Nz) r -= g.Nz; else if (r < 0) r += g.Nz;
                     r0 = (double)r * hz;
                     if (r0 <= zn && zn < r0 + hz) bz |= 1u << (d + 1);
+                }
+                if (trec) {
+                    const unsigned bb[3] = {bx, by, bz};
+                    for (int d = 0; d < 3; d++) {
+                        if (__popc(bb[d]) == 1) tape_put(tout, TAPE_RECV + 2 * d, 2, __ffs(bb[d]) - 1, 0);
+                        else tout |= 1ull << TAPE_BAD;
+                    }
+                }
+                if (trep) {
+                    /* the recorded receiver, one per axis (0..2) */
+                    unsigned *bb[3] = {&bx, &by, &bz};
+                    for (int d = 0; d < 3; d++) {
+                        const int k = tape_get(tw, TAPE_RECV + 2 * d, 2, 0);
+                        if (k > 2) bad = 1;
+                        *bb[d] = k > 2 ? 0u : 1u << k;
+                    }
                 }
                 const int ncopy = __popc(bx) * __popc(by) * __popc(bz);
                 if (ncopy == 0) atomicAdd(&sDrop, 1ull);
@@ -389,6 +472,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
                     }
                 }
             }
+        }
+
+        if constexpr (DIFF) {
+            if (valid && a.d.tape_mode == PIC_TAPE_RECORD && pid >= 0 && pid < a.d.tape_n)
+                a.d.tape_row[pid] = tout | (1ull << TAPE_VALID);
         }
 
         /* stayers compacted in flat order (the reference removes leavers by

@@ -88,9 +88,26 @@ struct Geo {
  * global x/h (s1..s6_interpolate: cell.h:1193-1236). */
 struct GatherIdx {
     int ix, i1x, iy, i1y, iz, i1z;
+    int f1[3], f15[3];   /* the weight floors (int)(x/h + 1), (int)(x/h + 1.5) per axis */
     double s1, s2, s3, s4, s5, s6;
     bool ok;          /* every index inside the tile stencil (2..3) */
 };
+
+/* the weights of GetField from given floors f1, f15 per axis (the same
+ * arithmetic as gather_index): s = f1 - x/h, s' = f15 - 0.5 - x/h */
+__device__ __forceinline__ void gather_weights(GatherIdx &G, double xh, double yh, double zh) {
+    G.s1 = G.f1[0] - xh;
+    G.s2 = G.f15[0] - 0.5 - xh;
+    G.s3 = G.f1[1] - yh;
+    G.s4 = G.f15[1] - 0.5 - yh;
+    G.s5 = G.f1[2] - zh;
+    G.s6 = G.f15[2] - 0.5 - zh;
+}
+
+__device__ __forceinline__ bool gather_ok(const GatherIdx &G) {
+    return !(G.ix < 2 || G.ix > 3 || G.i1x < 2 || G.i1x > 3 || G.iy < 2 || G.iy > 3 ||
+             G.i1y < 2 || G.i1y > 3 || G.iz < 2 || G.iz > 3 || G.i1z < 2 || G.i1z > 3);
+}
 
 __device__ __forceinline__ GatherIdx gather_index(double x, double y, double z, double frx,
                                                   double fry, double frz, const Grid &g,
@@ -100,18 +117,18 @@ __device__ __forceinline__ GatherIdx gather_index(double x, double y, double z, 
                  zh = divc(z, g.hz, dc.rhz);
     G.ix = (int)(frx + 1.0) + 1;
     G.i1x = (int)((frx + 1.0) + 1.5);
-    G.s1 = (int)(xh + 1.0) - xh;
-    G.s2 = (int)(xh + 1.5) - 0.5 - xh;
     G.iy = (int)(fry + 1.0) + 1;
     G.i1y = (int)((fry + 1.0) + 1.5);
-    G.s3 = (int)(yh + 1.0) - yh;
-    G.s4 = (int)(yh + 1.5) - 0.5 - yh;
     G.iz = (int)(frz + 1.0) + 1;
     G.i1z = (int)((frz + 1.0) + 1.5);
-    G.s5 = (int)(zh + 1.0) - zh;
-    G.s6 = (int)(zh + 1.5) - 0.5 - zh;
-    G.ok = !(G.ix < 2 || G.ix > 3 || G.i1x < 2 || G.i1x > 3 || G.iy < 2 || G.iy > 3 ||
-             G.i1y < 2 || G.i1y > 3 || G.iz < 2 || G.iz > 3 || G.i1z < 2 || G.i1z > 3);
+    G.f1[0] = (int)(xh + 1.0);
+    G.f15[0] = (int)(xh + 1.5);
+    G.f1[1] = (int)(yh + 1.0);
+    G.f15[1] = (int)(yh + 1.5);
+    G.f1[2] = (int)(zh + 1.0);
+    G.f15[2] = (int)(zh + 1.5);
+    gather_weights(G, xh, yh, zh);
+    G.ok = gather_ok(G);
     return G;
 }
 
@@ -178,19 +195,17 @@ struct Relay {
     double x2x, x2y, x2z;
 };
 
-__device__ __forceinline__ Relay relay_point(double x, double y, double z, double x1, double y1,
-                                             double z1, double frx, double fry, double frz,
-                                             const Geo &G, const DivConst &dc) {
+/* the relay geometry from given end cells i1 (start) and i2 (end) */
+__device__ __forceinline__ Relay relay_from(double x, double y, double z, double x1, double y1,
+                                            double z1, const Geo &G, int i1x, int i1y, int i1z,
+                                            int i2x, int i2y, int i2z) {
     Relay R;
-    const double t1x = divc(x1 - G.x0, G.hx, dc.rhx);
-    const double t1y = divc(y1 - G.y0, G.hy, dc.rhy);
-    const double t1z = divc(z1 - G.z0, G.hz, dc.rhz);
-    R.i1x = (int)((frx + 1.5) + 1);
-    R.i1y = (int)((fry + 1.5) + 1);
-    R.i1z = (int)((frz + 1.5) + 1);
-    R.i2x = (int)((t1x + 1.5) + 1);
-    R.i2y = (int)((t1y + 1.5) + 1);
-    R.i2z = (int)((t1z + 1.5) + 1);
+    R.i1x = i1x;
+    R.i1y = i1y;
+    R.i1z = i1z;
+    R.i2x = i2x;
+    R.i2y = i2y;
+    R.i2z = i2z;
     int m = 4 * abs(R.i2x - R.i1x) + 2 * abs(R.i2y - R.i1y) + abs(R.i2z - R.i1z);
     if (m > 7) m = 0;
     R.m = m;
@@ -224,6 +239,17 @@ __device__ __forceinline__ Relay relay_point(double x, double y, double z, doubl
     R.x2y = (m & 2) ? R.fy : (y + dys * R.sr);
     R.x2z = (m & 1) ? R.fz : (z + dzs * R.sr);
     return R;
+}
+
+__device__ __forceinline__ Relay relay_point(double x, double y, double z, double x1, double y1,
+                                             double z1, double frx, double fry, double frz,
+                                             const Geo &G, const DivConst &dc) {
+    const double t1x = divc(x1 - G.x0, G.hx, dc.rhx);
+    const double t1y = divc(y1 - G.y0, G.hy, dc.rhy);
+    const double t1z = divc(z1 - G.z0, G.hz, dc.rhz);
+    return relay_from(x, y, z, x1, y1, z1, G, (int)((frx + 1.5) + 1), (int)((fry + 1.5) + 1),
+                      (int)((frz + 1.5) + 1), (int)((t1x + 1.5) + 1), (int)((t1y + 1.5) + 1),
+                      (int)((t1z + 1.5) + 1));
 }
 
 #endif

@@ -26,6 +26,17 @@ __device__ __forceinline__ float tri(const float *F, int u, int v, int w, float 
     return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
 }
 
+/* the same from the corner node's address: p = F + (u BY + v) BZ + w */
+template <int BY, int BZ>
+__device__ __forceinline__ float tri_at(const float *p, float x0, float x1, float y0, float y1,
+                                        float z0, float z1) {
+    float a = z0 * p[0] + z1 * p[1];
+    float b = z0 * p[BZ] + z1 * p[BZ + 1];
+    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
+    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
+    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
+}
+
 struct Weights32 {
     int rd[3];        /* dual cell of the start (0/1 per axis) */
     float c0[3];      /* centre-type near weights */
@@ -44,6 +55,12 @@ __device__ __forceinline__ Weights32 weights32(float ox, float oy, float oz) {
     W.n0[1] = 1.0f - oy;
     W.n0[2] = 1.0f - oz;
     return W;
+}
+
+/* displacement in cell units, du = (tau/h) gi2 p, each product rounded */
+__device__ __forceinline__ float step_du(float tdh, float gi2, float p) {
+#pragma clang fp contract(off)
+    return (tdh * gi2) * p;
 }
 
 /* Boris push, Cayley form (particle.h:160-202), FP32, with the
@@ -86,10 +103,21 @@ __device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float E
     B.p1[2] = pw;
     const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
     B.gi2 = gi2;
-    B.du[0] = tdh[0] * gi2 * pu;
-    B.du[1] = tdh[1] * gi2 * pv;
-    B.du[2] = tdh[2] * gi2 * pw;
+    B.du[0] = step_du(tdh[0], gi2, pu);
+    B.du[1] = step_du(tdh[1], gi2, pv);
+    B.du[2] = step_du(tdh[2], gi2, pw);
     return B;
+}
+
+/* the end offset o + du.  Every decision of the deposit and the migration
+ * (end dual cell, crossing code, receiving cell) is taken from it, and the
+ * particle adjoint recomputes them in another kernel: both products of du
+ * and this sum are rounded one by one (no contraction into an FMA), so the
+ * two kernels derive them from identical values whatever the compiler fuses
+ * elsewhere. */
+__device__ __forceinline__ float end_offset(float o, float du) {
+#pragma clang fp contract(off)
+    return o + du;
 }
 
 /* deposit geometry (CurrentToMesh, cell.h:1306-1422, in cell units): dual

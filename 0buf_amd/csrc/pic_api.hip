@@ -239,6 +239,29 @@ int commit(pic_ctx *c) {
     if (c->committed) return PIC_SUCCESS;
     const long NC = c->NC;
     const long P = (long)c->staged.cell.size();
+    if (c->tape_mode == PIC_TAPE_RECORD) {
+        c->tape_cells.assign(c->n_ids, -1);
+        for (long j = 0; j < P; j++) c->tape_cells[c->staged.id[j]] = c->staged.cell[j];
+        c->tape_cells_set = true;
+    } else if (c->tape_mode == PIC_TAPE_REPLAY) {
+        /* every loaded particle starts in its recorded cell */
+        long kept = 0;
+        for (long k = 0; k < (long)c->tape_cells.size(); k++) kept += c->tape_cells[k] >= 0;
+        if (kept != P || (long)c->tape_cells.size() != c->n_ids) {
+            snprintf(g_hip_err, sizeof(g_hip_err),
+                     "decision tape: %ld particles recorded in cells, %ld loaded", kept, P);
+            return PIC_ERR_STATE;
+        }
+        for (long j = 0; j < P; j++) {
+            const int cell = c->tape_cells[c->staged.id[j]];
+            if (cell < 0 || cell >= NC) {
+                snprintf(g_hip_err, sizeof(g_hip_err), "decision tape: particle %d was dropped",
+                         c->staged.id[j]);
+                return PIC_ERR_STATE;
+            }
+            c->staged.cell[j] = cell;
+        }
+    }
     std::vector<long> off(NC + 1, 0);
     for (long j = 0; j < P; j++) off[c->staged.cell[j] + 1]++;
     int maxc = 0;
@@ -336,6 +359,12 @@ void enq_push(pic_ctx *c) {
         d.snap = c->rec_snap;
         d.snap_off = c->rec_off;
         d.snap_room = c->rec_room;
+        if (c->tape_mode != PIC_TAPE_OFF && c->tape_step < c->tape_nsteps) {
+            d.tape_mode = c->tape_mode;
+            d.tape_row = c->d_tape + (size_t)c->tape_step * c->tape_nids;
+            d.tape_n = c->tape_nids;
+            if (++c->tape_step >= c->tape_nsteps) c->tape_mode = PIC_TAPE_OFF;
+        }
         launch_push_deposit_f64(c->g, c->t, c->f, c->p, c->q, c->sp, c->prm.departure_cap,
                                 c->d_st, &d, c->stream);
         launch_jfix_to_f64(c->g, c->f, d, c->stream);
@@ -636,6 +665,7 @@ int pic_destroy(pic_ctx *c) {
     if (c->d_red) hipFree(c->d_red);
     if (c->h_red) hipHostFree(c->h_red);
     if (c->d_slot_sort) hipFree(c->d_slot_sort);
+    if (c->d_tape) hipFree(c->d_tape);
     for (int k = 0; k <= PIC_NSTAGES; k++)
         if (c->ev[k]) hipEventDestroy(c->ev[k]);
     if (c->ev_dep) hipEventDestroy(c->ev_dep);
@@ -1078,6 +1108,87 @@ int pic_cell_moments(pic_ctx *c, int sort, int op, double *out) {
     if (hipMemcpy(out, d, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) rc = PIC_ERR_HIP;
     (void)hipFree(d);
     return rc;
+}
+
+/* ------------------------------------------------------- decision tape */
+static int tape_alloc(pic_ctx *c, int nsteps) {
+    const size_t words = (size_t)nsteps * (size_t)c->n_ids;
+    if (c->d_tape && c->tape_nsteps == nsteps && c->tape_nids == c->n_ids) return PIC_SUCCESS;
+    if (c->d_tape) (void)hipFree(c->d_tape);
+    c->d_tape = nullptr;
+    hipError_t e = hipMalloc(&c->d_tape, std::max(words, (size_t)1) * sizeof(unsigned long long));
+    if (e != hipSuccess) return e == hipErrorOutOfMemory ? PIC_ERR_NOMEM : record_hip(e, "tape");
+    /* words never written (ids dropped at load) read as "not recorded" */
+    HIPCHK(hipMemset(c->d_tape, 0, std::max(words, (size_t)1) * sizeof(unsigned long long)));
+    c->tape_nsteps = nsteps;
+    c->tape_nids = c->n_ids;
+    return PIC_SUCCESS;
+}
+
+static int tape_usable(pic_ctx *c, int nsteps) {
+    if (!c || nsteps < 1) return PIC_ERR_INVALID;
+    if (c->fp32 || !c->diff) {
+        snprintf(g_hip_err, sizeof(g_hip_err), "decision tape: FP64 differentiable form only");
+        return PIC_ERR_INVALID;
+    }
+    return PIC_SUCCESS;
+}
+
+int pic_tape(pic_ctx *c, int mode, int nsteps) {
+    if (mode == PIC_TAPE_OFF) {
+        if (!c) return PIC_ERR_INVALID;
+        c->tape_mode = PIC_TAPE_OFF;
+        return PIC_SUCCESS;
+    }
+    int rc = tape_usable(c, nsteps);
+    if (rc) return rc;
+    if (mode != PIC_TAPE_RECORD && mode != PIC_TAPE_REPLAY) return PIC_ERR_INVALID;
+    /* the initial cells are taken (record) or imposed (replay) at the first
+     * step, when the loaded particles are committed to their buckets */
+    if (c->committed || c->n_ids == 0) {
+        snprintf(g_hip_err, sizeof(g_hip_err),
+                 "decision tape: needs loaded particles that have not stepped yet");
+        return PIC_ERR_STATE;
+    }
+    if (mode == PIC_TAPE_RECORD) {
+        if ((rc = tape_alloc(c, nsteps))) return rc;
+        c->tape_cells_set = false;
+    } else if (!c->tape_cells_set || !c->d_tape || c->tape_nsteps < nsteps ||
+               c->tape_nids != c->n_ids) {
+        snprintf(g_hip_err, sizeof(g_hip_err), "decision tape: no tape set for these particles");
+        return PIC_ERR_STATE;
+    }
+    c->tape_mode = mode;
+    c->tape_step = 0;
+    c->tape_nsteps = nsteps;
+    return PIC_SUCCESS;
+}
+
+int pic_tape_get(pic_ctx *c, unsigned long long *words, int *cells, long nids, int nsteps) {
+    int rc = tape_usable(c, nsteps);
+    if (rc) return rc;
+    if (!c->d_tape || !c->tape_cells_set || nids != c->tape_nids || nsteps > c->tape_nsteps)
+        return PIC_ERR_STATE;
+    if (c->stream) HIPCHK(hipStreamSynchronize(c->stream));
+    if (words)
+        HIPCHK(hipMemcpy(words, c->d_tape, (size_t)nsteps * nids * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    if (cells) std::memcpy(cells, c->tape_cells.data(), nids * sizeof(int));
+    return PIC_SUCCESS;
+}
+
+int pic_tape_set(pic_ctx *c, const unsigned long long *words, const int *cells, long nids,
+                 int nsteps) {
+    int rc = tape_usable(c, nsteps);
+    if (rc) return rc;
+    if (!words || !cells || nids != c->n_ids) return PIC_ERR_INVALID;
+    if (c->committed) return PIC_ERR_STATE;
+    if ((rc = tape_alloc(c, nsteps))) return rc;
+    HIPCHK(hipMemcpy(c->d_tape, words, (size_t)nsteps * nids * sizeof(unsigned long long),
+                     hipMemcpyHostToDevice));
+    c->tape_cells.assign(cells, cells + nids);
+    c->tape_cells_set = true;
+    return PIC_SUCCESS;
 }
 
 int pic_gauss_residual(pic_ctx *c, double *res, double *max_drift, int set_reference) {

@@ -54,6 +54,14 @@ struct pic_ctx {
     long long *d_jfix[3] = {nullptr, nullptr, nullptr};
     long n_ids = 0;
     AdjointStore *adj = nullptr;
+    /* decision tape (FP64 differentiable form): tape_nsteps rows of tape_nids
+     * words; the push of step tape_step uses row tape_step while tape_mode is
+     * on; tape_cells: the bulk cell of each id at load (-1 dropped) */
+    unsigned long long *d_tape = nullptr;
+    long tape_nids = 0;
+    int tape_nsteps = 0, tape_step = 0, tape_mode = 0;
+    bool tape_cells_set = false;
+    std::vector<int> tape_cells;
     PartPtrs rec_snap{};      /* record target of the next push (adjoint recompute) */
     PartPtrs32 rec_snap32{};
     const long *rec_off = nullptr;   /* tile offsets of that record */

@@ -148,15 +148,13 @@ __device__ __forceinline__ T &slot(T *base, unsigned rel) {
  * slot), so no other wavefront and no later chunk is disturbed, and the
  * buckets need no second pass.
  *
- * stable_rank: the rank of this lane's particle (valid lanes form a prefix
+ * stable_rank_runs: the rank of this lane's particle (valid lanes form a prefix
  * of the wavefront and hold consecutive flat indices); the
  * wavefront-uniform carry holds the cell of the previous chunk's last
  * particle and that cell's stayers so far. */
-__device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &carry_cell,
-                                           int &carry_cnt) {
+__device__ __forceinline__ int stable_rank_runs(unsigned long long R, int c, bool valid,
+                                                bool stays, int &carry_cell, int &carry_cnt) {
     const int lane = (int)(threadIdx.x & 63);
-    const int cprev = __shfl_up(c, 1);
-    const unsigned long long R = __ballot(valid && (lane == 0 || c != cprev));
     const unsigned long long S = __ballot(valid && stays);
     const unsigned long long V = __ballot(valid);
     const unsigned long long le = ~0ull >> (63 - lane);
@@ -173,6 +171,16 @@ __device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &c
         carry_cell = cL;
     }
     return rank;
+}
+
+/* R = the lanes where a run of one cell begins (lane 0 and every lane
+ * whose cell differs from the previous lane's) */
+__device__ __forceinline__ int stable_rank(int c, bool valid, bool stays, int &carry_cell,
+                                           int &carry_cnt) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int cprev = __shfl_up(c, 1);
+    const unsigned long long R = __ballot(valid && (lane == 0 || c != cprev));
+    return stable_rank_runs(R, c, valid, stays, carry_cell, carry_cnt);
 }
 
 /* Line groups.  A cell's bucket is split into groups of GS slots (GS slots =
@@ -238,6 +246,31 @@ __device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int g
     return g;
 }
 
+/* the same, also marking in `cstarts` the lane where each run of one cell
+ * begins (lane 0 and every group whose cell differs from the previous
+ * group's): the run starts stable_rank_runs needs, found on the scalar unit
+ * from the group entries instead of a lane shuffle */
+__device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int gend,
+                                             unsigned long long &starts, int &lanes,
+                                             unsigned long long &cstarts) {
+    int g = g0, n = 0, prev = -1;
+    unsigned long long m = 0, cs = 0;
+    while (g < gend) {
+        const unsigned e = __builtin_amdgcn_readfirstlane(sGrp[g]);
+        const int sz = group_size(e);
+        if (n + sz > 64) break;
+        m |= 1ull << n;
+        if (group_cell(e) != prev) cs |= 1ull << n;
+        prev = group_cell(e);
+        n += sz;
+        g++;
+    }
+    starts = m;
+    lanes = n;
+    cstarts = cs;
+    return g;
+}
+
 struct SpeciesTable {
     double m[PIC_MAX_SLOTS];
     double q_m[PIC_MAX_SLOTS];
@@ -273,6 +306,10 @@ struct DiffArgs {
     PartPtrs snap;
     const long *snap_off;   /* record of tile t starts at snap_off[t] (flat tile order) */
     long snap_room;         /* entries the record holds */
+    /* decision tape (k_tape.h): this step's row, one word per particle id */
+    unsigned long long *tape_row;
+    long tape_n;            /* ids the row holds */
+    int tape_mode;          /* PIC_TAPE_OFF / RECORD / REPLAY */
 };
 
 /* particle launchers (k_particles_f64.hip) */

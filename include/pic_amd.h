@@ -323,6 +323,33 @@ int pic_rewind(pic_ctx *ctx);
  * out[0] forward with checkpoints, out[1] recompute, out[2] adjoint steps */
 int pic_adjoint_timing(pic_ctx *ctx, double *out3);
 
+/* Decision tape of the FP64 differentiable form.  No counterpart in the
+ * reference; for checking gradients against finite differences.  The step
+ * is piecewise smooth in the particle state: the integers it derives from
+ * positions (the cell a particle is loaded into; per step the gather's tile
+ * indices and weight floors, the deposit's relay cells, the periodic wraps
+ * and the receiving cell) are constant on the pieces, and pic_adjoint
+ * differentiates the piece its run lies on.  A run that records these
+ * integers and a run from a perturbed state that replays them stay on the
+ * same piece, so central differences of the replaying runs converge to the
+ * adjoint at second order in the step.
+ *   pic_tape(ctx, PIC_TAPE_RECORD, n): the next n steps (the context holds
+ *     loaded, not yet stepped particles) record their decisions;
+ *   pic_tape_get: the recorded tape to the host: words (n x ids, step-major,
+ *     indexed by particle id) and cells (ids: the bulk cell each particle was
+ *     loaded into, -1 if dropped);
+ *   pic_tape_set + pic_tape(ctx, PIC_TAPE_REPLAY, n): the next n steps of a
+ *     context with the same particles (ids) loaded replay that tape; the
+ *     particles start in the recorded cells.  A step whose tape word is not
+ *     replayable (a decision outside its field, or a leaver with other than
+ *     one receiver when it was recorded) ends with PIC_ERR_CELL_RANGE.
+ * FP64 differentiable form only (PIC_ERR_INVALID otherwise). */
+enum pic_tape_mode { PIC_TAPE_OFF = 0, PIC_TAPE_RECORD = 1, PIC_TAPE_REPLAY = 2 };
+int pic_tape(pic_ctx *ctx, int mode, int nsteps);
+int pic_tape_get(pic_ctx *ctx, unsigned long long *words, int *cells, long nids, int nsteps);
+int pic_tape_set(pic_ctx *ctx, const unsigned long long *words, const int *cells, long nids,
+                 int nsteps);
+
 /* ------------------------------------------------------------------------
  * Seeded initial conditions (host only; the step never calls these).
  * Generators restate the reference's maxwell.cu: Wichmann-Hill uniform
