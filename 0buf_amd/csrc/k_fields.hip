@@ -219,6 +219,55 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_fold_j(Grid g, FV<T, JT> f, int d
     }
 }
 
+/* The periodic E ghost copies and the field energy in one pass (the step's
+ * own sequence: x copies, z copies, in ghost_mode 1 the y copies, then the
+ * energy over every ghosted node).  Each node's value after the copies is
+ * read where the copy chain takes it from: ghost_source follows the copies
+ * backwards (the last one first), each applied where its range covers the
+ * current node, down to a node no copy writes -- so every read is of a node
+ * this kernel does not write, and each copied node gets the value the
+ * sequential copies give it.  The energy is summed in k_energy's order. */
+__device__ __forceinline__ long ghost_source(const Grid &g, int comp, int i, int l, int k) {
+    if (g.ghost_mode == 1 && comp != 1) {            /* y copies of Ex, Ez */
+        if (l == 0) l = g.Ny; else if (l == g.Ny + 1) l = 1;
+    }
+    if ((comp == 0 && i <= g.Nx) || (comp == 1 && l <= g.Ny)) {        /* z copies */
+        if (k == 0) k = g.Nz; else if (k == g.Nz + 1) k = 1;
+    }
+    if ((comp == 1 && l <= g.Ny && k >= 1 && k <= g.Nz) ||
+        (comp == 2 && l >= 1 && l <= g.Ny && k <= g.Nz)) {               /* x copies */
+        if (i == 0) i = g.Nx; else if (i == g.Nx + 1) i = 1;
+    }
+    return (long)i * g.sx + (long)l * g.sy + k;
+}
+
+template <typename T, typename JT>
+__global__ __launch_bounds__(PIC_BLOCK) void k_e_ghosts_energy(Grid g, FV<T, JT> f,
+                                                               double *partials) {
+    __shared__ double red[PIC_BLOCK];
+    double s = 0.0;
+    for (long n = (long)blockIdx.x * PIC_BLOCK + threadIdx.x; n < g.G;
+         n += (long)gridDim.x * PIC_BLOCK) {
+        int i, l, k;
+        decode(g, n, i, l, k);
+        T v[3];
+        for (int c = 0; c < 3; c++) {
+            const long src = ghost_source(g, c, i, l, k);
+            v[c] = f.E[c][src];
+            if (src != n) f.E[c][n] = v[c];
+        }
+        double a = (double)v[0], b = (double)v[1], c = (double)v[2];
+        s += a * a + b * b + c * c;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = PIC_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
 /* getElectricEnergy (gpu_plasma.h:621-657) -> GPU_getCellEnergy
  * (wrap_kernel.cu:14-36): sum over every ghosted node of Ex^2+Ey^2+Ez^2,
  * accumulated in FP64 by a deterministic two-level tree instead of
@@ -297,6 +346,11 @@ void fold_j(const Grid &g, FV<T, JT> f, hipStream_t s) {
                            g, f, dir);
 }
 template <typename T, typename JT>
+void ghosts_energy(const Grid &g, FV<T, JT> f, double *partials, int nblocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_e_ghosts_energy<T, JT>), dim3(nblocks), dim3(PIC_BLOCK), 0, s, g, f,
+                       partials);
+}
+template <typename T, typename JT>
 void energy(const Grid &g, FV<T, JT> f, double *partials, int nblocks, double *out,
             hipStream_t s) {
     hipLaunchKernelGGL((k_energy<T, JT>), dim3(nblocks), dim3(PIC_BLOCK), 0, s, g, f, partials);
@@ -325,6 +379,18 @@ void launch_fold_j32(const Grid &g, FieldPtrs32 f, hipStream_t s) { fold_j(g, vi
 void launch_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks, double *out,
                      hipStream_t s) {
     energy(g, view(f), partials, nblocks, out, s);
+}
+
+void launch_ghosts_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks,
+                          hipStream_t s) {
+    ghosts_energy(g, view(f), partials, nblocks, s);
+}
+void launch_ghosts_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks,
+                            hipStream_t s) {
+    ghosts_energy(g, view(f), partials, nblocks, s);
+}
+void launch_reduce_final(const double *partials, int nblocks, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(PIC_BLOCK), 0, s, partials, nblocks, out);
 }
 
 void launch_reset_status(DevStatus *st, hipStream_t s) {

@@ -4,8 +4,9 @@
  * Same tile structure as the FP64 plain form (one workgroup per 2x4x8 cells,
  * E/H node box staged in LDS, tile departure regions), with:
  *   * each wavefront owning whole cells of the tile and walking them in
- *     chunks of whole 128-byte line groups; stayers are compacted in place in
- *     flat order (stable), so a bucket needs no second pass after the push;
+ *     chunks of 64 consecutive particles (a line may be split between two
+ *     consecutive chunks); stayers are compacted in place in flat order
+ *     (stable), so a bucket needs no second pass after the push;
  *   * positions held as FP32 offsets in [0,1) inside the owning cell, so the
  *     resolution is h * 2^-24 everywhere in the box;
  *   * FP32 gather and Boris push (the reference's Cayley form);
@@ -217,16 +218,20 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sFlags = 0;
     }
     __syncthreads();
-    if (tid == 0) {
-        int s = 0, ng = 0;
-        for (int c = 0; c < C; c++) {
-            sOff[c] = s;
-            sG[c] = ng;
-            s += sCnt[c];
-            ng += cell_groups32(sCnt[c], sBc[c]);
+    static_assert(C <= 64, "one wavefront scans the tile's cells");
+    if (tid < 64) {   /* first particle and first line group of each cell */
+        const int cnt = tid < C ? sCnt[tid] : 0;
+        const int ngc = tid < C ? cell_groups32(cnt, sBc[tid]) : 0;
+        int tot_s, tot_g;
+        const int so = wave_excl_scan(cnt, tot_s), sg = wave_excl_scan(ngc, tot_g);
+        if (tid < C) {
+            sOff[tid] = so;
+            sG[tid] = sg;
         }
-        sOff[C] = s;
-        sG[C] = ng;
+        if (tid == 0) {
+            sOff[C] = tot_s;
+            sG[C] = tot_g;
+        }
     }
     __syncthreads();
     build_groups32<C>(sCnt, sBc, sG, sGrp, tid);
@@ -257,12 +262,13 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     unsigned char qs = 0;
     int qid = 0;
     bool qin = false;     /* the particle comes from an inbox record */
-    auto fetch = [&](int g0, unsigned long long starts) {
+    auto fetch = [&](int g0, int o0, unsigned long long starts) {
         const unsigned long long m = starts & (~0ull >> (63 - lane));
-        const unsigned e = sGrp[g0 + __popcll(m) - 1];
+        const int gi = __popcll(m) - 1;
+        const unsigned e = sGrp[g0 + gi];
         const int cf = group_cell(e);
         nfc = cf;
-        nslot = group_start(e) + (lane - (63 - __clzll(m)));
+        nslot = group_start(e) + (lane - (63 - __clzll(m))) + (gi == 0 ? o0 : 0);
         unsigned nrel = (unsigned)(sBase[cf] + nslot);
         unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
         asm volatile("" : "+v"(nrel));
@@ -285,9 +291,9 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
     int ng0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
     unsigned long long nstarts = 0, ncstarts = 0;
-    int nlanes = 0;
-    int gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes, ncstarts);
-    if (lane < nlanes) fetch(ng0, nstarts);
+    int nlanes = 0, no0 = 0, onext = 0;
+    int gnext = compose_full_chunk(sGrp, ng0, no0, ghi, nstarts, nlanes, ncstarts, onext);
+    if (lane < nlanes) fetch(ng0, no0, nstarts);
     int carry_cell = -1, carry_cnt = 0;
     while (nlanes > 0) {
         const bool valid = lane < nlanes;
@@ -298,8 +304,9 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         const int pid = DIFF ? qid : 0;
         const bool fromin = qin;
         ng0 = gnext;
-        gnext = compose_chunk(sGrp, ng0, ghi, nstarts, nlanes, ncstarts);
-        if (lane < nlanes) fetch(ng0, nstarts);
+        no0 = onext;
+        gnext = compose_full_chunk(sGrp, ng0, no0, ghi, nstarts, nlanes, ncstarts, onext);
+        if (lane < nlanes) fetch(ng0, no0, nstarts);
         bool stays = false;
         float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
         if (valid) {

@@ -185,16 +185,20 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         sDup = 0;
     }
     __syncthreads();
-    if (tid == 0) {
-        int s = 0, ng = 0;
-        for (int c = 0; c < C; c++) {
-            sOff[c] = s;
-            sG[c] = ng;
-            s += sCnt[c];
-            ng += (sCnt[c] + GS64 - 1) / GS64;
+    static_assert(C <= 64, "one wavefront scans the tile's cells");
+    if (tid < 64) {   /* first particle and first line group of each cell */
+        const int cnt = tid < C ? sCnt[tid] : 0;
+        int tot_s, tot_g;
+        const int so = wave_excl_scan(cnt, tot_s),
+                  sg = wave_excl_scan((cnt + GS64 - 1) / GS64, tot_g);
+        if (tid < C) {
+            sOff[tid] = so;
+            sG[tid] = sg;
         }
-        sOff[C] = s;
-        sG[C] = ng;
+        if (tid == 0) {
+            sOff[C] = tot_s;
+            sG[C] = tot_g;
+        }
     }
     __syncthreads();
     build_groups<TX, TY, TZ, GS64>(sCnt, sG, sGrp, tid, PIC_BLOCK);
@@ -211,17 +215,19 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
 
     const double hx = g.hx, hy = g.hy, hz = g.hz, tau = g.tau;
     unsigned bad = 0;
-    /* each wavefront owns whole cells and walks their line groups in chunks
-     * of whole groups; stayers are compacted in place in flat order */
+    /* each wavefront owns whole cells and walks their particles in chunks of
+     * 64 consecutive ones (a line group may be split between two consecutive
+     * chunks); stayers are compacted in place in flat order */
     const int lane = tid & 63, wv = tid >> 6;
     const int ghi = __builtin_amdgcn_readfirstlane(
         sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
     int g0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
+    int o0 = 0;
     int carry_cell = -1, carry_cnt = 0;
     while (g0 < ghi) {
-        unsigned long long starts = 0;
-        int lanes = 0;
-        const int gnext = compose_chunk(sGrp, g0, ghi, starts, lanes);
+        unsigned long long starts = 0, runs = 0;
+        int lanes = 0, onext = 0;
+        const int gnext = compose_full_chunk(sGrp, g0, o0, ghi, starts, lanes, runs, onext);
         const bool valid = lane < lanes;
         int c = 0, j = 0;
         bool stays = false;
@@ -231,9 +237,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
         tape_word tout = 0;    /* this step's decisions (decision tape, record mode) */
         if (valid) {
             const unsigned long long msk = starts & (~0ull >> (63 - lane));
-            const unsigned ge = sGrp[g0 + __popcll(msk) - 1];
+            const int gidx = __popcll(msk) - 1;
+            const unsigned ge = sGrp[g0 + gidx];
             c = group_cell(ge);
-            j = group_start(ge) + (lane - (63 - __clzll(msk)));
+            j = group_start(ge) + (lane - (63 - __clzll(msk))) + (gidx == 0 ? o0 : 0);
             const unsigned rel = (unsigned)(sBase[c] + j);
             Geo G;
             G.ca = c / (TY * TZ);
@@ -483,7 +490,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
          * swap-with-last, removeParticleFromSurfaceDevice, cell.h:391-432;
          * the order inside a cell changes no result but the rounding of the
          * FP64 atomic sums, which have no fixed order either) */
-        const int rank = stable_rank(c, valid, stays, carry_cell, carry_cnt);
+        const int rank = stable_rank_runs(runs, c, valid, stays, carry_cell, carry_cnt);
         if (valid && stays) {
             const unsigned dst = (unsigned)(sBase[c] + rank);
             slot(P0, dst) = xn;
@@ -498,6 +505,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
             }
         }
         g0 = gnext;
+        o0 = onext;
     }
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();

@@ -450,10 +450,13 @@ int enq_step(pic_ctx *c) {
     enq_h2(c);
     mark(c, PIC_STAGE_E);
     enq_e(c);
+    /* the periodic E ghosts fused with the energy pass: the same values as
+     * the stage entry points' separate launches (k_fields.hip) */
     mark(c, PIC_STAGE_E_PERIODIC);
-    enq_e_periodic(c);
+    if (c->fp32) launch_ghosts_energy32(c->g, c->f32, c->d_part, c->npart, c->stream);
+    else launch_ghosts_energy(c->g, c->f, c->d_part, c->npart, c->stream);
     mark(c, PIC_STAGE_ENERGY);
-    enq_energy(c);
+    launch_reduce_final(c->d_part, c->npart, c->d_red, c->stream);
     mark(c, PIC_NSTAGES);
     if (overlap) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     HIPCHK(hipGetLastError());

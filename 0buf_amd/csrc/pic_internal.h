@@ -227,6 +227,19 @@ __device__ __forceinline__ int wave_first_cell(const int *sOff, int w, int nw) {
     return hi;
 }
 
+/* exclusive prefix sum of v over the 64 lanes of a wavefront; *total
+ * receives the sum over all lanes */
+__device__ __forceinline__ int wave_excl_scan(int v, int &total) {
+    const int lane = (int)(threadIdx.x & 63);
+    int x = v;
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d);
+        if (lane >= d) x += y;
+    }
+    total = __shfl(x, 63);
+    return x - v;
+}
+
 /* chunk composition (wavefront-uniform): whole groups from g0 while they
  * fit 64 lanes; lane start of each group in `starts`; returns the next
  * group */
@@ -271,12 +284,54 @@ __device__ __forceinline__ int compose_chunk(const unsigned *sGrp, int g0, int g
     return g;
 }
 
+/* chunks of exactly 64 particles (fewer only at the end of the wavefront's
+ * cells): from group g0, whose first o0 particles an earlier chunk took,
+ * whole groups while they fit and then the first part of the next group.
+ * starts: the lane where each group's part begins; cstarts: the lane where
+ * each run of one cell begins (as above); o_next: how many particles of the
+ * returned group this chunk took (0: none, the group is untouched).  A line
+ * split between two consecutive chunks is read twice, back to back. */
+__device__ __forceinline__ int compose_full_chunk(const unsigned *sGrp, int g0, int o0, int gend,
+                                                  unsigned long long &starts, int &lanes,
+                                                  unsigned long long &cstarts, int &o_next) {
+    int g = g0, n = 0, prev = -1, off = o0;
+    unsigned long long m = 0, cs = 0;
+    o_next = 0;
+    while (g < gend && n < 64) {
+        const unsigned e = __builtin_amdgcn_readfirstlane(sGrp[g]);
+        const int sz = group_size(e) - off;
+        m |= 1ull << n;
+        if (group_cell(e) != prev) cs |= 1ull << n;
+        prev = group_cell(e);
+        if (n + sz > 64) {
+            o_next = off + (64 - n);
+            n = 64;
+            break;
+        }
+        n += sz;
+        off = 0;
+        g++;
+    }
+    starts = m;
+    lanes = n;
+    cstarts = cs;
+    return g;
+}
+
 struct SpeciesTable {
     double m[PIC_MAX_SLOTS];
     double q_m[PIC_MAX_SLOTS];
 };
 
 /* kernel launchers (k_fields.hip) */
+/* the step's fused pass: the periodic E ghosts with the energy partials
+ * (as launch_e_periodic followed by launch_energy's first kernel) */
+void launch_ghosts_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks,
+                          hipStream_t s);
+void launch_ghosts_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks,
+                            hipStream_t s);
+/* sum of nblocks partials into *out (deterministic tree) */
+void launch_reduce_final(const double *partials, int nblocks, double *out, hipStream_t s);
 void launch_h_half(const Grid &g, FieldPtrs f, hipStream_t s);
 void launch_h2(const Grid &g, FieldPtrs f, hipStream_t s);
 void launch_e(const Grid &g, FieldPtrs f, hipStream_t s);
