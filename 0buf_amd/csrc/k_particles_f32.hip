@@ -30,6 +30,9 @@
 #ifndef PIC_JCOPIES32
 #define PIC_JCOPIES32 2
 #endif
+#ifndef PIC_JPAD32
+#define PIC_JPAD32 8
+#endif
 
 namespace {
 
@@ -144,7 +147,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
     __shared__ float sF[6 * NB];
-    __shared__ long long sJ[JCOPIES * 3 * NB];
+    /* J box copies JS elements apart: JS = 3 NB rounded to 8 mod 16, so
+     * that copy 1 sits 16 (of 32) or 48 (of 64) dword banks from copy 0 and
+     * the banks one deposit instruction touches in the two copies (the 8
+     * dual-cell offsets of a cell, 2 banks each) do not overlap */
+    constexpr int JS = (3 * NB + 7) / 16 * 16 + PIC_JPAD32;
+    __shared__ long long sJ[JCOPIES * JS];
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
@@ -157,7 +165,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     __shared__ unsigned sFlags;
     __shared__ float4 sSpec[PIC_MAX_SLOTS];  /* (q/m tau/2, cj) per species slot */
     extern __shared__ unsigned sGrp[];    /* line groups of the tile */
-    long long *const myJ = sJ + (threadIdx.x % JCOPIES) * (3 * NB);
+    long long *const myJ = sJ + (threadIdx.x % JCOPIES) * JS;
 
     const Grid &g = a.g;
     const int tid = threadIdx.x;
@@ -205,7 +213,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             val = Fsrc[fi][(long)gu * g.sx + (long)gv * g.sy + gw];
         sF[e] = val;
     }
-    for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0;
+    for (int e = tid; e < JCOPIES * JS; e += PIC_BLOCK) sJ[e] = 0;
     for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
     /* the per-species constants in LDS: a kernel-argument table indexed by a
      * per-particle value would be a vector memory load in the loop, and its
@@ -425,7 +433,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
 
     for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
         long long val = sJ[e];
-        for (int r = 1; r < JCOPIES; r++) val += sJ[r * 3 * NB + e];
+        for (int r = 1; r < JCOPIES; r++) val += sJ[r * JS + e];
         if (val == 0) continue;
         int comp = e / NB, nn = e % NB;
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
