@@ -63,7 +63,7 @@ class PicStepStatus(C.Structure):
     _fields_ = [("departures", C.c_longlong), ("dropped", C.c_longlong),
                 ("duplicated", C.c_longlong), ("max_cell_departures", C.c_int),
                 ("bucket_regrows", C.c_int), ("bucket_capacity", C.c_int),
-                ("flags", C.c_int)]
+                ("flags", C.c_int), ("spill_repairs", C.c_int)]
 
 
 class PicRng(C.Structure):

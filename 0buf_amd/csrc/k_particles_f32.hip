@@ -5,8 +5,9 @@
  * E/H node box staged in LDS, tile departure regions), with:
  *   * each wavefront owning whole cells of the tile and walking them in
  *     chunks of 64 consecutive particles (a line may be split between two
- *     consecutive chunks); stayers are compacted in place in flat order
- *     (stable), so a bucket needs no second pass after the push;
+ *     consecutive chunks); stayers written in flat order into the other
+ *     particle set, packed per tile (Layout32 in pic_internal.h), arrivals
+ *     into inbox records and spill lists that the next push reads;
  *   * positions held as FP32 offsets in [0,1) inside the owning cell, so the
  *     resolution is h * 2^-24 everywhere in the box;
  *   * FP32 gather and Boris push (the reference's Cayley form);
@@ -39,12 +40,15 @@ namespace {
 constexpr int TXC = 2, TYC = 4, TZC = 8;
 constexpr int GS32 = 32;    /* slots of an FP32 array in one 128-byte line */
 constexpr int JCOPIES = PIC_JCOPIES32;
+static_assert(PIC_SPILL_CAP <= 256, "spill records are indexed in 8 bits in LDS");
 
 struct PushArgs32 {
     Grid g;
     TileGeom t;
     FieldPtrs32 f;
-    PartPtrs32 p;
+    PartPtrs32 p;                  /* the live set, read */
+    PartPtrs32 o;                  /* the other set, written packed */
+    Layout32 L;
     QueuePtrs32 q;
     float qm[PIC_MAX_SLOTS];       /* charge/mass per species slot */
     float cj[PIC_MAX_SLOTS][3];    /* m h_d / tau * 2^40: fixed-point current per
@@ -104,32 +108,54 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
     add64(&Jz[n + BY * BZ + BZ], fixmul(qz, mx * my + sxy));
 }
 
-/* groups of a cell of the live FP32 buckets, in slot order: bucket lines of
- * [0, bc), inbox records [bc, bc + nin) by 4 (one 128-byte line of
- * records), bucket slots [bc + PIC_INBOX, cnt) by line (arrivals past a full
- * inbox) */
+/* Line groups of one cell, in slot order.  Buckets: lines of [0, bc),
+ * inbox records [bc, bc + nin) by 4 (one 128-byte line of records), bucket
+ * slots [bc + PIC_INBOX, cnt) by line (arrivals past a full inbox).
+ * Packed: the particle-array part [0, bc), which starts at slot ps of the
+ * tile's region, split where the region's lines begin; the inbox records by
+ * 4; the spill records [bc + PIC_INBOX, cnt) by 32 (they are read one by one
+ * through the tile's spill index, so their grouping only sets how many
+ * entries they take). */
 __device__ __forceinline__ int cell_groups32(int cnt, int bc) {
     const int nin = min(PIC_INBOX, cnt - bc), t0 = bc + PIC_INBOX;
     return (bc + GS32 - 1) / GS32 + (nin + 3) / 4 + (cnt > t0 ? (cnt - 1) / GS32 - t0 / GS32 + 1 : 0);
 }
+__device__ __forceinline__ int cell_groups_packed(int ps, int bc, int nin, int nsp) {
+    const int nb = bc > 0 ? (ps + bc - 1) / GS32 - ps / GS32 + 1 : 0;
+    return nb + (nin + 3) / 4 + (nsp + GS32 - 1) / GS32;
+}
 
 template <int C>
-__device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, const int *sG,
-                                               unsigned *sGrp, int tid) {
+__device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, const int *sBase,
+                                               const int *sG, bool packed, unsigned *sGrp,
+                                               int tid) {
     for (int w = tid; w < 4 * C; w += PIC_BLOCK) {
         const int c = w >> 2, part = w & 3;
         const int cnt = sCnt[c], bc = sBc[c];
-        const int nsoa = (bc + GS32 - 1) / GS32, nin = min(PIC_INBOX, cnt - bc);
-        const int ninb = (nin + 3) / 4, t0 = bc + PIC_INBOX, ng = sG[c + 1] - sG[c];
+        const int nin = min(PIC_INBOX, cnt - bc), ninb = (nin + 3) / 4, t0 = bc + PIC_INBOX;
+        const int ng = sG[c + 1] - sG[c];
+        const int ps = sBase[c];
+        const int nsoa = packed ? (bc > 0 ? (ps + bc - 1) / GS32 - ps / GS32 + 1 : 0)
+                                : (bc + GS32 - 1) / GS32;
         for (int k = part; k < ng; k += 4) {
             int kind = 0, start, size;
             if (k < nsoa) {
-                start = GS32 * k;
-                size = min(GS32, bc - start);
+                if (packed) {
+                    const int lo = (ps / GS32 + k) * GS32 - ps;
+                    start = max(0, lo);
+                    size = min(bc, lo + GS32) - start;
+                } else {
+                    start = GS32 * k;
+                    size = min(GS32, bc - start);
+                }
             } else if (k < nsoa + ninb) {
                 kind = 1;
                 start = bc + 4 * (k - nsoa);
                 size = min(4, bc + nin - start);
+            } else if (packed) {
+                kind = 1;
+                start = t0 + GS32 * (k - nsoa - ninb);
+                size = min(GS32, cnt - start);
             } else {
                 const int line = t0 / GS32 + (k - nsoa - ninb);
                 start = max(t0, GS32 * line);
@@ -146,6 +172,7 @@ template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
+    constexpr int NW = PIC_BLOCK / 64;
     __shared__ float sF[6 * NB];
     /* J box copies JS elements apart: JS = 3 NB rounded to 8 mod 16, so
      * that copy 1 sits 16 (of 32) or 48 (of 64) dword banks from copy 0 and
@@ -156,10 +183,17 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
     __shared__ int sLeave[C];
-    __shared__ int sBase[C];     /* first slot of each cell, relative to the tile's first cell */
-    __shared__ int sBc[C];       /* slots of each cell held by the bucket arrays */
+    __shared__ int sBase[C];     /* first slot of each cell: relative to the tile's first
+                                    cell's bucket, or in the tile's region (packed) */
+    __shared__ int sBc[C];       /* slots of each cell held by the particle arrays */
     __shared__ int sIB[C];       /* first inbox record of each cell, relative to the tile's */
     __shared__ int sG[C + 1];    /* first line group of each cell */
+    __shared__ int sSpN[C];      /* packed: spill records of each cell */
+    __shared__ int sSpOff[C];    /* packed: first entry of each cell in sSpIdx */
+    __shared__ int sSpFill[C];
+    __shared__ unsigned char sSpC[PIC_SPILL_CAP];    /* cell of each spill record */
+    __shared__ unsigned char sSpIdx[PIC_SPILL_CAP];  /* spill records in cell order */
+    __shared__ int sFirst[NW + 1];  /* first cell of each wavefront */
     __shared__ int sHalo[NB];    /* leavers bound for each cell of the halo box */
     __shared__ int sQ;
     __shared__ unsigned sFlags;
@@ -170,6 +204,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     const Grid &g = a.g;
     const int tid = threadIdx.x;
     const bool snap = DIFF && a.snap_on;
+    const bool packed = a.L.packed != 0;
     long tile;
     {
         long nt = a.t.ntiles, b = blockIdx.x;
@@ -185,22 +220,27 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     const int L0 = __builtin_amdgcn_readfirstlane(ty * TY);
     const int K0 = __builtin_amdgcn_readfirstlane(tz * TZ);
     const int cap = a.p.cap;
+    const int nsp = packed ? min(a.L.spill_cnt[tile], a.L.scap) : 0;
+    const float4 *const SPB = a.L.spill + 2 * (tile * (long)a.L.scap);
 
     for (int c = tid; c < C; c += PIC_BLOCK) {
         int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
         int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        int cnt = 0, bc = 0;
+        int cnt = 0, bc = 0, ps = 0;
         if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
             const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
             cnt = a.p.count[cell];
             cnt = cnt < cap ? cnt : cap;
             bc = max(0, min(a.p.bcount[cell], cnt));
+            if (packed) ps = a.L.pstart[cell];
             if (snap) a.snap.count[cell] = cnt;
         }
         sCnt[c] = cnt;
         sBc[c] = bc;
         sLeave[c] = 0;
-        sBase[c] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
+        sSpN[c] = 0;
+        sSpFill[c] = 0;
+        sBase[c] = packed ? ps : ((ca * g.Ny + cb) * g.Nz + cc) * cap;
         sIB[c] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
     const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
@@ -226,15 +266,44 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sFlags = 0;
     }
     __syncthreads();
+    /* the tile's spill records counted by cell (a record whose cell lies
+     * outside the tile is not read) */
+    for (int k = tid; k < nsp; k += PIC_BLOCK) {
+        const int lc = key_halo(__float_as_uint(SPB[2 * k + 1].z));
+        if (lc < C) {
+            sSpC[k] = (unsigned char)lc;
+            atomicAdd(&sSpN[lc], 1);
+        } else {
+            sSpC[k] = 255;
+            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
+        }
+    }
+    __syncthreads();
     static_assert(C <= 64, "one wavefront scans the tile's cells");
-    if (tid < 64) {   /* first particle and first line group of each cell */
-        const int cnt = tid < C ? sCnt[tid] : 0;
-        const int ngc = tid < C ? cell_groups32(cnt, sBc[tid]) : 0;
-        int tot_s, tot_g;
+    if (tid < 64) {   /* first particle, line group and spill entry of each cell */
+        int cnt = tid < C ? sCnt[tid] : 0, ngc = 0, nspc = 0;
+        if (tid < C) {
+            const int bc = sBc[tid], nin = min(PIC_INBOX, cnt - bc);
+            if (packed) {
+                /* the cell's slots past the inbox are its spill records */
+                const int want = cnt - bc - nin;
+                nspc = min(sSpN[tid], want);
+                if (sSpN[tid] != want) atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
+                cnt = bc + nin + nspc;
+                sCnt[tid] = cnt;
+                sSpN[tid] = nspc;
+                ngc = cell_groups_packed(sBase[tid], bc, nin, nspc);
+            } else {
+                ngc = cell_groups32(cnt, bc);
+            }
+        }
+        int tot_s, tot_g, tot_p;
         const int so = wave_excl_scan(cnt, tot_s), sg = wave_excl_scan(ngc, tot_g);
+        const int spo = wave_excl_scan(nspc, tot_p);
         if (tid < C) {
             sOff[tid] = so;
             sG[tid] = sg;
+            sSpOff[tid] = spo;
         }
         if (tid == 0) {
             sOff[C] = tot_s;
@@ -242,34 +311,47 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         }
     }
     __syncthreads();
-    build_groups32<C>(sCnt, sBc, sG, sGrp, tid);
+    build_groups32<C>(sCnt, sBc, sBase, sG, packed, sGrp, tid);
+    if (tid <= NW) sFirst[tid] = wave_first_cell<C>(sOff, tid, NW);
+    for (int k = tid; k < nsp; k += PIC_BLOCK) {
+        const int lc = sSpC[k];
+        if (lc == 255) continue;
+        const int r = atomicAdd(&sSpFill[lc], 1);
+        if (r < sSpN[lc]) sSpIdx[sSpOff[lc] + r] = (unsigned char)k;
+    }
     __syncthreads();
 
     /* per-tile bases of the particle arrays: slot rel of the tile is
-     * cell-bucket slot pbase + rel */
-    const long pbase = (((long)I0 * g.Ny + L0) * g.Nz + K0) * cap;
-    float *const P0 = a.p.p[0] + pbase, *const P1 = a.p.p[1] + pbase,
-                 *const P2 = a.p.p[2] + pbase, *const P3 = a.p.p[3] + pbase,
-                 *const P4 = a.p.p[4] + pbase, *const P5 = a.p.p[5] + pbase;
-    unsigned char *const SP = a.p.sp + pbase;
-    int *const ID = DIFF ? a.p.id + pbase : nullptr;
+     * cell-bucket slot pbase + rel, or slot rel of the tile's region */
+    const long pbase = packed ? tile * a.L.tcap : (((long)I0 * g.Ny + L0) * g.Nz + K0) * cap;
+    const float *const P0 = a.p.p[0] + pbase, *const P1 = a.p.p[1] + pbase,
+                *const P2 = a.p.p[2] + pbase, *const P3 = a.p.p[3] + pbase,
+                *const P4 = a.p.p[4] + pbase, *const P5 = a.p.p[5] + pbase;
+    const unsigned char *const SP = a.p.sp + pbase;
+    const int *const ID = DIFF ? a.p.id + pbase : nullptr;
     const float4 *const IB = a.p.inbox + 2 * ((((long)I0 * g.Ny + L0) * g.Nz + K0) * PIC_INBOX);
+    const long obase = tile * a.L.tcap;
+    float *const O0 = a.o.p[0] + obase, *const O1 = a.o.p[1] + obase,
+                 *const O2 = a.o.p[2] + obase, *const O3 = a.o.p[3] + obase,
+                 *const O4 = a.o.p[4] + obase, *const O5 = a.o.p[5] + obase;
+    unsigned char *const OSP = a.o.sp + obase;
+    int *const OID = DIFF ? a.o.id + obase : nullptr;
 
     /* the wavefront's next chunk is composed and its particles loaded one
      * chunk ahead, so the memory latency overlaps this chunk's arithmetic */
     const int lane = tid & 63, wv = tid >> 6;
     int nfc = 0, nslot = 0;
-    /* the next chunk's particles, from the bucket arrays or from inbox
-     * records.  Both paths load into the same registers, so neither may
-     * compute anything into a register after the other path's loads were
-     * issued (that would wait for those loads): both addresses are formed
+    /* the next chunk's particles, from the particle arrays or from inbox or
+     * spill records.  The paths load into the same registers, so none may
+     * compute anything into a register after another path's loads were
+     * issued (that would wait for those loads): the addresses are formed
      * before the branch (the empty asm statements keep the compiler from
-     * sinking them into it), and the inbox path loads the species byte of
+     * sinking them into it), and the record path loads the species byte of
      * the record's key word by itself rather than masking the loaded word. */
     float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
     unsigned char qs = 0;
     int qid = 0;
-    bool qin = false;     /* the particle comes from an inbox record */
+    bool qin = false;     /* the particle comes from an inbox or spill record */
     auto fetch = [&](int g0, int o0, unsigned long long starts) {
         const unsigned long long m = starts & (~0ull >> (63 - lane));
         const int gi = __popcll(m) - 1;
@@ -278,7 +360,14 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         nfc = cf;
         nslot = group_start(e) + (lane - (63 - __clzll(m))) + (gi == 0 ? o0 : 0);
         unsigned nrel = (unsigned)(sBase[cf] + nslot);
-        unsigned ir = 2u * (unsigned)(sIB[cf] + nslot - sBc[cf]);
+        /* record r of the cell: inbox record r below PIC_INBOX, else spill
+         * entry r - PIC_INBOX of the cell (index clamped for the lanes that
+         * do not read one) */
+        const int r = nslot - sBc[cf];
+        const bool spl = r >= PIC_INBOX;
+        const int si = min(max(sSpOff[cf] + r - PIC_INBOX, 0), PIC_SPILL_CAP - 1);
+        unsigned ir = 2u * (unsigned)(spl ? (int)sSpIdx[si] : sIB[cf] + r);
+        const float4 *const RB = spl ? SPB : IB;
         asm volatile("" : "+v"(nrel));
         asm volatile("" : "+v"(ir));
         qin = group_kind(e) != 0;
@@ -288,29 +377,29 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             qs = slot(SP, nrel);
             if (DIFF) qid = slot(ID, nrel);
         } else {
-            const float4 A = slot(IB, ir);
-            const float2 B = *(const float2 *)&slot(IB, ir + 1u);
+            const float4 A = slot(RB, ir);
+            const float2 B = *(const float2 *)&slot(RB, ir + 1u);
             qox = A.x; qoy = A.y; qoz = A.z; qpu = A.w; qpv = B.x; qpw = B.y;
-            qs = *((const unsigned char *)&slot(IB, ir + 1u) + 8);   /* key & 255 */
-            if (DIFF) qid = *((const int *)&slot(IB, ir + 1u) + 3);
+            qs = *((const unsigned char *)&slot(RB, ir + 1u) + 8);   /* key & 255 */
+            if (DIFF) qid = *((const int *)&slot(RB, ir + 1u) + 3);
         }
     };
-    const int ghi = __builtin_amdgcn_readfirstlane(
-        sG[wave_first_cell<C>(sOff, wv + 1, PIC_BLOCK / 64)]);
-    int ng0 = __builtin_amdgcn_readfirstlane(sG[wave_first_cell<C>(sOff, wv, PIC_BLOCK / 64)]);
+    const int ghi = __builtin_amdgcn_readfirstlane(sG[sFirst[wv + 1]]);
+    int ng0 = __builtin_amdgcn_readfirstlane(sG[sFirst[wv]]);
+    /* the wavefront's stayers go to its region from the flat index of its
+     * first particle on, in flat order */
+    const int wout = __builtin_amdgcn_readfirstlane(sOff[sFirst[wv]]);
+    int wcarry = 0;
     unsigned long long nstarts = 0, ncstarts = 0;
     int nlanes = 0, no0 = 0, onext = 0;
     int gnext = compose_full_chunk(sGrp, ng0, no0, ghi, nstarts, nlanes, ncstarts, onext);
     if (lane < nlanes) fetch(ng0, no0, nstarts);
-    int carry_cell = -1, carry_cnt = 0;
     while (nlanes > 0) {
         const bool valid = lane < nlanes;
-        const unsigned long long runs = ncstarts;
         const int c = nfc, j = nslot;
         const float ox0 = qox, oy0 = qoy, oz0 = qoz, pu0 = qpu, pv0 = qpv, pw0 = qpw;
         const unsigned char s = qs;
         const int pid = DIFF ? qid : 0;
-        const bool fromin = qin;
         ng0 = gnext;
         no0 = onext;
         gnext = compose_full_chunk(sGrp, ng0, no0, ghi, nstarts, nlanes, ncstarts, onext);
@@ -319,7 +408,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
         if (valid) {
             const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-            const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
             const float ox = ox0, oy = oy0, oz = oz0;
             const long r = snap ? a.snap_off[tile] + sOff[c] + j : 0;
             if (snap && r < a.snap_room) {
@@ -399,36 +487,45 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             }
         }
 
-        /* stayers compacted in flat order: slot rank of the cell */
-        const int rank = stable_rank_runs(runs, c, valid, stays, carry_cell, carry_cnt);
+        /* stayers in flat order into the other set */
+        const unsigned long long S = __ballot(valid && stays);
         if (valid && stays) {
-            const unsigned dst = (unsigned)(sBase[c] + rank);
-            slot(P0, dst) = nxo;
-            slot(P1, dst) = nyo;
-            slot(P2, dst) = nzo;
-            slot(P3, dst) = pu;
-            slot(P4, dst) = pv;
-            slot(P5, dst) = pw;
-            /* species and id only move with the particle (or come from an
-             * inbox record) */
-            if (rank != j || fromin) {
-                slot(SP, dst) = s;
-                if (DIFF) slot(ID, dst) = pid;
-            }
+            const unsigned dst = (unsigned)(wout + wcarry + __popcll(S & ((1ull << lane) - 1ull)));
+            slot(O0, dst) = nxo;
+            slot(O1, dst) = nyo;
+            slot(O2, dst) = nzo;
+            slot(O3, dst) = pu;
+            slot(O4, dst) = pv;
+            slot(O5, dst) = pw;
+            slot(OSP, dst) = s;
+            if (DIFF) slot(OID, dst) = pid;
         }
+        wcarry += __popcll(S);
     }
     __syncthreads();
 
     int maxout = 0;
-    for (int c = tid; c < C; c += PIC_BLOCK) {
-        const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
-        const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-        const int m = sLeave[c];
-        a.p.count[cell] = sCnt[c] - m;
-        a.p.bcount[cell] = sCnt[c] - m;
-        if (m > maxout) maxout = m;
+    if (tid < 64) {   /* counts and packed starts of the tile's cells */
+        const int c = tid;
+        const int m = c < C ? sLeave[c] : 0;
+        const int nc = c < C ? sCnt[c] - m : 0;
+        int tot;
+        const int x = wave_excl_scan(nc, tot);
+        int w = 0;
+        for (int k = 1; k < NW; k++) w += c >= sFirst[k] ? 1 : 0;
+        const int f = min(sFirst[w], 63);
+        const int xf = __shfl(x, f);
+        if (c < C) {
+            const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
+            const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+            if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+                const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+                a.p.count[cell] = nc;
+                a.p.bcount[cell] = nc;
+                a.L.pstart[cell] = sOff[f] + x - xf;
+                maxout = m;
+            }
+        }
     }
 
     for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
@@ -455,6 +552,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             nq = a.q.qcap;
         }
         a.q.count[tile] = nq;
+        a.L.spill_cnt[tile] = 0;   /* read above; this step's arrivals refill it */
         if (sFlags) atomicOr(&a.st->flags, sFlags);
     }
 }
@@ -545,22 +643,39 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_settle32(long ncell, PartPtrs32 p
     if (r == 0) p.bcount[cell] = cnt;
 }
 
+/* tile and local cell of bulk cell `cell` */
+template <int TX, int TY, int TZ>
+__device__ __forceinline__ void cell_tile(const Grid &g, const TileGeom &t, long cell, long &tile,
+                                          int &lc) {
+    const int gi = (int)(cell / ((long)g.Ny * g.Nz)), gl = (int)((cell / g.Nz) % g.Ny),
+              gk = (int)(cell % g.Nz);
+    tile = ((long)(gi / TX) * t.nty + gl / TY) * t.ntz + gk / TZ;
+    lc = ((gi % TX) * TY + gl % TY) * TZ + gk % TZ;
+}
+
 /* Arrival of the FP32 departures (GPU_ArrangeFlights / Cell::Insert,
  * kernels.cu:316-370), one workgroup per source tile: one claim per
  * destination cell on its count (the push counted the tile's leavers per
- * halo cell and ranked them), then each record takes slot s = base + rank:
- * inbox record s - bcount while that is below PIC_INBOX (one 32-byte
- * write), else bucket slot s.  A slot past the capacity is kept in q.slot
- * for the regrow path and raises PIC_F_BUCKET_OVERFLOW. */
+ * halo cell and ranked them), then each record takes slot s = base + rank,
+ * r = s - bcount past the cell's particle-array part: inbox record r while
+ * r < PIC_INBOX (one 32-byte write), else the next record of the
+ * destination tile's spill list, its key holding the local cell in place
+ * of the halo index (bucket slot s when the live set holds buckets, which
+ * no push leaves behind).  A full spill list raises PIC_F_SPILL_OVER and a slot
+ * past the capacity PIC_F_BUCKET_OVERFLOW: the host then places these from
+ * the queue (q.slot keeps every slot). */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, PartPtrs32 p,
-                                                       QueuePtrs32 q, DevStatus *st) {
+                                                       Layout32 L, QueuePtrs32 q, DevStatus *st) {
     constexpr int NH = (TX + 2) * (TY + 2) * (TZ + 2);
     static_assert(NH <= 256, "a record key holds the halo index in 8 bits");
+    static_assert(TX * TY * TZ <= 256, "a spill record key holds the local cell in 8 bits");
     __shared__ int hn[NH];
     __shared__ int hbase[NH];
     __shared__ int hbc[NH];
     __shared__ long hcell[NH];
+    __shared__ long htile[NH];
+    __shared__ int hlc[NH];
     const long tile = blockIdx.x;
     const int tid = threadIdx.x;
     int n = q.count[tile];
@@ -578,15 +693,22 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
         hbase[h] = -1;
         hbc[h] = 0;
         hcell[h] = 0;
+        htile[h] = 0;
+        hlc[h] = 0;
         if (c > 0) {
             const long cell = halo_cell<TX, TY, TZ>(g, t, tile, h);
             hcell[h] = cell;
             hbc[h] = p.bcount[cell];
             hbase[h] = atomicAdd(&p.count[cell], c);
+            long dt;
+            int lc;
+            cell_tile<TX, TY, TZ>(g, t, cell, dt, lc);
+            htile[h] = dt;
+            hlc[h] = lc;
         }
     }
     __syncthreads();
-    bool over = false, bad = false;
+    bool over = false, bad = false, spill_over = false;
     for (int j = tid; j < n; j += PIC_BLOCK) {
         float4 A = A0, B = B0;
         if (j != tid) {
@@ -602,16 +724,114 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, Part
             bad |= verdict == PIC_ARR_BAD;
             continue;
         }
-        const long cell = hcell[key_halo(key)];
+        const int h = key_halo(key);
+        const long cell = hcell[h];
         if (r < PIC_INBOX) {
             p.inbox[2 * (cell * PIC_INBOX + r)] = A;
             p.inbox[2 * (cell * PIC_INBOX + r) + 1] = B;
+        } else if (!L.packed) {
+            place32(p, cell, slot, A, B, key);     /* bucket slot past the inbox */
         } else {
-            place32(p, cell, slot, A, B, key);
+            const long dt = htile[h];
+            const int k = atomicAdd(&L.spill_cnt[dt], 1);
+            if (k < L.scap) {
+                B.z = __uint_as_float((key & 255u) | ((unsigned)hlc[h] << 8));
+                float4 *d = L.spill + 2 * (dt * L.scap + k);
+                d[0] = A;
+                d[1] = B;
+            } else {
+                spill_over = true;
+            }
         }
     }
     if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
     if (bad) atomicOr(&st->flags, (unsigned)PIC_F_QUEUE_CORRUPT);
+    if (spill_over) atomicOr(&st->flags, (unsigned)PIC_F_SPILL_OVER);
+}
+
+/* the packed live set into buckets: slot j of a cell from its tile's region
+ * (j < bcount) or from inbox record j - bcount */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_unpack32(Grid g, TileGeom t, long ncell,
+                                                       PartPtrs32 p, Layout32 L, PartPtrs32 to) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= ncell * p.cap) return;
+    const long cell = e / p.cap;
+    const int j = (int)(e % p.cap);
+    const int cnt = min(p.count[cell], p.cap), bc = max(0, min(p.bcount[cell], cnt));
+    const int nin = min(PIC_INBOX, cnt - bc);
+    if (j < bc) {
+        long tile;
+        int lc;
+        cell_tile<TX, TY, TZ>(g, t, cell, tile, lc);
+        const long src = tile * L.tcap + L.pstart[cell] + j;
+        for (int k = 0; k < 6; k++) to.p[k][e] = p.p[k][src];
+        to.sp[e] = p.sp[src];
+        if (to.id) to.id[e] = p.id[src];
+    } else if (j < bc + nin) {
+        const long rec = 2 * (cell * PIC_INBOX + (j - bc));
+        const float4 A = p.inbox[rec], B = p.inbox[rec + 1];
+        place32(to, cell, j, A, B, __float_as_uint(B.z));
+    }
+}
+
+/* the spill lists into buckets, one workgroup per tile: record k of cell
+ * c takes slot bcount + PIC_INBOX + (its rank among c's records) */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_unspill32(Grid g, TileGeom t, PartPtrs32 p,
+                                                        Layout32 L, PartPtrs32 to) {
+    constexpr int C = TX * TY * TZ;
+    __shared__ int fill[C];
+    const long tile = blockIdx.x;
+    const int n = min(L.spill_cnt[tile], L.scap);
+    for (int c = threadIdx.x; c < C; c += PIC_BLOCK) fill[c] = 0;
+    __syncthreads();
+    const int tz = (int)(tile % t.ntz), ty = (int)((tile / t.ntz) % t.nty),
+              tx = (int)(tile / ((long)t.ntz * t.nty));
+    for (int k = threadIdx.x; k < n; k += PIC_BLOCK) {
+        const float4 A = L.spill[2 * (tile * L.scap + k)], B = L.spill[2 * (tile * L.scap + k) + 1];
+        const unsigned key = __float_as_uint(B.z);
+        const int lc = key_halo(key);
+        if (lc >= C) continue;
+        const int gi = tx * TX + lc / (TY * TZ), gl = ty * TY + (lc / TZ) % TY,
+                  gk = tz * TZ + lc % TZ;
+        if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
+        const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+        const int cnt = min(p.count[cell], p.cap), bc = max(0, min(p.bcount[cell], cnt));
+        const int slot = bc + PIC_INBOX + atomicAdd(&fill[lc], 1);
+        if (slot < cnt) place32(to, cell, slot, A, B, key);
+    }
+}
+
+/* after an unpack: every slot is in the bucket arrays, the spill lists are
+ * empty */
+__global__ __launch_bounds__(PIC_BLOCK) void k_unpack_done32(long ncell, long ntiles,
+                                                            PartPtrs32 p, Layout32 L) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e < ncell) p.bcount[e] = min(p.count[e], p.cap);
+    if (e < ntiles) L.spill_cnt[e] = 0;
+}
+
+__global__ void k_clear_flags32(DevStatus *st, unsigned mask) { st->flags &= ~mask; }
+
+/* the arrivals past the inbox (r >= PIC_INBOX, below the capacity) from
+ * the departure queues into bucket slots: the repair after a spill list
+ * ran full, which reads every such arrival from its queue record */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_place_spills32(Grid g, TileGeom t, PartPtrs32 p,
+                                                             QueuePtrs32 q, PartPtrs32 to) {
+    const long tile = blockIdx.x;
+    int n = q.count[tile];
+    n = n < q.qcap ? n : q.qcap;
+    const long qb = tile * (long)q.qcap;
+    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
+        const int slot = q.slot[qb + j];
+        if (slot < 0 || slot >= to.cap) continue;
+        const float4 A = q.e[2 * (qb + j)], B = q.e[2 * (qb + j) + 1];
+        const unsigned key = __float_as_uint(B.z);
+        const long cell = halo_cell<TX, TY, TZ>(g, t, tile, key_halo(key));
+        if (slot - p.bcount[cell] >= PIC_INBOX) place32(to, cell, slot, A, B, key);
+    }
 }
 
 /* regrow: the arrivals whose claimed slot lay past the old capacity (a
@@ -746,14 +966,16 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 }  // namespace
 
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
-                             QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             const PartPtrs32 *diff, const long *snap_off, long snap_room,
-                             hipStream_t s) {
+                             PartPtrs32 out, Layout32 L, QueuePtrs32 q, SpeciesTable sp,
+                             int dep_cap, DevStatus *st, const PartPtrs32 *diff,
+                             const long *snap_off, long snap_room, hipStream_t s) {
     PushArgs32 a;
     a.g = g;
     a.t = t;
     a.f = f;
     a.p = p;
+    a.o = out;
+    a.L = L;
     a.q = q;
     const double h[3] = {g.hx, g.hy, g.hz};
     for (int k = 0; k < PIC_MAX_SLOTS; k++) {
@@ -771,9 +993,12 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.snap_on = diff && diff->p[0] != nullptr;
     a.snap_off = snap_off;
     a.snap_room = snap_room;
-    /* dynamic LDS: one 4-byte entry per group (bucket lines before and after
-     * the inbox, inbox lines) */
-    const size_t lds = (size_t)4 * TXC * TYC * TZC * (2 * ((p.cap + GS32 - 1) / GS32) + 6);
+    /* dynamic LDS: one 4-byte entry per group (per cell: particle-array
+     * lines before and after the inbox, one more for a packed part that
+     * starts inside a line, inbox lines; per tile: spill groups) */
+    const size_t lds = (size_t)4 * (TXC * TYC * TZC * (2 * ((p.cap + GS32 - 1) / GS32) +
+                                                     PIC_INBOX / 4 + 2) +
+                                    PIC_SPILL_CAP / GS32 + 1);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), lds, s, a);
@@ -787,10 +1012,34 @@ void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s) {
                        ncell, p);
 }
 
-void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
-                       DevStatus *st, hipStream_t s) {
+void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                       QueuePtrs32 q, DevStatus *st, hipStream_t s) {
     hipLaunchKernelGGL((k_arrive32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0,
-                       s, g, t, p, q, st);
+                       s, g, t, p, L, q, st);
+}
+
+void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
+                     bool with_spill, hipStream_t s) {
+    const long ncell = (long)g.Nx * g.Ny * g.Nz;
+    to.cap = p.cap;
+    hipLaunchKernelGGL((k_unpack32<TXC, TYC, TZC>), dim3(blocks_for(ncell * p.cap)),
+                       dim3(PIC_BLOCK), 0, s, g, t, ncell, p, L, to);
+    if (with_spill)
+        hipLaunchKernelGGL((k_unspill32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), 0, s, g, t, p, L, to);
+    hipLaunchKernelGGL(k_unpack_done32, dim3(blocks_for(std::max(ncell, t.ntiles))),
+                       dim3(PIC_BLOCK), 0, s, ncell, t.ntiles, p, L);
+}
+
+void launch_clear_flags32(DevStatus *st, unsigned mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_flags32, dim3(1), dim3(1), 0, s, st, mask);
+}
+
+void launch_place_spills32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
+                           PartPtrs32 to, hipStream_t s) {
+    to.cap = p.cap;
+    hipLaunchKernelGGL((k_place_spills32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                       dim3(PIC_BLOCK), 0, s, g, t, p, q, to);
 }
 
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],

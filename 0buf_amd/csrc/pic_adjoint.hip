@@ -183,6 +183,8 @@ int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
         launch_scatter_load32(src.off, c->NC, v, src.sp, c->p32, c->stream);   /* bcount = count */
         launch_scatter_ids32(src.off, c->NC, src.id, c->p32, c->stream);
         c->unsettled = false;
+        c->lay32.packed = 0;           /* buckets; the next push ignores the spill lists */
+        c->arrival_pending = false;
     } else {
         const double *v[6];
         for (int j = 0; j < 6; j++) v[j] = (const double *)src.v[j];
@@ -222,6 +224,22 @@ int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
     c->rec_off = nullptr;
     c->rec_room = 0;
     enq_arrive(c);
+    if (c->fp32) {
+        /* the status is checked at the segment's end; only where the first
+         * forward needed the repair of a full spill list is each step's
+         * arrival examined (and repaired) here */
+        c->arrival_pending = false;
+        if (c->spill_repairs > 0) {
+            HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost,
+                                  c->stream));
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->lay32.packed && (c->h_st->flags & PIC_F_SPILL_OVER)) {
+                int rc = despill(c);
+                if (rc) return rc;
+                launch_clear_flags32(c->d_st, PIC_F_SPILL_OVER, c->stream);
+            }
+        }
+    }
     enq_fold(c);
     enq_h2(c);
     enq_e(c);

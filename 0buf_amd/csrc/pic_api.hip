@@ -70,6 +70,50 @@ template void free_parts<PartPtrs>(PartPtrs &);
 template int alloc_parts<PartPtrs32>(PartPtrs32 &, long, int, bool, bool);
 template void free_parts<PartPtrs32>(PartPtrs32 &);
 
+/* FP32 form: slots of one particle set, enough for the buckets (cell * cap)
+ * and for the packed tile regions (tile * tcap) */
+long live_slots32(const pic_ctx *c) {
+    return std::max(c->NC, c->t.ntiles * (long)(c->t.TX * c->t.TY * c->t.TZ));
+}
+
+/* FP32 form: the live set p (per-cell count, bcount and inbox included)
+ * and the other set the push writes, both with capacity cap; the packed
+ * layout's per-cell starts and per-tile spill lists on first use */
+int alloc_live32(pic_ctx *c, PartPtrs32 &p, int cap, bool with_count) {
+    const long ns = live_slots32(c);
+    int rc = alloc_parts(p, ns, cap, false, c->diff);
+    if (rc) return rc;
+    if (with_count) HIPCHK(hipMalloc(&p.count, c->NC * sizeof(int)));
+    if ((rc = alloc_inbox(p, c->NC))) return rc;
+    free_parts(c->alt32);
+    if ((rc = alloc_parts(c->alt32, ns, cap, false, c->diff))) return rc;
+    Layout32 &L = c->lay32;
+    if (!L.pstart) {
+        HIPCHK(hipMalloc(&L.pstart, c->NC * sizeof(int)));
+        L.scap = PIC_SPILL_CAP;
+        HIPCHK(hipMalloc(&L.spill, (size_t)c->t.ntiles * L.scap * 2 * sizeof(float4)));
+        HIPCHK(hipMalloc(&L.spill_cnt, c->t.ntiles * sizeof(int)));
+        HIPCHK(hipMemset(L.spill_cnt, 0, c->t.ntiles * sizeof(int)));
+    }
+    L.tcap = (long)c->t.TX * c->t.TY * c->t.TZ * cap;
+    L.packed = 0;
+    return PIC_SUCCESS;
+}
+
+void free_layout32(Layout32 &L) {
+    if (L.pstart) (void)hipFree(L.pstart);
+    if (L.spill) (void)hipFree(L.spill);
+    if (L.spill_cnt) (void)hipFree(L.spill_cnt);
+    L = Layout32{};
+}
+
+/* the live and the other FP32 set trade places (per-cell arrays stay) */
+void swap_sets32(pic_ctx *c) {
+    for (int k = 0; k < 6; k++) std::swap(c->p32.p[k], c->alt32.p[k]);
+    std::swap(c->p32.sp, c->alt32.sp);
+    std::swap(c->p32.id, c->alt32.id);
+}
+
 template <class Q>
 void free_queue(Q &q) {
     for (int k = 0; k < 6; k++)
@@ -192,11 +236,10 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
             if (c->diff) sid[d] = c->staged.id[j];
         }
     }
-    int rc = alloc_parts(parts, NC, cap, true, c->diff);
+    int rc;
+    if constexpr (std::is_same<P, PartPtrs32>::value) rc = alloc_live32(c, parts, cap, true);
+    else rc = alloc_parts(parts, NC, cap, true, c->diff);
     if (rc) return rc;
-    if constexpr (std::is_same<P, PartPtrs32>::value) {
-        if ((rc = alloc_inbox(parts, NC))) return rc;
-    }
     rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
     long *d_off = nullptr;
@@ -294,14 +337,23 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
                              (oldcap + 8 + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN);
     if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     P np;
-    int rc = alloc_parts(np, NC, cap, false, c->diff);
+    int rc;
+    if constexpr (std::is_same<P, PartPtrs32>::value) {
+        /* the buckets only (finish_step unpacks a packed set first) */
+        if (c->lay32.packed) {
+            snprintf(g_hip_err, sizeof(g_hip_err), "regrow of a packed particle set");
+            return PIC_ERR_STATE;
+        }
+        rc = alloc_live32(c, np, cap, false);
+    } else {
+        rc = alloc_parts(np, NC, cap, false, c->diff);
+    }
     if (rc) return rc;
     np.count = parts.count;
     if constexpr (std::is_same<P, PartPtrs32>::value) {
         /* the inbox records (all below the old capacity) into the old
          * bucket arrays, then the bucket arrays into the grown ones */
         launch_settle32(NC, parts, c->stream);
-        if ((rc = alloc_inbox(np, NC))) return rc;
         launch_repack32(NC, parts, np, c->stream);
         launch_fix_arrivals32(c->g, c->t, queue, oldcap, np, c->stream);
         HIPCHK(hipMemcpyAsync(np.bcount, np.count, NC * sizeof(int), hipMemcpyDeviceToDevice,
@@ -334,10 +386,42 @@ int regrow(pic_ctx *c) {
 int *count_ptr(pic_ctx *c) { return c->fp32 ? c->p32.count : c->p.count; }
 int bucket_cap(pic_ctx *c) { return c->fp32 ? c->p32.cap : c->p.cap; }
 
-int settle(pic_ctx *c) {
-    if (!c->fp32 || !c->unsettled || !c->p32.inbox) return PIC_SUCCESS;
+int settle_noalloc(pic_ctx *c) {
+    if (!c->fp32 || !c->p32.inbox) return PIC_SUCCESS;
+    if (c->arrival_pending) {
+        snprintf(g_hip_err, sizeof(g_hip_err),
+                 "the last arrival's status is not examined yet (pic_sync first)");
+        return PIC_ERR_STATE;
+    }
+    if (c->lay32.packed) {
+        launch_unpack32(c->g, c->t, c->p32, c->lay32, c->alt32, true, c->stream);
+        HIPCHK(hipGetLastError());
+        swap_sets32(c);
+        c->lay32.packed = 0;
+        c->unsettled = false;
+        return PIC_SUCCESS;
+    }
+    if (!c->unsettled) return PIC_SUCCESS;
     launch_settle32(c->NC, c->p32, c->stream);
     HIPCHK(hipGetLastError());
+    c->unsettled = false;
+    return PIC_SUCCESS;
+}
+
+int settle(pic_ctx *c) {
+    if (c->fp32 && c->arrival_pending) {
+        const int rc = finish_step(c);
+        if (rc) return rc;
+    }
+    return settle_noalloc(c);
+}
+
+int despill(pic_ctx *c) {
+    launch_place_spills32(c->g, c->t, c->p32, c->q32, c->alt32, c->stream);
+    launch_unpack32(c->g, c->t, c->p32, c->lay32, c->alt32, false, c->stream);
+    HIPCHK(hipGetLastError());
+    swap_sets32(c);
+    c->lay32.packed = 0;
     c->unsettled = false;
     return PIC_SUCCESS;
 }
@@ -350,9 +434,13 @@ void enq_h_half(pic_ctx *c) {
 }
 void enq_push(pic_ctx *c) {
     if (c->fp32) {
-        launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->q32, c->sp, c->prm.departure_cap,
-                                c->d_st, c->diff ? &c->rec_snap32 : nullptr, c->rec_off,
-                                c->rec_room, c->stream);
+        launch_push_deposit_f32(c->g, c->t, c->f32, c->p32, c->alt32, c->lay32, c->q32, c->sp,
+                                c->prm.departure_cap, c->d_st,
+                                c->diff ? &c->rec_snap32 : nullptr, c->rec_off, c->rec_room,
+                                c->stream);
+        swap_sets32(c);
+        c->lay32.packed = 1;
+        c->unsettled = false;
     } else if (c->diff) {
         DiffArgs d{};
         for (int k = 0; k < 3; k++) d.jfix[k] = c->d_jfix[k];
@@ -375,8 +463,9 @@ void enq_push(pic_ctx *c) {
 }
 void enq_arrive(pic_ctx *c, hipStream_t st) {
     if (c->fp32) {
-        launch_arrive_f32(c->g, c->t, c->p32, c->q32, c->d_st, st);
-        c->unsettled = true;
+        launch_arrive_f32(c->g, c->t, c->p32, c->lay32, c->q32, c->d_st, st);
+        c->unsettled = !c->lay32.packed;
+        c->arrival_pending = true;
     }
     else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, st);
 }
@@ -468,13 +557,26 @@ int finish_step(pic_ctx *c) {
     HIPCHK(hipMemcpyAsync(c->h_st, c->d_st, sizeof(DevStatus), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     const DevStatus st = *c->h_st;
+    const bool arrived = c->arrival_pending;
+    c->arrival_pending = false;
     c->last.dropped = (long long)st.dropped;
     c->last.duplicated = (long long)st.duplicated;
     c->last.max_cell_departures = st.max_dep;
-    c->last.flags = (int)st.flags;
+    c->last.flags = (int)(st.flags & ~(unsigned)PIC_F_SPILL_OVER);
     c->last.bucket_capacity = bucket_cap(c);
     c->last.bucket_regrows = c->regrows;
+    c->last.spill_repairs = (int)c->spill_repairs;
     if (st.flags & PIC_F_QUEUE_CORRUPT) return PIC_ERR_QUEUE_CORRUPT;
+    /* FP32 form: arrivals that found no room in their tile's spill list or
+     * past the bucket capacity are still in the departure queues: every
+     * particle into buckets (then the regrow, for the latter) */
+    if (c->fp32 && arrived && c->lay32.packed &&
+        (st.flags & (PIC_F_SPILL_OVER | PIC_F_BUCKET_OVERFLOW))) {
+        int rc = despill(c);
+        if (rc) return rc;
+        if (st.flags & PIC_F_SPILL_OVER) c->spill_repairs++;
+        c->last.spill_repairs = (int)c->spill_repairs;
+    }
     if (st.flags & PIC_F_BUCKET_OVERFLOW) {
         int rc = regrow(c);
         if (rc) return rc;
@@ -659,6 +761,8 @@ int pic_destroy(pic_ctx *c) {
     free_parts(c->p);
     free_queue(c->q);
     free_parts(c->p32);
+    free_parts(c->alt32);
+    free_layout32(c->lay32);
     free_queue(c->q32);
     if (c->d_rho) (void)hipFree(c->d_rho);
     if (c->d_gres) (void)hipFree(c->d_gres);
@@ -767,10 +871,9 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     if (maxc > cap) cap = auto_capacity(maxc);
     if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(c->dev));
-    int rc = c->fp32 ? alloc_parts(c->p32, NC, cap, true, c->diff)
+    int rc = c->fp32 ? alloc_live32(c, c->p32, cap, true)
                      : alloc_parts(c->p, NC, cap, true, c->diff);
     if (rc) return rc;
-    if (c->fp32 && (rc = alloc_inbox(c->p32, NC))) return rc;
     rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), c->diff)
                  : alloc_queue(c->q, c->t, queue_capacity(c->t, cap), c->diff);
     if (rc) return rc;
@@ -892,6 +995,9 @@ int pic_yee_h_half(pic_ctx *c) {
 int pic_push_deposit(pic_ctx *c) {
     int rc = check_ready(c);
     if (rc) return rc;
+    /* an arrival whose status was not examined (no pic_sync since): its
+     * repair comes first */
+    if (c->arrival_pending && (rc = finish_step(c))) return rc;
     enq_push(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
@@ -954,6 +1060,7 @@ int pic_step(pic_ctx *c, int nt, double *energy) {
     int rc = check_ready(c);
     if (rc) return rc;
     HIPCHK(hipSetDevice(c->dev));
+    if (c->arrival_pending && (rc = finish_step(c))) return rc;
     rc = enq_step(c);
     if (rc) return rc;
     rc = finish_step(c);

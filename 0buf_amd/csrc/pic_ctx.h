@@ -43,7 +43,9 @@ struct pic_ctx {
     PartPtrs p{};
     QueuePtrs q{};
     FieldPtrs32 f32{};      /* FP32 precision form */
-    PartPtrs32 p32{};
+    PartPtrs32 p32{};       /* the live set (count, bcount, inbox: per cell) */
+    PartPtrs32 alt32{};     /* the other set the push writes (p, sp, id) */
+    Layout32 lay32{};       /* packed layout state; lay32.packed: p32 holds it */
     QueuePtrs32 q32{};
     long long *d_rho = nullptr;   /* Gauss residual scratch */
     double *d_gres = nullptr;
@@ -79,6 +81,11 @@ struct pic_ctx {
     Staged staged;
     bool committed = false;
     bool unsettled = false;   /* FP32 form: inbox records not yet in the bucket arrays */
+    /* FP32 form: an arrival ran whose status no host call has examined yet
+     * (a full spill list or bucket needs the host's repair before the next
+     * push or any reader) */
+    bool arrival_pending = false;
+    long spill_repairs = 0;   /* steps whose spill list ran full (repaired) */
     int regrows = 0;
     pic_step_status last{};
     /* stage timing */
@@ -95,9 +102,17 @@ int commit(pic_ctx *c);
 int check_ready(pic_ctx *c);
 int enq_step(pic_ctx *c);
 int finish_step(pic_ctx *c);
-/* FP32 form: move the inbox records into the bucket arrays before anything
- * other than the push reads the buckets (no-op otherwise) */
+/* FP32 form: put every particle into the bucket arrays (unpack the packed
+ * set, or move the inbox records) before anything other than the push reads
+ * the buckets (no-op otherwise); examines a pending arrival's status first */
 int settle(pic_ctx *c);
+/* the same without any allocation: PIC_ERR_STATE while an arrival's status
+ * is pending (its repair may allocate) */
+int settle_noalloc(pic_ctx *c);
+/* FP32 form: the arrival's status after a full spill list or bucket: every
+ * particle into the bucket arrays of the other set, from the packed set, its
+ * inbox records and the departure queues (the set becomes the live one) */
+int despill(pic_ctx *c);
 template <class P>
 int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id);
 template <class P>

@@ -9,6 +9,8 @@
  *   particles  bucketed SoA: bulk cell c = (i*ny + l)*nz + k owns slots
  *           [c*cap, c*cap + count[c]) of x, y, z, px, py, pz (FP64) and a
  *           one-byte species slot; per-species weight and q/m live in a table;
+ *           the FP32 form steps between two sets in a packed per-tile
+ *           layout (Layout32) and unpacks into buckets for other readers;
  *   departures  one fixed region of qcap entries per tile of cells.
  */
 #ifndef PIC_INTERNAL_H
@@ -30,7 +32,9 @@ enum {
     PIC_F_RANGE = 8,
     PIC_F_RECOMPUTE = 16,    /* differentiable form: recompute differs from the first run */
     PIC_F_RECORD = 32,       /* differentiable form: a step record ran out of room */
-    PIC_F_QUEUE_CORRUPT = 64 /* an arrival refused a departure record out of range */
+    PIC_F_QUEUE_CORRUPT = 64, /* an arrival refused a departure record out of range */
+    PIC_F_SPILL_OVER = 128    /* FP32 form: a tile's spill list was full (host repairs;
+                                 not a status the caller sees) */
 };
 
 struct DevStatus {
@@ -76,17 +80,18 @@ struct PartPtrs {
 /* particle storage of the FP32 form: x, y, z are offsets in [0,1) inside the
  * owning cell (cell units), px, py, pz the momenta.
  *
- * The live buckets also have an inbox per cell: PIC_INBOX records of 32
+ * The live particles also have an inbox per cell: PIC_INBOX records of 32
  * bytes (the departure-queue record format) where the arrival kernel puts
  * the particles that joined the cell this step, one 32-byte write each
  * instead of seven scattered ones.  count is the cell's total; slots
- * [0, bcount) are in the bucket arrays, slots bcount + r for r < PIC_INBOX
+ * [0, bcount) are in the particle arrays, slots bcount + r for r < PIC_INBOX
  * are inbox record r, and slots from bcount + PIC_INBOX on (arrivals past a
- * full inbox) are in the bucket arrays again.  The next push reads the
- * cell in slot order and compacts it into the bucket arrays; the settle
- * kernel does the same for every other reader.  Checkpoints and staging
- * copies have no inbox (bcount, inbox null). */
-constexpr int PIC_INBOX = 16;
+ * full inbox) are in the spill list of the cell's tile while the arrays are
+ * packed (Layout32 below), in the bucket arrays otherwise.  The next push
+ * reads the cell in slot order; the unpack and settle kernels put every
+ * slot into the bucket arrays for the other readers.  Checkpoints and
+ * staging copies have no inbox (bcount, inbox null). */
+constexpr int PIC_INBOX = 32;
 struct PartPtrs32 {
     float *p[6];
     unsigned char *sp;
@@ -96,6 +101,29 @@ struct PartPtrs32 {
     int *bcount;             /* live buckets: slots held by the bucket arrays */
     float4 *inbox;           /* live buckets: PIC_INBOX records (2 float4) per cell */
 };
+
+/* The packed layout of the live FP32 particles.  The push reads the live
+ * set and writes its stayers into the other set, packed: tile t owns slots
+ * [t*tcap, (t+1)*tcap) (tcap = cells per tile x cap), and each wavefront of
+ * the tile's workgroup writes the stayers of its cells contiguously in flat
+ * order from the flat index of its first particle, so a cell's particles
+ * lie at pstart[cell] + [0, bcount[cell]) of its tile's region and share
+ * lines with their neighbours (no partial line per cell and array).  The
+ * arrivals of the step go to the inbox records of their cell (the first
+ * PIC_INBOX) and past those to the spill list of the destination tile
+ * (scap records of the departure-record format, key = species | local cell
+ * << 8), which the next push reads with the cell's packed part and inbox.
+ * The host turns the packed set back into buckets (unpack) for every other
+ * reader, and when a spill list overflows. */
+struct Layout32 {
+    int packed;              /* the set being read is packed (else buckets) */
+    int *pstart;             /* per cell: first slot in its tile's region */
+    float4 *spill;           /* per tile: scap records (2 float4 each) */
+    int *spill_cnt;          /* per tile: records claimed (may exceed scap) */
+    int scap;
+    long tcap;               /* slots per tile region */
+};
+constexpr int PIC_SPILL_CAP = 256;  /* spill records per tile (indices kept in 8 bits) */
 
 /* FP32 departure queue: one 32-byte record per leaver, two float4
  * (ox, oy, oz, pu) and (pv, pw, key, id), with key = species | h << 8 |
@@ -396,14 +424,27 @@ void launch_moments(long ncell, PartPtrs p, SpeciesTable sp, const int *slot_sor
 /* diff: differentiable form (ids; compact record of the pre-step state into
  * *diff at snap_off[tile] + flat index when diff->p[0] is set), null
  * otherwise */
+/* p: the live set (read; packed or buckets as L.packed says), out: the
+ * other set, written packed (p[6], sp, id) */
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
-                             QueuePtrs32 q, SpeciesTable sp, int dep_cap, DevStatus *st,
-                             const PartPtrs32 *diff, const long *snap_off, long snap_room,
-                             hipStream_t s);
+                             PartPtrs32 out, Layout32 L, QueuePtrs32 q, SpeciesTable sp,
+                             int dep_cap, DevStatus *st, const PartPtrs32 *diff,
+                             const long *snap_off, long snap_room, hipStream_t s);
 /* inbox records into the bucket arrays (bcount = count, at most cap) */
 void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s);
-void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
-                       DevStatus *st, hipStream_t s);
+void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                       QueuePtrs32 q, DevStatus *st, hipStream_t s);
+/* the packed live set p (with its inbox records, and its spill lists when
+ * with_spill) into buckets in `to` (p[6], sp, id; the counts stay those of
+ * p); bcount = count (at most cap) and the spill lists emptied after */
+void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
+                     bool with_spill, hipStream_t s);
+/* the arrivals past the inbox, from the departure queues, into bucket slots
+ * of `to` (the repair after a full spill list; before launch_unpack32 with
+ * with_spill false, which reads the same bcount) */
+void launch_place_spills32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
+                           PartPtrs32 to, hipStream_t s);
+void launch_clear_flags32(DevStatus *st, unsigned mask, hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
 void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);

@@ -82,7 +82,9 @@ extern "C" int pic_moments_noalloc(pic_ctx *c, int nblocks, double *d_out27) {
                  c->dev);
         return PIC_ERR_STATE;
     }
-    int rc = settle(c);
+    /* the FP32 form's packed particles are unpacked into the other set (no
+     * allocation); a pending arrival's repair could allocate: refused */
+    int rc = settle_noalloc(c);
     if (rc) return rc;
     if (c->fp32)
         launch_moments32(c->g, c->NC, c->p32, c->sp, c->d_slot_sort, partials, nblocks, d_out27,

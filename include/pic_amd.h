@@ -138,6 +138,8 @@ typedef struct pic_step_status {
     int flags;                /* bit0 too many, bit1 queue overflow,
                                  bit2 bucket overflow (repaired), bit3 range,
                                  bit6 departure record refused */
+    int spill_repairs;        /* FP32 form: steps so far whose arrivals overflowed a
+                                 tile's spill list (repaired by the host) */
 } pic_step_status;
 
 void pic_params_default(pic_params *p);
