@@ -20,7 +20,7 @@
  *   * migration by the cell-local offset: a particle whose offset leaves
  *     [0,1) moves to the neighbouring cell (periodic), exactly one receiver.
  */
-#include "k_arrive.h"
+#include "pic_internal.h"
 #include "k_push_math32.h"
 #include "k_queue_key.h"
 
@@ -520,9 +520,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
             if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
                 const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-                a.p.count[cell] = nc;
+                a.p.count[cell] = nc;       /* the arrivals are added after the push */
                 a.p.bcount[cell] = nc;
                 a.L.pstart[cell] = sOff[f] + x - xf;
+                a.L.arr[cell] = 0;          /* read above; refilled two steps on */
                 maxout = m;
             }
         }
@@ -543,16 +544,70 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         if (a.dep_cap > 0 && maxout > a.dep_cap) atomicOr(&sFlags, (unsigned)PIC_F_TOO_MANY);
         if (maxout > a.st->max_dep) atomicMax(&a.st->max_dep, maxout);
     }
-    for (int h = tid; h < NB; h += PIC_BLOCK) a.q.hcnt[tile * NB + h] = sHalo[h];
+
+    /* The arrival of the tile's leavers (GPU_ArrangeFlights / Cell::Insert,
+     * kernels.cu:316-370), here, where the kernel is bound by its
+     * instruction issue rather than by memory: one claim per destination
+     * cell on its arrival counter (the leavers were counted and ranked per
+     * halo cell above), then each record, written to the tile's queue in
+     * the loop and read back from the cache, takes arrival index r =
+     * base + rank: inbox record r of its cell while r < PIC_INBOX (one
+     * 32-byte write), else the next record of the destination tile's spill
+     * list.  The counters and inboxes written here are not the ones this
+     * launch reads.  q.slot keeps r for the host's repair of a full spill
+     * list. */
+    auto dest_of = [&](int h, long &cell, long &dtile, int &lc) {
+        int gx = I0 - 1 + h / (BY * BZ), gy = L0 - 1 + (h / BZ) % BY, gz = K0 - 1 + h % BZ;
+        if (gx < 0) gx += g.Nx; else if (gx >= g.Nx) gx -= g.Nx;
+        if (gy < 0) gy += g.Ny; else if (gy >= g.Ny) gy -= g.Ny;
+        if (gz < 0) gz += g.Nz; else if (gz >= g.Nz) gz -= g.Nz;
+        cell = ((long)gx * g.Ny + gy) * g.Nz + gz;
+        dtile = ((long)(gx / TX) * a.t.nty + gy / TY) * a.t.ntz + gz / TZ;
+        lc = ((gx % TX) * TY + gy % TY) * TZ + gz % TZ;
+    };
+    for (int h = tid; h < NB; h += PIC_BLOCK) {
+        const int n = sHalo[h];
+        if (n > 0) {
+            long cell, dt;
+            int lc;
+            dest_of(h, cell, dt, lc);
+            sHalo[h] = atomicAdd(&a.L.arr_next[cell], n);   /* now the run's first index */
+        }
+    }
+    __syncthreads();
+    const int nq = min(sQ, a.q.qcap);
+    const float4 *const QE = a.q.e + 2 * (tile * (long)a.q.qcap);
+    for (int j = tid; j < nq; j += PIC_BLOCK) {
+        const float4 A = QE[2 * j];
+        float4 B = QE[2 * j + 1];
+        const unsigned key = __float_as_uint(B.z);
+        const int h = key_halo(key);
+        const int r = sHalo[h] + key_rank(key);
+        a.q.slot[tile * (long)a.q.qcap + j] = r;
+        long cell, dt;
+        int lc;
+        dest_of(h, cell, dt, lc);
+        if (r < PIC_INBOX) {
+            float4 *d = a.L.inbox_next + 2 * (cell * PIC_INBOX + r);
+            d[0] = A;
+            d[1] = B;
+        } else {
+            const int k = atomicAdd(&a.L.spill_cnt_next[dt], 1);
+            if (k < a.L.scap) {
+                B.z = __uint_as_float((key & 255u) | ((unsigned)lc << 8));
+                float4 *d = a.L.spill_next + 2 * (dt * a.L.scap + k);
+                d[0] = A;
+                d[1] = B;
+            } else {
+                atomicOr(&sFlags, (unsigned)PIC_F_SPILL_OVER);
+            }
+        }
+    }
     __syncthreads();
     if (tid == 0) {
-        int nq = sQ;
-        if (nq > a.q.qcap) {
-            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_OVERFLOW);
-            nq = a.q.qcap;
-        }
+        if (sQ > a.q.qcap) atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_OVERFLOW);
         a.q.count[tile] = nq;
-        a.L.spill_cnt[tile] = 0;   /* read above; this step's arrivals refill it */
+        a.L.spill_cnt[tile] = 0;   /* read above; refilled two steps on */
         if (sFlags) atomicOr(&a.st->flags, sFlags);
     }
 }
@@ -584,21 +639,6 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_scatter_load32(const long *off, l
     p.sp[e] = ssp[src];
 }
 
-__global__ __launch_bounds__(PIC_BLOCK) void k_repack32(long ncell, PartPtrs32 from,
-                                                       PartPtrs32 to) {
-    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
-    if (e >= ncell * from.cap) return;
-    const long c = e / from.cap;
-    const int j = (int)(e % from.cap);
-    int n = from.count[c];
-    if (n > from.cap) n = from.cap;
-    if (j >= n) return;
-    const long dst = c * to.cap + j;
-    for (int k = 0; k < 6; k++) to.p[k][dst] = from.p[k][e];
-    to.sp[dst] = from.sp[e];
-    if (to.id) to.id[dst] = from.id[e];
-}
-
 /* bulk cell of halo-box cell h of a tile (periodic wrap) */
 template <int TX, int TY, int TZ>
 __device__ __forceinline__ long halo_cell(const Grid &g, const TileGeom &t, long tile, int h) {
@@ -613,7 +653,7 @@ __device__ __forceinline__ long halo_cell(const Grid &g, const TileGeom &t, long
     return ((long)gx * g.Ny + gy) * g.Nz + gz;
 }
 
-/* one queue record into its bucket slot */
+/* one departure record into bucket slot `slot` of `cell` */
 __device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, float4 A, float4 B,
                                         unsigned key) {
     const long to = cell * p.cap + slot;
@@ -627,22 +667,6 @@ __device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, floa
     if (p.id) p.id[to] = __float_as_int(B.w);
 }
 
-/* inbox records into the bucket arrays: PIC_INBOX lanes per cell, all in
- * one wavefront (they read bcount before lane 0 rewrites it) */
-static_assert(64 % PIC_INBOX == 0, "a cell's settle lanes share a wavefront");
-__global__ __launch_bounds__(PIC_BLOCK) void k_settle32(long ncell, PartPtrs32 p) {
-    const long cell = ((long)blockIdx.x * PIC_BLOCK + threadIdx.x) / PIC_INBOX;
-    const int r = (int)(threadIdx.x % PIC_INBOX);
-    if (cell >= ncell) return;
-    const int cnt = min(p.count[cell], p.cap), bc = p.bcount[cell];
-    if (r < min(PIC_INBOX, cnt - bc)) {
-        const float4 A = p.inbox[2 * (cell * PIC_INBOX + r)];
-        const float4 B = p.inbox[2 * (cell * PIC_INBOX + r) + 1];
-        place32(p, cell, bc + r, A, B, __float_as_uint(B.z));
-    }
-    if (r == 0) p.bcount[cell] = cnt;
-}
-
 /* tile and local cell of bulk cell `cell` */
 template <int TX, int TY, int TZ>
 __device__ __forceinline__ void cell_tile(const Grid &g, const TileGeom &t, long cell, long &tile,
@@ -653,112 +677,33 @@ __device__ __forceinline__ void cell_tile(const Grid &g, const TileGeom &t, long
     lc = ((gi % TX) * TY + gl % TY) * TZ + gk % TZ;
 }
 
-/* Arrival of the FP32 departures (GPU_ArrangeFlights / Cell::Insert,
- * kernels.cu:316-370), one workgroup per source tile: one claim per
- * destination cell on its count (the push counted the tile's leavers per
- * halo cell and ranked them), then each record takes slot s = base + rank,
- * r = s - bcount past the cell's particle-array part: inbox record r while
- * r < PIC_INBOX (one 32-byte write), else the next record of the
- * destination tile's spill list, its key holding the local cell in place
- * of the halo index (bucket slot s when the live set holds buckets, which
- * no push leaves behind).  A full spill list raises PIC_F_SPILL_OVER and a slot
- * past the capacity PIC_F_BUCKET_OVERFLOW: the host then places these from
- * the queue (q.slot keeps every slot). */
-template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(PIC_BLOCK) void k_arrive32(Grid g, TileGeom t, PartPtrs32 p,
-                                                       Layout32 L, QueuePtrs32 q, DevStatus *st) {
-    constexpr int NH = (TX + 2) * (TY + 2) * (TZ + 2);
-    static_assert(NH <= 256, "a record key holds the halo index in 8 bits");
-    static_assert(TX * TY * TZ <= 256, "a spill record key holds the local cell in 8 bits");
-    __shared__ int hn[NH];
-    __shared__ int hbase[NH];
-    __shared__ int hbc[NH];
-    __shared__ long hcell[NH];
-    __shared__ long htile[NH];
-    __shared__ int hlc[NH];
-    const long tile = blockIdx.x;
-    const int tid = threadIdx.x;
-    int n = q.count[tile];
-    n = n < q.qcap ? n : q.qcap;
-    const long qb = tile * (long)q.qcap;
-    /* the first record of each thread is loaded before the claims */
-    float4 A0 = make_float4(0.f, 0.f, 0.f, 0.f), B0 = A0;
-    if (tid < n) {
-        A0 = q.e[2 * (qb + tid)];
-        B0 = q.e[2 * (qb + tid) + 1];
+/* after a push: a cell holds its stayers and its arrivals; a count past the
+ * capacity raises PIC_F_BUCKET_OVERFLOW (the host unpacks into larger
+ * buckets before the next push) */
+__global__ __launch_bounds__(PIC_BLOCK) void k_arrivals_done32(long ncell, PartPtrs32 p,
+                                                              const int *arr, DevStatus *st) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    bool over = false;
+    if (e < ncell) {
+        const int n = p.bcount[e] + arr[e];
+        p.count[e] = n;
+        over = n > p.cap;
     }
-    for (int h = tid; h < NH; h += PIC_BLOCK) {
-        const int c = q.hcnt[tile * NH + h];
-        hn[h] = c > 0 ? c : 0;
-        hbase[h] = -1;
-        hbc[h] = 0;
-        hcell[h] = 0;
-        htile[h] = 0;
-        hlc[h] = 0;
-        if (c > 0) {
-            const long cell = halo_cell<TX, TY, TZ>(g, t, tile, h);
-            hcell[h] = cell;
-            hbc[h] = p.bcount[cell];
-            hbase[h] = atomicAdd(&p.count[cell], c);
-            long dt;
-            int lc;
-            cell_tile<TX, TY, TZ>(g, t, cell, dt, lc);
-            htile[h] = dt;
-            hlc[h] = lc;
-        }
-    }
-    __syncthreads();
-    bool over = false, bad = false, spill_over = false;
-    for (int j = tid; j < n; j += PIC_BLOCK) {
-        float4 A = A0, B = B0;
-        if (j != tid) {
-            A = q.e[2 * (qb + j)];
-            B = q.e[2 * (qb + j) + 1];
-        }
-        const unsigned key = __float_as_uint(B.z);
-        int slot, r;
-        const int verdict = arrival_slot32(key, NH, hn, hbase, hbc, p.cap, &slot, &r);
-        q.slot[qb + j] = slot;
-        if (verdict != PIC_ARR_OK) {
-            over |= verdict == PIC_ARR_OVER;
-            bad |= verdict == PIC_ARR_BAD;
-            continue;
-        }
-        const int h = key_halo(key);
-        const long cell = hcell[h];
-        if (r < PIC_INBOX) {
-            p.inbox[2 * (cell * PIC_INBOX + r)] = A;
-            p.inbox[2 * (cell * PIC_INBOX + r) + 1] = B;
-        } else if (!L.packed) {
-            place32(p, cell, slot, A, B, key);     /* bucket slot past the inbox */
-        } else {
-            const long dt = htile[h];
-            const int k = atomicAdd(&L.spill_cnt[dt], 1);
-            if (k < L.scap) {
-                B.z = __uint_as_float((key & 255u) | ((unsigned)hlc[h] << 8));
-                float4 *d = L.spill + 2 * (dt * L.scap + k);
-                d[0] = A;
-                d[1] = B;
-            } else {
-                spill_over = true;
-            }
-        }
-    }
-    if (over) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
-    if (bad) atomicOr(&st->flags, (unsigned)PIC_F_QUEUE_CORRUPT);
-    if (spill_over) atomicOr(&st->flags, (unsigned)PIC_F_SPILL_OVER);
+    if (__ballot(over) && (threadIdx.x & 63) == 0)
+        atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
 }
 
-/* the packed live set into buckets: slot j of a cell from its tile's region
- * (j < bcount) or from inbox record j - bcount */
+/* the packed live set into buckets of `to` (capacity to.cap >= every
+ * count): slot j of a cell from its tile's region (j < bcount) or from inbox
+ * record j - bcount */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_unpack32(Grid g, TileGeom t, long ncell,
                                                        PartPtrs32 p, Layout32 L, PartPtrs32 to) {
     const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
-    if (e >= ncell * p.cap) return;
-    const long cell = e / p.cap;
-    const int j = (int)(e % p.cap);
-    const int cnt = min(p.count[cell], p.cap), bc = max(0, min(p.bcount[cell], cnt));
+    if (e >= ncell * to.cap) return;
+    const long cell = e / to.cap;
+    const int j = (int)(e % to.cap);
+    const int cnt = min(p.count[cell], to.cap), bc = max(0, min(p.bcount[cell], cnt));
     const int nin = min(PIC_INBOX, cnt - bc);
     if (j < bc) {
         long tile;
@@ -797,60 +742,46 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_unspill32(Grid g, TileGeom t, Par
                   gk = tz * TZ + lc % TZ;
         if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
         const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-        const int cnt = min(p.count[cell], p.cap), bc = max(0, min(p.bcount[cell], cnt));
+        const int cnt = min(p.count[cell], to.cap), bc = max(0, min(p.bcount[cell], cnt));
         const int slot = bc + PIC_INBOX + atomicAdd(&fill[lc], 1);
         if (slot < cnt) place32(to, cell, slot, A, B, key);
     }
 }
 
-/* after an unpack: every slot is in the bucket arrays, the spill lists are
- * empty */
-__global__ __launch_bounds__(PIC_BLOCK) void k_unpack_done32(long ncell, long ntiles,
-                                                            PartPtrs32 p, Layout32 L) {
-    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
-    if (e < ncell) p.bcount[e] = min(p.count[e], p.cap);
-    if (e < ntiles) L.spill_cnt[e] = 0;
-}
-
-__global__ void k_clear_flags32(DevStatus *st, unsigned mask) { st->flags &= ~mask; }
-
-/* the arrivals past the inbox (r >= PIC_INBOX, below the capacity) from
- * the departure queues into bucket slots: the repair after a spill list
- * ran full, which reads every such arrival from its queue record */
+/* the arrivals past the inbox from the departure queues of the last push
+ * (q.slot = arrival index r) into bucket slots bcount + r of `to`: the
+ * repair after a spill list ran full, which reads every such arrival from
+ * its queue record */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_place_spills32(Grid g, TileGeom t, PartPtrs32 p,
                                                              QueuePtrs32 q, PartPtrs32 to) {
     const long tile = blockIdx.x;
-    int n = q.count[tile];
-    n = n < q.qcap ? n : q.qcap;
+    const int n = min(q.count[tile], q.qcap);
     const long qb = tile * (long)q.qcap;
     for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
-        const int slot = q.slot[qb + j];
-        if (slot < 0 || slot >= to.cap) continue;
+        const int r = q.slot[qb + j];
+        if (r < PIC_INBOX) continue;
         const float4 A = q.e[2 * (qb + j)], B = q.e[2 * (qb + j) + 1];
         const unsigned key = __float_as_uint(B.z);
         const long cell = halo_cell<TX, TY, TZ>(g, t, tile, key_halo(key));
-        if (slot - p.bcount[cell] >= PIC_INBOX) place32(to, cell, slot, A, B, key);
+        const int slot = p.bcount[cell] + r;
+        if (slot < min(p.count[cell], to.cap)) place32(to, cell, slot, A, B, key);
     }
 }
 
-/* regrow: the arrivals whose claimed slot lay past the old capacity (a
- * record the arrival refused carries slot -1) */
-template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(PIC_BLOCK) void k_fix_arrivals32(Grid g, TileGeom t, QueuePtrs32 q,
-                                                             int oldcap, PartPtrs32 to) {
-    const long tile = blockIdx.x;
-    int n = q.count[tile];
-    n = n < q.qcap ? n : q.qcap;
-    const long qb = tile * (long)q.qcap;
-    for (int j = threadIdx.x; j < n; j += PIC_BLOCK) {
-        const int slot = q.slot[qb + j];
-        if (slot < oldcap || slot >= to.cap) continue;
-        const float4 A = q.e[2 * (qb + j)], B = q.e[2 * (qb + j) + 1];
-        const unsigned key = __float_as_uint(B.z);
-        place32(to, halo_cell<TX, TY, TZ>(g, t, tile, key_halo(key)), slot, A, B, key);
+/* after an unpack: every particle is in the bucket arrays (bcount =
+ * count), no arrival is pending */
+__global__ __launch_bounds__(PIC_BLOCK) void k_unpack_done32(long ncell, long ntiles,
+                                                            PartPtrs32 p, Layout32 L, int cap) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e < ncell) {
+        p.bcount[e] = min(p.count[e], cap);
+        L.arr[e] = 0;
     }
+    if (e < ntiles) L.spill_cnt[e] = 0;
 }
+
+__global__ void k_clear_flags32(DevStatus *st, unsigned mask) { st->flags &= ~mask; }
 
 /* global coordinates x = (i + o) h in FP64 */
 __global__ __launch_bounds__(PIC_BLOCK) void k_gather_particles32(Grid g, long ncell,
@@ -1007,56 +938,42 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                            dim3(PIC_BLOCK), lds, s, a);
 }
 
-void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s) {
-    hipLaunchKernelGGL(k_settle32, dim3(blocks_for(ncell * PIC_INBOX)), dim3(PIC_BLOCK), 0, s,
-                       ncell, p);
-}
-
-void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
-                       QueuePtrs32 q, DevStatus *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_arrive32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), 0,
-                       s, g, t, p, L, q, st);
+void launch_arrivals_done32(long ncell, PartPtrs32 p, Layout32 L, DevStatus *st,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(k_arrivals_done32, dim3(blocks_for(ncell)), dim3(PIC_BLOCK), 0, s, ncell,
+                       p, (const int *)L.arr, st);
 }
 
 void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
-                     bool with_spill, hipStream_t s) {
+                     const QueuePtrs32 *from_queue, hipStream_t s) {
     const long ncell = (long)g.Nx * g.Ny * g.Nz;
-    to.cap = p.cap;
-    hipLaunchKernelGGL((k_unpack32<TXC, TYC, TZC>), dim3(blocks_for(ncell * p.cap)),
+    if (from_queue)
+        hipLaunchKernelGGL((k_place_spills32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                           dim3(PIC_BLOCK), 0, s, g, t, p, *from_queue, to);
+    hipLaunchKernelGGL((k_unpack32<TXC, TYC, TZC>), dim3(blocks_for(ncell * to.cap)),
                        dim3(PIC_BLOCK), 0, s, g, t, ncell, p, L, to);
-    if (with_spill)
+    if (!from_queue)
         hipLaunchKernelGGL((k_unspill32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), 0, s, g, t, p, L, to);
     hipLaunchKernelGGL(k_unpack_done32, dim3(blocks_for(std::max(ncell, t.ntiles))),
-                       dim3(PIC_BLOCK), 0, s, ncell, t.ntiles, p, L);
+                       dim3(PIC_BLOCK), 0, s, ncell, t.ntiles, p, L, to.cap);
+}
+
+void launch_clear_arrivals32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                             hipStream_t s) {
+    const long ncell = (long)g.Nx * g.Ny * g.Nz;
+    hipLaunchKernelGGL(k_unpack_done32, dim3(blocks_for(std::max(ncell, t.ntiles))),
+                       dim3(PIC_BLOCK), 0, s, ncell, t.ntiles, p, L, p.cap);
 }
 
 void launch_clear_flags32(DevStatus *st, unsigned mask, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_flags32, dim3(1), dim3(1), 0, s, st, mask);
 }
 
-void launch_place_spills32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
-                           PartPtrs32 to, hipStream_t s) {
-    to.cap = p.cap;
-    hipLaunchKernelGGL((k_place_spills32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
-                       dim3(PIC_BLOCK), 0, s, g, t, p, q, to);
-}
-
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s) {
     hipLaunchKernelGGL(k_scatter_load32, dim3(blocks_for(ncell * p.cap)), dim3(PIC_BLOCK), 0, s,
                        offsets, ncell, src[0], src[1], src[2], src[3], src[4], src[5], srcsp, p);
-}
-
-void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s) {
-    hipLaunchKernelGGL(k_repack32, dim3(blocks_for(ncell * from.cap)), dim3(PIC_BLOCK), 0, s,
-                       ncell, from, to);
-}
-
-void launch_fix_arrivals32(const Grid &g, const TileGeom &t, QueuePtrs32 q, int oldcap,
-                           PartPtrs32 to, hipStream_t s) {
-    hipLaunchKernelGGL((k_fix_arrivals32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
-                       dim3(PIC_BLOCK), 0, s, g, t, q, oldcap, to);
 }
 
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,

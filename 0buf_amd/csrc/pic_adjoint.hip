@@ -182,8 +182,8 @@ int restore_state(pic_ctx *c, const Compact &src, const FieldSet &f) {
         for (int j = 0; j < 6; j++) v[j] = (const float *)src.v[j];
         launch_scatter_load32(src.off, c->NC, v, src.sp, c->p32, c->stream);   /* bcount = count */
         launch_scatter_ids32(src.off, c->NC, src.id, c->p32, c->stream);
-        c->unsettled = false;
-        c->lay32.packed = 0;           /* buckets; the next push ignores the spill lists */
+        launch_clear_arrivals32(c->g, c->t, c->p32, c->lay32, c->stream);
+        c->lay32.packed = 0;
         c->arrival_pending = false;
     } else {
         const double *v[6];

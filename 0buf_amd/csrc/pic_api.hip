@@ -43,14 +43,6 @@ void free_parts(P &p) {
     p = P{};
 }
 
-/* inbox of the live FP32 buckets; bcount is set equal to count by whoever
- * fills the buckets (scatter load, initializer, regrow) */
-int alloc_inbox(PartPtrs32 &p, long ncell) {
-    HIPCHK(hipMalloc(&p.bcount, ncell * sizeof(int)));
-    HIPCHK(hipMalloc(&p.inbox, (size_t)ncell * PIC_INBOX * 2 * sizeof(float4)));
-    return PIC_SUCCESS;
-}
-
 /* buckets of ncell x cap slots; ids only for the differentiable form */
 template <class P>
 int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id) {
@@ -76,35 +68,84 @@ long live_slots32(const pic_ctx *c) {
     return std::max(c->NC, c->t.ntiles * (long)(c->t.TX * c->t.TY * c->t.TZ));
 }
 
-/* FP32 form: the live set p (per-cell count, bcount and inbox included)
- * and the other set the push writes, both with capacity cap; the packed
- * layout's per-cell starts and per-tile spill lists on first use */
-int alloc_live32(pic_ctx *c, PartPtrs32 &p, int cap, bool with_count) {
-    const long ns = live_slots32(c);
-    int rc = alloc_parts(p, ns, cap, false, c->diff);
-    if (rc) return rc;
-    if (with_count) HIPCHK(hipMalloc(&p.count, c->NC * sizeof(int)));
-    if ((rc = alloc_inbox(p, c->NC))) return rc;
-    free_parts(c->alt32);
-    if ((rc = alloc_parts(c->alt32, ns, cap, false, c->diff))) return rc;
+/* FP32 form, once per context: the per-cell counts, both inboxes and
+ * arrival counters, the packed starts and both spill lists per tile */
+int alloc_cells32(pic_ctx *c) {
+    PartPtrs32 &p = c->p32;
     Layout32 &L = c->lay32;
-    if (!L.pstart) {
-        HIPCHK(hipMalloc(&L.pstart, c->NC * sizeof(int)));
-        L.scap = PIC_SPILL_CAP;
-        HIPCHK(hipMalloc(&L.spill, (size_t)c->t.ntiles * L.scap * 2 * sizeof(float4)));
-        HIPCHK(hipMalloc(&L.spill_cnt, c->t.ntiles * sizeof(int)));
-        HIPCHK(hipMemset(L.spill_cnt, 0, c->t.ntiles * sizeof(int)));
-    }
-    L.tcap = (long)c->t.TX * c->t.TY * c->t.TZ * cap;
+    if (p.count) return PIC_SUCCESS;
+    const long NC = c->NC, nt = c->t.ntiles;
+    const size_t inbox = (size_t)NC * PIC_INBOX * 2 * sizeof(float4);
+    HIPCHK(hipMalloc(&p.count, NC * sizeof(int)));
+    HIPCHK(hipMalloc(&p.bcount, NC * sizeof(int)));
+    HIPCHK(hipMalloc(&p.inbox, inbox));
+    HIPCHK(hipMalloc(&L.inbox_next, inbox));
+    HIPCHK(hipMalloc(&L.pstart, NC * sizeof(int)));
+    HIPCHK(hipMalloc(&L.arr, NC * sizeof(int)));
+    HIPCHK(hipMalloc(&L.arr_next, NC * sizeof(int)));
+    HIPCHK(hipMemset(L.arr, 0, NC * sizeof(int)));
+    HIPCHK(hipMemset(L.arr_next, 0, NC * sizeof(int)));
+    L.scap = PIC_SPILL_CAP;
+    const size_t spill = (size_t)nt * L.scap * 2 * sizeof(float4);
+    HIPCHK(hipMalloc(&L.spill, spill));
+    HIPCHK(hipMalloc(&L.spill_next, spill));
+    HIPCHK(hipMalloc(&L.spill_cnt, nt * sizeof(int)));
+    HIPCHK(hipMalloc(&L.spill_cnt_next, nt * sizeof(int)));
+    HIPCHK(hipMemset(L.spill_cnt, 0, nt * sizeof(int)));
+    HIPCHK(hipMemset(L.spill_cnt_next, 0, nt * sizeof(int)));
     L.packed = 0;
     return PIC_SUCCESS;
 }
 
+/* FP32 form: the particle arrays of one set (p, sp, id), capacity cap */
+int alloc_set32(pic_ctx *c, PartPtrs32 &s, int cap) {
+    const size_t n = (size_t)live_slots32(c) * cap;
+    for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&s.p[k], n * sizeof(float)));
+    HIPCHK(hipMalloc(&s.sp, n));
+    if (c->diff) HIPCHK(hipMalloc(&s.id, n * sizeof(int)));
+    s.cap = cap;
+    return PIC_SUCCESS;
+}
+
+void free_set32(PartPtrs32 &s) {
+    for (int k = 0; k < 6; k++) {
+        if (s.p[k]) (void)hipFree(s.p[k]);
+        s.p[k] = nullptr;
+    }
+    if (s.sp) (void)hipFree(s.sp);
+    if (s.id) (void)hipFree(s.id);
+    s.sp = nullptr;
+    s.id = nullptr;
+}
+
+/* the live set and the set the push writes, both of capacity cap (the
+ * per-cell arrays on first use) */
+int alloc_live32(pic_ctx *c, int cap) {
+    int rc = alloc_cells32(c);
+    if (rc) return rc;
+    free_set32(c->p32);
+    free_set32(c->alt32);
+    if ((rc = alloc_set32(c, c->p32, cap)) || (rc = alloc_set32(c, c->alt32, cap))) return rc;
+    c->lay32.tcap = (long)c->t.TX * c->t.TY * c->t.TZ * cap;
+    c->lay32.packed = 0;
+    return PIC_SUCCESS;
+}
+
 void free_layout32(Layout32 &L) {
-    if (L.pstart) (void)hipFree(L.pstart);
-    if (L.spill) (void)hipFree(L.spill);
-    if (L.spill_cnt) (void)hipFree(L.spill_cnt);
+    for (void *q : {(void *)L.pstart, (void *)L.arr, (void *)L.arr_next, (void *)L.inbox_next,
+                    (void *)L.spill, (void *)L.spill_next, (void *)L.spill_cnt,
+                    (void *)L.spill_cnt_next})
+        if (q) (void)hipFree(q);
     L = Layout32{};
+}
+
+/* the arrays the next push reads take the place of those this push wrote */
+void swap_arrivals32(pic_ctx *c) {
+    Layout32 &L = c->lay32;
+    std::swap(c->p32.inbox, L.inbox_next);
+    std::swap(L.arr, L.arr_next);
+    std::swap(L.spill, L.spill_next);
+    std::swap(L.spill_cnt, L.spill_cnt_next);
 }
 
 /* the live and the other FP32 set trade places (per-cell arrays stay) */
@@ -148,7 +189,6 @@ void free_queue(QueuePtrs32 &q) {
     if (q.e) (void)hipFree(q.e);
     if (q.slot) (void)hipFree(q.slot);
     if (q.count) (void)hipFree(q.count);
-    if (q.hcnt) (void)hipFree(q.hcnt);
     q = QueuePtrs32{};
 }
 
@@ -156,12 +196,10 @@ int alloc_queue(QueuePtrs32 &q, const TileGeom &t, int qcap, bool) {
     q = QueuePtrs32{};
     q.qcap = qcap;
     const size_t n = (size_t)t.ntiles * qcap;
-    const size_t nh = (size_t)(t.TX + 2) * (t.TY + 2) * (t.TZ + 2);
     HIPCHK(hipMalloc(&q.e, 2 * n * sizeof(float4)));
     HIPCHK(hipMalloc(&q.slot, n * sizeof(int)));
     HIPCHK(hipMalloc(&q.count, t.ntiles * sizeof(int)));
     HIPCHK(hipMemset(q.count, 0, t.ntiles * sizeof(int)));
-    HIPCHK(hipMalloc(&q.hcnt, t.ntiles * nh * sizeof(int)));
     return PIC_SUCCESS;
 }
 
@@ -237,7 +275,7 @@ int commit_into(pic_ctx *c, P &parts, Q &queue, int cap, const std::vector<long>
         }
     }
     int rc;
-    if constexpr (std::is_same<P, PartPtrs32>::value) rc = alloc_live32(c, parts, cap, true);
+    if constexpr (std::is_same<P, PartPtrs32>::value) rc = alloc_live32(c, cap);
     else rc = alloc_parts(parts, NC, cap, true, c->diff);
     if (rc) return rc;
     rc = alloc_queue(queue, c->t, queue_capacity(c->t, cap), c->diff);
@@ -324,7 +362,7 @@ int commit(pic_ctx *c) {
 }
 
 /* bucket overflow repair: grow every bucket, re-place the arrivals whose
- * claimed slot did not fit */
+ * claimed slot did not fit (FP64 form) */
 template <class P, class Q>
 int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     const long NC = c->NC;
@@ -337,32 +375,11 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
                              (oldcap + 8 + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN);
     if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     P np;
-    int rc;
-    if constexpr (std::is_same<P, PartPtrs32>::value) {
-        /* the buckets only (finish_step unpacks a packed set first) */
-        if (c->lay32.packed) {
-            snprintf(g_hip_err, sizeof(g_hip_err), "regrow of a packed particle set");
-            return PIC_ERR_STATE;
-        }
-        rc = alloc_live32(c, np, cap, false);
-    } else {
-        rc = alloc_parts(np, NC, cap, false, c->diff);
-    }
+    int rc = alloc_parts(np, NC, cap, false, c->diff);
     if (rc) return rc;
     np.count = parts.count;
-    if constexpr (std::is_same<P, PartPtrs32>::value) {
-        /* the inbox records (all below the old capacity) into the old
-         * bucket arrays, then the bucket arrays into the grown ones */
-        launch_settle32(NC, parts, c->stream);
-        launch_repack32(NC, parts, np, c->stream);
-        launch_fix_arrivals32(c->g, c->t, queue, oldcap, np, c->stream);
-        HIPCHK(hipMemcpyAsync(np.bcount, np.count, NC * sizeof(int), hipMemcpyDeviceToDevice,
-                              c->stream));
-        c->unsettled = false;
-    } else {
-        launch_repack(NC, parts, np, c->stream);
-        launch_fix_arrivals(c->t, queue, oldcap, np, c->stream);
-    }
+    launch_repack(NC, parts, np, c->stream);
+    launch_fix_arrivals(c->t, queue, oldcap, np, c->stream);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(c->stream));
     parts.count = nullptr;
@@ -379,33 +396,72 @@ int regrow_parts(pic_ctx *c, P &parts, Q &queue) {
     return PIC_SUCCESS;
 }
 
-int regrow(pic_ctx *c) {
-    return c->fp32 ? regrow_parts(c, c->p32, c->q32) : regrow_parts(c, c->p, c->q);
+/* FP32 form: a count past the capacity after the arrivals of a push: the
+ * packed set, its inboxes and its arrivals past the inbox (from the spill
+ * lists, or from the departure queues after a full spill list) unpacked
+ * into buckets of a larger capacity */
+int regrow32(pic_ctx *c, bool from_queue) {
+    const long NC = c->NC;
+    if (!c->lay32.packed) {
+        snprintf(g_hip_err, sizeof(g_hip_err), "bucket overflow of an unpacked particle set");
+        return PIC_ERR_STATE;
+    }
+    std::vector<int> cnt(NC);
+    HIPCHK(hipMemcpy(cnt.data(), c->p32.count, NC * sizeof(int), hipMemcpyDeviceToHost));
+    int maxc = 0;
+    for (long k = 0; k < NC; k++) maxc = std::max(maxc, cnt[k]);
+    const int oldcap = c->p32.cap;
+    const int cap = std::max(auto_capacity(maxc),
+                             (oldcap + 8 + PIC_CAP_ALIGN - 1) / PIC_CAP_ALIGN * PIC_CAP_ALIGN);
+    if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
+    PartPtrs32 na{}, nb{};
+    int rc;
+    if ((rc = alloc_set32(c, na, cap)) || (rc = alloc_set32(c, nb, cap))) return rc;
+    launch_unpack32(c->g, c->t, c->p32, c->lay32, na, from_queue ? &c->q32 : nullptr, c->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(c->stream));
+    free_set32(c->p32);
+    free_set32(c->alt32);
+    for (int k = 0; k < 6; k++) c->p32.p[k] = na.p[k];
+    c->p32.sp = na.sp;
+    c->p32.id = na.id;
+    c->p32.cap = cap;
+    c->alt32 = nb;
+    c->lay32.tcap = (long)c->t.TX * c->t.TY * c->t.TZ * cap;
+    c->lay32.packed = 0;
+    QueuePtrs32 nq;
+    if ((rc = alloc_queue(nq, c->t, queue_capacity(c->t, cap), c->diff))) return rc;
+    free_queue(c->q32);
+    c->q32 = nq;
+    c->regrows++;
+    c->last.bucket_capacity = cap;
+    c->last.bucket_regrows = c->regrows;
+    return PIC_SUCCESS;
 }
 
 int *count_ptr(pic_ctx *c) { return c->fp32 ? c->p32.count : c->p.count; }
 int bucket_cap(pic_ctx *c) { return c->fp32 ? c->p32.cap : c->p.cap; }
 
+/* FP32 form: the packed live set into buckets of the other set, which
+ * becomes the live one */
+int unpack_live32(pic_ctx *c, bool from_queue) {
+    PartPtrs32 to = c->alt32;
+    to.cap = c->p32.cap;
+    launch_unpack32(c->g, c->t, c->p32, c->lay32, to, from_queue ? &c->q32 : nullptr, c->stream);
+    HIPCHK(hipGetLastError());
+    swap_sets32(c);
+    c->lay32.packed = 0;
+    return PIC_SUCCESS;
+}
+
 int settle_noalloc(pic_ctx *c) {
-    if (!c->fp32 || !c->p32.inbox) return PIC_SUCCESS;
+    if (!c->fp32 || !c->committed || !c->lay32.packed) return PIC_SUCCESS;
     if (c->arrival_pending) {
         snprintf(g_hip_err, sizeof(g_hip_err),
                  "the last arrival's status is not examined yet (pic_sync first)");
         return PIC_ERR_STATE;
     }
-    if (c->lay32.packed) {
-        launch_unpack32(c->g, c->t, c->p32, c->lay32, c->alt32, true, c->stream);
-        HIPCHK(hipGetLastError());
-        swap_sets32(c);
-        c->lay32.packed = 0;
-        c->unsettled = false;
-        return PIC_SUCCESS;
-    }
-    if (!c->unsettled) return PIC_SUCCESS;
-    launch_settle32(c->NC, c->p32, c->stream);
-    HIPCHK(hipGetLastError());
-    c->unsettled = false;
-    return PIC_SUCCESS;
+    return unpack_live32(c, false);
 }
 
 int settle(pic_ctx *c) {
@@ -416,15 +472,7 @@ int settle(pic_ctx *c) {
     return settle_noalloc(c);
 }
 
-int despill(pic_ctx *c) {
-    launch_place_spills32(c->g, c->t, c->p32, c->q32, c->alt32, c->stream);
-    launch_unpack32(c->g, c->t, c->p32, c->lay32, c->alt32, false, c->stream);
-    HIPCHK(hipGetLastError());
-    swap_sets32(c);
-    c->lay32.packed = 0;
-    c->unsettled = false;
-    return PIC_SUCCESS;
-}
+int despill(pic_ctx *c) { return unpack_live32(c, true); }
 int *qcount_ptr(pic_ctx *c) { return c->fp32 ? c->q32.count : c->q.count; }
 
 /* stage launchers dispatching on the form */
@@ -439,8 +487,8 @@ void enq_push(pic_ctx *c) {
                                 c->diff ? &c->rec_snap32 : nullptr, c->rec_off, c->rec_room,
                                 c->stream);
         swap_sets32(c);
+        swap_arrivals32(c);
         c->lay32.packed = 1;
-        c->unsettled = false;
     } else if (c->diff) {
         DiffArgs d{};
         for (int k = 0; k < 3; k++) d.jfix[k] = c->d_jfix[k];
@@ -463,8 +511,8 @@ void enq_push(pic_ctx *c) {
 }
 void enq_arrive(pic_ctx *c, hipStream_t st) {
     if (c->fp32) {
-        launch_arrive_f32(c->g, c->t, c->p32, c->lay32, c->q32, c->d_st, st);
-        c->unsettled = !c->lay32.packed;
+        /* the push placed the arrivals; the counts take them in */
+        launch_arrivals_done32(c->NC, c->p32, c->lay32, c->d_st, st);
         c->arrival_pending = true;
     }
     else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, st);
@@ -567,18 +615,19 @@ int finish_step(pic_ctx *c) {
     c->last.bucket_regrows = c->regrows;
     c->last.spill_repairs = (int)c->spill_repairs;
     if (st.flags & PIC_F_QUEUE_CORRUPT) return PIC_ERR_QUEUE_CORRUPT;
-    /* FP32 form: arrivals that found no room in their tile's spill list or
-     * past the bucket capacity are still in the departure queues: every
-     * particle into buckets (then the regrow, for the latter) */
-    if (c->fp32 && arrived && c->lay32.packed &&
-        (st.flags & (PIC_F_SPILL_OVER | PIC_F_BUCKET_OVERFLOW))) {
-        int rc = despill(c);
+    /* FP32 form: a count past the capacity (larger buckets), arrivals that
+     * found no room in their tile's spill list (still in the departure
+     * queues): every particle into buckets */
+    if (c->fp32 && arrived && c->lay32.packed) {
+        const bool over = st.flags & PIC_F_SPILL_OVER;
+        int rc = PIC_SUCCESS;
+        if (st.flags & PIC_F_BUCKET_OVERFLOW) rc = regrow32(c, over);
+        else if (over) rc = despill(c);
         if (rc) return rc;
-        if (st.flags & PIC_F_SPILL_OVER) c->spill_repairs++;
+        if (over) c->spill_repairs++;
         c->last.spill_repairs = (int)c->spill_repairs;
-    }
-    if (st.flags & PIC_F_BUCKET_OVERFLOW) {
-        int rc = regrow(c);
+    } else if (!c->fp32 && (st.flags & PIC_F_BUCKET_OVERFLOW)) {
+        int rc = regrow_parts(c, c->p, c->q);
         if (rc) return rc;
     }
     if (st.flags & PIC_F_QUEUE_OVERFLOW) return PIC_ERR_QUEUE_OVERFLOW;
@@ -871,8 +920,7 @@ int pic_generate_device(pic_ctx *c, const pic_species_spec *specs, int nspec,
     if (maxc > cap) cap = auto_capacity(maxc);
     if (!push_span_ok(c, cap)) return PIC_ERR_INVALID;
     HIPCHK(hipSetDevice(c->dev));
-    int rc = c->fp32 ? alloc_live32(c, c->p32, cap, true)
-                     : alloc_parts(c->p, NC, cap, true, c->diff);
+    int rc = c->fp32 ? alloc_live32(c, cap) : alloc_parts(c->p, NC, cap, true, c->diff);
     if (rc) return rc;
     rc = c->fp32 ? alloc_queue(c->q32, c->t, queue_capacity(c->t, cap), c->diff)
                  : alloc_queue(c->q, c->t, queue_capacity(c->t, cap), c->diff);

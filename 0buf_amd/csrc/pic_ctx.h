@@ -80,7 +80,6 @@ struct pic_ctx {
     double *h_red = nullptr;
     Staged staged;
     bool committed = false;
-    bool unsettled = false;   /* FP32 form: inbox records not yet in the bucket arrays */
     /* FP32 form: an arrival ran whose status no host call has examined yet
      * (a full spill list or bucket needs the host's repair before the next
      * push or any reader) */
@@ -103,15 +102,15 @@ int check_ready(pic_ctx *c);
 int enq_step(pic_ctx *c);
 int finish_step(pic_ctx *c);
 /* FP32 form: put every particle into the bucket arrays (unpack the packed
- * set, or move the inbox records) before anything other than the push reads
+ * set with its pending arrivals) before anything other than the push reads
  * the buckets (no-op otherwise); examines a pending arrival's status first */
 int settle(pic_ctx *c);
 /* the same without any allocation: PIC_ERR_STATE while an arrival's status
  * is pending (its repair may allocate) */
 int settle_noalloc(pic_ctx *c);
-/* FP32 form: the arrival's status after a full spill list or bucket: every
- * particle into the bucket arrays of the other set, from the packed set, its
- * inbox records and the departure queues (the set becomes the live one) */
+/* FP32 form: the arrival's status after a full spill list: every particle
+ * into the bucket arrays of the other set, from the packed set, its inbox
+ * records and the departure queues (the set becomes the live one) */
 int despill(pic_ctx *c);
 template <class P>
 int alloc_parts(P &p, long ncell, int cap, bool with_count, bool with_id);

@@ -108,18 +108,26 @@ struct PartPtrs32 {
  * the tile's workgroup writes the stayers of its cells contiguously in flat
  * order from the flat index of its first particle, so a cell's particles
  * lie at pstart[cell] + [0, bcount[cell]) of its tile's region and share
- * lines with their neighbours (no partial line per cell and array).  The
- * arrivals of the step go to the inbox records of their cell (the first
- * PIC_INBOX) and past those to the spill list of the destination tile
- * (scap records of the departure-record format, key = species | local cell
- * << 8), which the next push reads with the cell's packed part and inbox.
- * The host turns the packed set back into buckets (unpack) for every other
- * reader, and when a spill list overflows. */
+ * lines with their neighbours (no partial line per cell and array).
+ *
+ * The push also places its tile's leavers (the arrival): per destination
+ * cell one atomic on arr_next claims a run of arrival indices r, and record
+ * r goes to the cell's inbox (r < PIC_INBOX, inbox_next) or past it to the
+ * spill list of the destination tile (spill_next: scap records of the
+ * departure-record format, key = species | local cell << 8).  The arrays
+ * the push reads (arr, the live inbox, spill) hold the previous step's
+ * arrivals; the host swaps them with the *_next ones after each push.  A
+ * cell holds bcount + arr particles: bcount in the particle arrays, then
+ * its arrivals in order.  The host turns the packed set back into buckets
+ * (unpack) for every other reader, and when a spill list or a bucket
+ * overflows. */
 struct Layout32 {
     int packed;              /* the set being read is packed (else buckets) */
     int *pstart;             /* per cell: first slot in its tile's region */
-    float4 *spill;           /* per tile: scap records (2 float4 each) */
-    int *spill_cnt;          /* per tile: records claimed (may exceed scap) */
+    int *arr, *arr_next;     /* per cell: arrivals of the last step / being placed */
+    float4 *inbox_next;      /* per cell: PIC_INBOX records being placed */
+    float4 *spill, *spill_next;  /* per tile: scap records (2 float4 each) */
+    int *spill_cnt, *spill_cnt_next;  /* per tile: records claimed (may exceed scap) */
     int scap;
     long tcap;               /* slots per tile region */
 };
@@ -129,13 +137,11 @@ constexpr int PIC_SPILL_CAP = 256;  /* spill records per tile (indices kept in 8
  * (ox, oy, oz, pu) and (pv, pw, key, id), with key = species | h << 8 |
  * rank << 16 (h = the destination's index in the source tile's
  * (TX+2)(TY+2)(TZ+2) halo box, rank = the leaver's rank among the tile's
- * leavers bound for h) and id as int bits (differentiable form); hcnt holds
- * per tile the leavers bound for each halo cell */
+ * leavers bound for h) and id as int bits (differentiable form) */
 struct QueuePtrs32 {
     float4 *e;
-    int *slot;               /* slot claimed by the arrival kernel */
+    int *slot;               /* arrival index r the push gave the record */
     int *count;              /* per tile */
-    int *hcnt;               /* per tile and halo cell */
     int qcap;                /* entries per tile */
 };
 
@@ -430,26 +436,22 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                              PartPtrs32 out, Layout32 L, QueuePtrs32 q, SpeciesTable sp,
                              int dep_cap, DevStatus *st, const PartPtrs32 *diff,
                              const long *snap_off, long snap_room, hipStream_t s);
-/* inbox records into the bucket arrays (bcount = count, at most cap) */
-void launch_settle32(long ncell, PartPtrs32 p, hipStream_t s);
-void launch_arrive_f32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
-                       QueuePtrs32 q, DevStatus *st, hipStream_t s);
-/* the packed live set p (with its inbox records, and its spill lists when
- * with_spill) into buckets in `to` (p[6], sp, id; the counts stay those of
- * p); bcount = count (at most cap) and the spill lists emptied after */
+/* after the push: count = bcount + arrivals (PIC_F_BUCKET_OVERFLOW past cap) */
+void launch_arrivals_done32(long ncell, PartPtrs32 p, Layout32 L, DevStatus *st,
+                            hipStream_t s);
+/* the packed live set p, its inbox records and its arrivals past the inbox
+ * (from the spill lists, or from the departure queues of the last push when
+ * from_queue is set: the repair after a full spill list) into buckets of
+ * `to` (p[6], sp, id, cap >= every count); then bcount = count and no
+ * arrival pending */
 void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
-                     bool with_spill, hipStream_t s);
-/* the arrivals past the inbox, from the departure queues, into bucket slots
- * of `to` (the repair after a full spill list; before launch_unpack32 with
- * with_spill false, which reads the same bcount) */
-void launch_place_spills32(const Grid &g, const TileGeom &t, PartPtrs32 p, QueuePtrs32 q,
-                           PartPtrs32 to, hipStream_t s);
+                     const QueuePtrs32 *from_queue, hipStream_t s);
+/* buckets loaded whole (bcount = count): no arrival pending */
+void launch_clear_arrivals32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                             hipStream_t s);
 void launch_clear_flags32(DevStatus *st, unsigned mask, hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
-void launch_repack32(long ncell, PartPtrs32 from, PartPtrs32 to, hipStream_t s);
-void launch_fix_arrivals32(const Grid &g, const TileGeom &t, QueuePtrs32 q, int oldcap,
-                           PartPtrs32 to, hipStream_t s);
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
                                double *dst[6], unsigned char *dsp, int *dcell, int *did,
                                hipStream_t s);
