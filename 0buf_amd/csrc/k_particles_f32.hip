@@ -390,6 +390,23 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
      * first particle on, in flat order */
     const int wout = __builtin_amdgcn_readfirstlane(sOff[sFirst[wv]]);
     int wcarry = 0;
+    /* Second deposit segments (particles whose path crosses a dual face)
+     * wait in the lane's registers until a later particle of the same lane
+     * has one too: the wavefront then deposits every waiting segment at
+     * once, so it runs that code in fewer chunks than every one (integer
+     * sums: the current does not depend on when a contribution is added).
+     * pa: relay point, pb: end offset, pmeta: first node in the J box | the
+     * end's dual cell << 8 | species << 11. */
+    constexpr bool DEFER_SEG = !DIFF;   /* the recording form has no registers to spare */
+    bool pend = false;
+    float pa[3] = {0.f, 0.f, 0.f}, pb[3] = {0.f, 0.f, 0.f};
+    int pmeta = 0;
+    auto deposit_pending = [&]() {
+        const float4 sp4 = sSpec[pmeta >> 11];
+        const float cjp[3] = {sp4.y, sp4.z, sp4.w};
+        vb_seg<BY, BZ, NB>(myJ, pmeta & 255, (pmeta >> 8) & 1, (pmeta >> 9) & 1,
+                           (pmeta >> 10) & 1, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], cjp);
+    };
     unsigned long long nstarts = 0, ncstarts = 0;
     int nlanes = 0, no0 = 0, onext = 0;
     int gnext = compose_full_chunk(sGrp, ng0, no0, ghi, nstarts, nlanes, ncstarts, onext);
@@ -406,6 +423,9 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         if (lane < nlanes) fetch(ng0, no0, nstarts);
         bool stays = false;
         float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
+        bool nseg = false;
+        float sa[3], sb[3];
+        int smeta = 0;
         if (valid) {
             const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
             const float ox = ox0, oy = oy0, oz = oz0;
@@ -452,11 +472,18 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                 const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
                 vb_seg<BY, BZ, NB>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
                                    e1z, cj);
-                if (m)
+                if (m && !DEFER_SEG) {
                     vb_seg<BY, BZ, NB>(myJ,
                                        n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2],
                                        R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1], R.x2[2], o1x,
                                        o1y, o1z, cj);
+                } else if (m) {   /* the second segment, deposited later (below) */
+                    nseg = true;
+                    sa[0] = R.x2[0]; sa[1] = R.x2[1]; sa[2] = R.x2[2];
+                    sb[0] = o1x; sb[1] = o1y; sb[2] = o1z;
+                    smeta = (n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2]) |
+                            (R.r1[0] << 8) | (R.r1[1] << 9) | (R.r1[2] << 10) | ((int)s << 11);
+                }
             }
 
             /* migration by the cell-local offset */
@@ -487,6 +514,19 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             }
         }
 
+        /* a lane with a waiting segment and a new one: deposit every
+         * waiting segment of the wavefront first */
+        if (__ballot(nseg && pend)) {
+            if (pend) deposit_pending();
+            pend = false;
+        }
+        if (nseg) {
+            pend = true;
+            pa[0] = sa[0]; pa[1] = sa[1]; pa[2] = sa[2];
+            pb[0] = sb[0]; pb[1] = sb[1]; pb[2] = sb[2];
+            pmeta = smeta;
+        }
+
         /* stayers in flat order into the other set */
         const unsigned long long S = __ballot(valid && stays);
         if (valid && stays) {
@@ -502,6 +542,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         }
         wcarry += __popcll(S);
     }
+    if (pend) deposit_pending();
     __syncthreads();
 
     int maxout = 0;
