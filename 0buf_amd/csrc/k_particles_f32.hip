@@ -423,9 +423,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         if (lane < nlanes) fetch(ng0, no0, nstarts);
         bool stays = false;
         float nxo = 0.f, nyo = 0.f, nzo = 0.f, pu = pu0, pv = pv0, pw = pw0;
-        bool nseg = false;
-        float sa[3], sb[3];
-        int smeta = 0;
         if (valid) {
             const int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
             const float ox = ox0, oy = oy0, oz = oz0;
@@ -477,12 +474,21 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                                        n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2],
                                        R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1], R.x2[2], o1x,
                                        o1y, o1z, cj);
-                } else if (m) {   /* the second segment, deposited later (below) */
-                    nseg = true;
-                    sa[0] = R.x2[0]; sa[1] = R.x2[1]; sa[2] = R.x2[2];
-                    sb[0] = o1x; sb[1] = o1y; sb[2] = o1z;
-                    smeta = (n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2]) |
-                            (R.r1[0] << 8) | (R.r1[1] << 9) | (R.r1[2] << 10) | ((int)s << 11);
+                } else {
+                    /* a lane with a waiting segment and a new one: every
+                     * waiting segment of the wavefront is deposited first
+                     * (lanes past the chunk's end keep theirs) */
+                    if (__ballot(m && pend)) {
+                        if (pend) deposit_pending();
+                        pend = false;
+                    }
+                    if (m) {
+                        pend = true;
+                        pa[0] = R.x2[0]; pa[1] = R.x2[1]; pa[2] = R.x2[2];
+                        pb[0] = o1x; pb[1] = o1y; pb[2] = o1z;
+                        pmeta = (n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2]) |
+                                (R.r1[0] << 8) | (R.r1[1] << 9) | (R.r1[2] << 10) | ((int)s << 11);
+                    }
                 }
             }
 
@@ -512,19 +518,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                     q[1] = make_float4(pv, pw, __uint_as_float(key), __int_as_float(pid));
                 }
             }
-        }
-
-        /* a lane with a waiting segment and a new one: deposit every
-         * waiting segment of the wavefront first */
-        if (__ballot(nseg && pend)) {
-            if (pend) deposit_pending();
-            pend = false;
-        }
-        if (nseg) {
-            pend = true;
-            pa[0] = sa[0]; pa[1] = sa[1]; pa[2] = sa[2];
-            pb[0] = sb[0]; pb[1] = sb[1]; pb[2] = sb[2];
-            pmeta = smeta;
         }
 
         /* stayers in flat order into the other set */
