@@ -449,10 +449,13 @@ class Pic:
                "pic_forward_record")
         return out
 
-    def adjoint(self, weights=None, seed=None):
+    def adjoint(self, weights=None, seed=None, out=None):
         """Vector-Jacobian product of sum_t w_t W_t + <seed, s_T> with respect to
         the initial state.  seed / result: dict with particle keys x, y, z, px,
-        py, pz (arrays over particle ids) and field keys Ex..Hz (ghosted)."""
+        py, pz (arrays over particle ids) and field keys Ex..Hz (ghosted).
+        out: optional dict of float64 arrays (id_count() or G entries) that
+        receive the result in place of new ones (pinned host arrays make the
+        device-to-host copy several times faster)."""
         keep = []
 
         def io(d, alloc):
@@ -483,7 +486,7 @@ class Pic:
             w = np.ascontiguousarray(weights, dtype=np.float64)
             keep.append(w)
         sd = io(dict(seed), False) if seed else None
-        out = {}
+        out = dict(out) if out else {}
         gr = io(out, True)
         _check(self.lib.pic_adjoint(self._ctx, _dptr(w), C.byref(sd) if sd else None,
                                     C.byref(gr)), "pic_adjoint")

@@ -299,8 +299,13 @@ def device_gradient(cfg=4, precision=pic.FP64, first=150, last=200, ckpt_every=2
         P = g.particle_count()
         w = np.zeros(T)
         w[first - 1:last] = 1.0
+        # the gradient lands in pinned host arrays, allocated once
+        pinned = {k: torch.empty(max(g.id_count(), 1), dtype=torch.float64,
+                                 pin_memory=True).numpy() for k in PART_KEYS}
+        pinned.update({k: torch.empty(g.G, dtype=torch.float64, pin_memory=True).numpy()
+                       for k in FIELD_KEYS[:6]})
         g.forward_record(T, ckpt_every)
-        g.adjoint(weights=w)
+        g.adjoint(weights=w, out=pinned)
         g.rewind()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -312,7 +317,7 @@ def device_gradient(cfg=4, precision=pic.FP64, first=150, last=200, ckpt_every=2
         t0 = time.perf_counter()
         Wr = g.forward_record(T, ckpt_every)
         t1 = time.perf_counter()
-        gr = g.adjoint(weights=w)
+        gr = g.adjoint(weights=w, out=pinned)
         t2 = time.perf_counter()
         dev = g.adjoint_timing()
         st = g.status()

@@ -457,8 +457,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
         const int m = R.m;
 
         /* ---- reverse ---- */
-        double lx1[3] = {a.lp.v[0][pid], a.lp.v[1][pid], a.lp.v[2][pid]};
-        double lp1[3] = {a.lp.v[3][pid], a.lp.v[4][pid], a.lp.v[5][pid]};
+        double2 *const lq = (double2 *)(a.lp.v + (long)pid * PIC_LSTRIDE);
+        const double2 q0 = lq[0], q1 = lq[1], q2 = lq[2];
+        double lx1[3] = {q0.x, q0.y, q1.x};
+        double lp1[3] = {q1.y, q2.x, q2.y};
         double lx[3];
         const double X0[3] = {x, y, z}, X1[3] = {x1, y1, z1};
         /* deposit: J = pqr(i1; x -> e1) + [m != 0] pqr(i2; x2 -> x1) */
@@ -576,12 +578,9 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
             lx[1] -= dsy * a.dc.rhy;
             lx[2] -= dsz * a.dc.rhz;
         }
-        a.lp.v[0][pid] = lx[0];
-        a.lp.v[1][pid] = lx[1];
-        a.lp.v[2][pid] = lx[2];
-        a.lp.v[3][pid] = lpm[0];
-        a.lp.v[4][pid] = lpm[1];
-        a.lp.v[5][pid] = lpm[2];
+        lq[0] = make_double2(lx[0], lx[1]);
+        lq[1] = make_double2(lx[2], lpm[0]);
+        lq[2] = make_double2(lpm[1], lpm[2]);
     }
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();

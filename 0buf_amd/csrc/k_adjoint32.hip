@@ -229,8 +229,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         const int m = R.m;
 
         /* ---- reverse ---- */
-        float lo1[3] = {(float)a.lp.v[0][pid], (float)a.lp.v[1][pid], (float)a.lp.v[2][pid]};
-        float lp1[3] = {(float)a.lp.v[3][pid], (float)a.lp.v[4][pid], (float)a.lp.v[5][pid]};
+        double2 *const lq = (double2 *)(a.lp.v + (long)pid * PIC_LSTRIDE);
+        const double2 q0 = lq[0], q1 = lq[1], q2 = lq[2];
+        float lo1[3] = {(float)q0.x, (float)q0.y, (float)q1.x};
+        float lp1[3] = {(float)q1.y, (float)q2.x, (float)q2.y};
         float lo[3], ldu[3] = {0.0f, 0.0f, 0.0f};
         const float *kj = a.kj[s];
         {
@@ -331,12 +333,9 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
             lo[1] -= dy;
             lo[2] -= dz;
         }
-        a.lp.v[0][pid] = lo[0];
-        a.lp.v[1][pid] = lo[1];
-        a.lp.v[2][pid] = lo[2];
-        a.lp.v[3][pid] = lpm[0];
-        a.lp.v[4][pid] = lpm[1];
-        a.lp.v[5][pid] = lpm[2];
+        lq[0] = make_double2(lo[0], lo[1]);
+        lq[1] = make_double2(lo[2], lpm[0]);
+        lq[2] = make_double2(lpm[1], lpm[2]);
     }
     __syncthreads();
     for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
@@ -357,9 +356,18 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_scale_lam(LamParts lp, double sx,
                                                          double sz) {
     const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
     if (e >= lp.n) return;
-    lp.v[0][e] *= sx;
-    lp.v[1][e] *= sy;
-    lp.v[2][e] *= sz;
+    double *const v = lp.v + e * PIC_LSTRIDE;
+    v[0] *= sx;
+    v[1] *= sy;
+    v[2] *= sz;
+}
+
+__global__ __launch_bounds__(PIC_BLOCK) void k_lam_copy(LamParts lp, double *flat, int k,
+                                                        int to_lines) {
+    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (e >= lp.n) return;
+    if (to_lines) lp.v[e * PIC_LSTRIDE + k] = flat[e];
+    else flat[e] = lp.v[e * PIC_LSTRIDE + k];
 }
 
 __global__ __launch_bounds__(PIC_BLOCK) void k_adj_seed_energy32(Grid g, const float *Ex,
@@ -400,6 +408,11 @@ void launch_adj_particles_f32(const Grid &g, const TileGeom &t, AdjFwd32 fwd, Pa
     a.st = st;
     hipLaunchKernelGGL((k_adj_particles32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
                        dim3(PIC_BLOCK), 0, s, a);
+}
+
+void launch_lam_copy(LamParts lp, double *flat, int k, bool to_lines, hipStream_t s) {
+    hipLaunchKernelGGL(k_lam_copy, dim3((unsigned)((lp.n + PIC_BLOCK - 1) / PIC_BLOCK)),
+                       dim3(PIC_BLOCK), 0, s, lp, flat, k, (int)to_lines);
 }
 
 void launch_scale_lam(LamParts lp, double sx, double sy, double sz, hipStream_t s) {

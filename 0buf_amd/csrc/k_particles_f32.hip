@@ -40,7 +40,6 @@ namespace {
 constexpr int TXC = 2, TYC = 4, TZC = 8;
 constexpr int GS32 = 32;    /* slots of an FP32 array in one 128-byte line */
 constexpr int JCOPIES = PIC_JCOPIES32;
-static_assert(PIC_SPILL_CAP <= 256, "spill records are indexed in 8 bits in LDS");
 
 struct PushArgs32 {
     Grid g;
@@ -113,16 +112,16 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
  * slots [bc + PIC_INBOX, cnt) by line (arrivals past a full inbox).
  * Packed: the particle-array part [0, bc), which starts at slot ps of the
  * tile's region, split where the region's lines begin; the inbox records by
- * 4; the spill records [bc + PIC_INBOX, cnt) by 32 (they are read one by one
- * through the tile's spill index, so their grouping only sets how many
- * entries they take). */
+ * 4; the spill records [bc + PIC_INBOX, cnt) by SPG (they lie in cell order
+ * in the tile's sorted spill list). */
+constexpr int SPG = 128;
 __device__ __forceinline__ int cell_groups32(int cnt, int bc) {
     const int nin = min(PIC_INBOX, cnt - bc), t0 = bc + PIC_INBOX;
     return (bc + GS32 - 1) / GS32 + (nin + 3) / 4 + (cnt > t0 ? (cnt - 1) / GS32 - t0 / GS32 + 1 : 0);
 }
 __device__ __forceinline__ int cell_groups_packed(int ps, int bc, int nin, int nsp) {
     const int nb = bc > 0 ? (ps + bc - 1) / GS32 - ps / GS32 + 1 : 0;
-    return nb + (nin + 3) / 4 + (nsp + GS32 - 1) / GS32;
+    return nb + (nin + 3) / 4 + (nsp + SPG - 1) / SPG;
 }
 
 template <int C>
@@ -154,8 +153,8 @@ __device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, 
                 size = min(4, bc + nin - start);
             } else if (packed) {
                 kind = 1;
-                start = t0 + GS32 * (k - nsoa - ninb);
-                size = min(GS32, cnt - start);
+                start = t0 + SPG * (k - nsoa - ninb);
+                size = min(SPG, cnt - start);
             } else {
                 const int line = t0 / GS32 + (k - nsoa - ninb);
                 start = max(t0, GS32 * line);
@@ -188,11 +187,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     __shared__ int sBc[C];       /* slots of each cell held by the particle arrays */
     __shared__ int sIB[C];       /* first inbox record of each cell, relative to the tile's */
     __shared__ int sG[C + 1];    /* first line group of each cell */
-    __shared__ int sSpN[C];      /* packed: spill records of each cell */
-    __shared__ int sSpOff[C];    /* packed: first entry of each cell in sSpIdx */
-    __shared__ int sSpFill[C];
-    __shared__ unsigned char sSpC[PIC_SPILL_CAP];    /* cell of each spill record */
-    __shared__ unsigned char sSpIdx[PIC_SPILL_CAP];  /* spill records in cell order */
+    __shared__ int sSpOff[C];    /* packed: first spill record of each cell */
     __shared__ int sFirst[NW + 1];  /* first cell of each wavefront */
     __shared__ int sHalo[NB];    /* leavers bound for each cell of the halo box */
     __shared__ int sQ;
@@ -220,8 +215,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     const int L0 = __builtin_amdgcn_readfirstlane(ty * TY);
     const int K0 = __builtin_amdgcn_readfirstlane(tz * TZ);
     const int cap = a.p.cap;
-    const int nsp = packed ? min(a.L.spill_cnt[tile], a.L.scap) : 0;
-    const float4 *const SPB = a.L.spill + 2 * (tile * (long)a.L.scap);
+    const float4 *const SPB = a.L.spill_s + 2 * (tile * (long)a.L.scap);
 
     for (int c = tid; c < C; c += PIC_BLOCK) {
         int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
@@ -238,8 +232,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sCnt[c] = cnt;
         sBc[c] = bc;
         sLeave[c] = 0;
-        sSpN[c] = 0;
-        sSpFill[c] = 0;
         sBase[c] = packed ? ps : ((ca * g.Ny + cb) * g.Nz + cc) * cap;
         sIB[c] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
@@ -266,32 +258,15 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sFlags = 0;
     }
     __syncthreads();
-    /* the tile's spill records counted by cell (a record whose cell lies
-     * outside the tile is not read) */
-    for (int k = tid; k < nsp; k += PIC_BLOCK) {
-        const int lc = key_halo(__float_as_uint(SPB[2 * k + 1].z));
-        if (lc < C) {
-            sSpC[k] = (unsigned char)lc;
-            atomicAdd(&sSpN[lc], 1);
-        } else {
-            sSpC[k] = 255;
-            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
-        }
-    }
-    __syncthreads();
     static_assert(C <= 64, "one wavefront scans the tile's cells");
     if (tid < 64) {   /* first particle, line group and spill entry of each cell */
         int cnt = tid < C ? sCnt[tid] : 0, ngc = 0, nspc = 0;
         if (tid < C) {
             const int bc = sBc[tid], nin = min(PIC_INBOX, cnt - bc);
             if (packed) {
-                /* the cell's slots past the inbox are its spill records */
-                const int want = cnt - bc - nin;
-                nspc = min(sSpN[tid], want);
-                if (sSpN[tid] != want) atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
-                cnt = bc + nin + nspc;
-                sCnt[tid] = cnt;
-                sSpN[tid] = nspc;
+                /* the cell's slots past the inbox are its spill records,
+                 * sorted by cell in the tile's list */
+                nspc = cnt - bc - nin;
                 ngc = cell_groups_packed(sBase[tid], bc, nin, nspc);
             } else {
                 ngc = cell_groups32(cnt, bc);
@@ -300,6 +275,8 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         int tot_s, tot_g, tot_p;
         const int so = wave_excl_scan(cnt, tot_s), sg = wave_excl_scan(ngc, tot_g);
         const int spo = wave_excl_scan(nspc, tot_p);
+        if (tid == 0 && tot_p > a.L.scap)   /* a full list is repaired before a push */
+            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
         if (tid < C) {
             sOff[tid] = so;
             sG[tid] = sg;
@@ -313,12 +290,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     __syncthreads();
     build_groups32<C>(sCnt, sBc, sBase, sG, packed, sGrp, tid);
     if (tid <= NW) sFirst[tid] = wave_first_cell<C>(sOff, tid, NW);
-    for (int k = tid; k < nsp; k += PIC_BLOCK) {
-        const int lc = sSpC[k];
-        if (lc == 255) continue;
-        const int r = atomicAdd(&sSpFill[lc], 1);
-        if (r < sSpN[lc]) sSpIdx[sSpOff[lc] + r] = (unsigned char)k;
-    }
     __syncthreads();
 
     /* per-tile bases of the particle arrays: slot rel of the tile is
@@ -361,12 +332,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         nslot = group_start(e) + (lane - (63 - __clzll(m))) + (gi == 0 ? o0 : 0);
         unsigned nrel = (unsigned)(sBase[cf] + nslot);
         /* record r of the cell: inbox record r below PIC_INBOX, else spill
-         * entry r - PIC_INBOX of the cell (index clamped for the lanes that
-         * do not read one) */
+         * record r - PIC_INBOX of the cell in the tile's sorted list */
         const int r = nslot - sBc[cf];
         const bool spl = r >= PIC_INBOX;
-        const int si = min(max(sSpOff[cf] + r - PIC_INBOX, 0), PIC_SPILL_CAP - 1);
-        unsigned ir = 2u * (unsigned)(spl ? (int)sSpIdx[si] : sIB[cf] + r);
+        unsigned ir = 2u * (unsigned)(spl ? min(sSpOff[cf] + r - PIC_INBOX, a.L.scap - 1)
+                                          : sIB[cf] + r);
         const float4 *const RB = spl ? SPB : IB;
         asm volatile("" : "+v"(nrel));
         asm volatile("" : "+v"(ir));
@@ -626,10 +596,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             d[0] = A;
             d[1] = B;
         } else {
-            const int k = atomicAdd(&a.L.spill_cnt_next[dt], 1);
+            const int k = atomicAdd(&a.L.spill_u_cnt[dt], 1);
             if (k < a.L.scap) {
-                B.z = __uint_as_float((key & 255u) | ((unsigned)lc << 8));
-                float4 *d = a.L.spill_next + 2 * (dt * a.L.scap + k);
+                B.z = __uint_as_float((key & 255u) | ((unsigned)lc << 8) |
+                                      ((unsigned)(r - PIC_INBOX) << 16));
+                float4 *d = a.L.spill_u + 2 * (dt * a.L.scap + k);
                 d[0] = A;
                 d[1] = B;
             } else {
@@ -641,7 +612,6 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     if (tid == 0) {
         if (sQ > a.q.qcap) atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_OVERFLOW);
         a.q.count[tile] = nq;
-        a.L.spill_cnt[tile] = 0;   /* read above; refilled two steps on */
         if (sFlags) atomicOr(&a.st->flags, sFlags);
     }
 }
@@ -687,10 +657,9 @@ __device__ __forceinline__ long halo_cell(const Grid &g, const TileGeom &t, long
     return ((long)gx * g.Ny + gy) * g.Nz + gz;
 }
 
-/* one departure record into bucket slot `slot` of `cell` */
-__device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, float4 A, float4 B,
-                                        unsigned key) {
-    const long to = cell * p.cap + slot;
+/* one departure record into slot `to` of the arrays of p */
+__device__ __forceinline__ void place32_at(PartPtrs32 &p, long to, float4 A, float4 B,
+                                           unsigned key) {
     p.p[0][to] = A.x;
     p.p[1][to] = A.y;
     p.p[2][to] = A.z;
@@ -699,6 +668,11 @@ __device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, floa
     p.p[5][to] = B.y;
     p.sp[to] = (unsigned char)(key & 255u);
     if (p.id) p.id[to] = __float_as_int(B.w);
+}
+/* ... into bucket slot `slot` of `cell`, or into compact slot off[cell] + slot */
+__device__ __forceinline__ void place32(PartPtrs32 &p, long cell, int slot, float4 A, float4 B,
+                                        unsigned key, const long *off = nullptr) {
+    place32_at(p, off ? off[cell] + slot : cell * p.cap + slot, A, B, key);
 }
 
 /* tile and local cell of bulk cell `cell` */
@@ -711,32 +685,75 @@ __device__ __forceinline__ void cell_tile(const Grid &g, const TileGeom &t, long
     lc = ((gi % TX) * TY + gl % TY) * TZ + gk % TZ;
 }
 
-/* after a push: a cell holds its stayers and its arrivals; a count past the
- * capacity raises PIC_F_BUCKET_OVERFLOW (the host unpacks into larger
- * buckets before the next push) */
-__global__ __launch_bounds__(PIC_BLOCK) void k_arrivals_done32(long ncell, PartPtrs32 p,
-                                                              const int *arr, DevStatus *st) {
-    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
-    bool over = false;
-    if (e < ncell) {
-        const int n = p.bcount[e] + arr[e];
-        p.count[e] = n;
-        over = n > p.cap;
+/* after a push, one workgroup per tile: each cell holds its stayers and its
+ * arrivals (a count past the capacity raises PIC_F_BUCKET_OVERFLOW: the host
+ * unpacks into larger buckets before the next push), and the tile's spill
+ * records move from the unsorted list into the sorted one: cell c's record
+ * with arrival index PIC_INBOX + k at the exclusive prefix over the tile's
+ * cells of their spill counts, plus k (no atomics: the position follows
+ * from the record) */
+template <int TX, int TY, int TZ>
+__global__ __launch_bounds__(PIC_BLOCK) void k_arrivals_done32(Grid g, TileGeom t, PartPtrs32 p,
+                                                              Layout32 L, DevStatus *st) {
+    constexpr int C = TX * TY * TZ;
+    static_assert(C <= 64, "one wavefront scans the tile's cells");
+    __shared__ int off[C];
+    const long tile = blockIdx.x;
+    const int tz = (int)(tile % t.ntz), ty = (int)((tile / t.ntz) % t.nty),
+              tx = (int)(tile / ((long)t.ntz * t.nty));
+    if (threadIdx.x < 64) {
+        const int c = threadIdx.x;
+        int nsp = 0;
+        bool over = false;
+        if (c < C) {
+            const int gi = tx * TX + c / (TY * TZ), gl = ty * TY + (c / TZ) % TY,
+                      gk = tz * TZ + c % TZ;
+            if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+                const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+                const int na = L.arr[cell];
+                const int n = p.bcount[cell] + na;
+                p.count[cell] = n;
+                over = n > p.cap;
+                nsp = max(0, na - PIC_INBOX);
+            }
+        }
+        int tot;
+        const int o = wave_excl_scan(nsp, tot);
+        if (c < C) off[c] = o;
+        if (__ballot(over) && c == 0) atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
     }
-    if (__ballot(over) && (threadIdx.x & 63) == 0)
-        atomicOr(&st->flags, (unsigned)PIC_F_BUCKET_OVERFLOW);
+    __syncthreads();
+    const int n = min(L.spill_u_cnt[tile], L.scap);
+    const float4 *const U = L.spill_u + 2 * (tile * (long)L.scap);
+    float4 *const S = L.spill_s + 2 * (tile * (long)L.scap);
+    for (int k = threadIdx.x; k < n; k += PIC_BLOCK) {
+        const float4 A = U[2 * k], B = U[2 * k + 1];
+        const unsigned key = __float_as_uint(B.z);
+        const int lc = key_halo(key);
+        if (lc >= C) continue;
+        const int d = off[lc] + (int)(key >> 16);
+        if (d < L.scap) {
+            S[2 * d] = A;
+            S[2 * d + 1] = B;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) L.spill_u_cnt[tile] = 0;
 }
 
 /* the packed live set into buckets of `to` (capacity to.cap >= every
- * count): slot j of a cell from its tile's region (j < bcount) or from inbox
+ * count), or into the compact arrays of `to` at off[cell] + slot when off is
+ * given: slot j of a cell from its tile's region (j < bcount) or from inbox
  * record j - bcount */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_unpack32(Grid g, TileGeom t, long ncell,
-                                                       PartPtrs32 p, Layout32 L, PartPtrs32 to) {
-    const long e = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
-    if (e >= ncell * to.cap) return;
-    const long cell = e / to.cap;
-    const int j = (int)(e % to.cap);
+                                                       PartPtrs32 p, Layout32 L, PartPtrs32 to,
+                                                       const long *off) {
+    const long i = (long)blockIdx.x * PIC_BLOCK + threadIdx.x;
+    if (i >= ncell * to.cap) return;
+    const long cell = i / to.cap;
+    const int j = (int)(i % to.cap);
+    const long e = off ? off[cell] + j : i;
     const int cnt = min(p.count[cell], to.cap), bc = max(0, min(p.bcount[cell], cnt));
     const int nin = min(PIC_INBOX, cnt - bc);
     if (j < bc) {
@@ -750,35 +767,57 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_unpack32(Grid g, TileGeom t, long
     } else if (j < bc + nin) {
         const long rec = 2 * (cell * PIC_INBOX + (j - bc));
         const float4 A = p.inbox[rec], B = p.inbox[rec + 1];
-        place32(to, cell, j, A, B, __float_as_uint(B.z));
+        place32_at(to, e, A, B, __float_as_uint(B.z));
     }
 }
 
-/* the spill lists into buckets, one workgroup per tile: record k of cell
- * c takes slot bcount + PIC_INBOX + (its rank among c's records) */
+/* the sorted spill lists into buckets, one workgroup per tile: cell c's
+ * k-th spill record (at the exclusive prefix over the tile's cells of their
+ * spill counts, plus k) takes slot bcount + PIC_INBOX + k */
 template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_unspill32(Grid g, TileGeom t, PartPtrs32 p,
-                                                        Layout32 L, PartPtrs32 to) {
+                                                        Layout32 L, PartPtrs32 to,
+                                                        const long *coff) {
     constexpr int C = TX * TY * TZ;
-    __shared__ int fill[C];
+    static_assert(C <= 64, "one wavefront scans the tile's cells");
+    __shared__ int off[C + 1], bcs[C];
+    __shared__ long cellof[C];
     const long tile = blockIdx.x;
-    const int n = min(L.spill_cnt[tile], L.scap);
-    for (int c = threadIdx.x; c < C; c += PIC_BLOCK) fill[c] = 0;
-    __syncthreads();
     const int tz = (int)(tile % t.ntz), ty = (int)((tile / t.ntz) % t.nty),
               tx = (int)(tile / ((long)t.ntz * t.nty));
+    if (threadIdx.x < 64) {
+        const int c = threadIdx.x;
+        int nsp = 0;
+        if (c < C) {
+            const int gi = tx * TX + c / (TY * TZ), gl = ty * TY + (c / TZ) % TY,
+                      gk = tz * TZ + c % TZ;
+            long cell = -1;
+            if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+                cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
+                const int cnt = min(p.count[cell], to.cap), bc = max(0, min(p.bcount[cell], cnt));
+                nsp = max(0, cnt - bc - PIC_INBOX);
+                bcs[c] = bc;
+            }
+            cellof[c] = cell;
+        }
+        int tot;
+        const int o = wave_excl_scan(nsp, tot);
+        if (c < C) off[c] = o;
+        if (c == 0) off[C] = tot;
+    }
+    __syncthreads();
+    const int n = min(off[C], L.scap);
+    const float4 *const S = L.spill_s + 2 * (tile * (long)L.scap);
     for (int k = threadIdx.x; k < n; k += PIC_BLOCK) {
-        const float4 A = L.spill[2 * (tile * L.scap + k)], B = L.spill[2 * (tile * L.scap + k) + 1];
-        const unsigned key = __float_as_uint(B.z);
-        const int lc = key_halo(key);
-        if (lc >= C) continue;
-        const int gi = tx * TX + lc / (TY * TZ), gl = ty * TY + (lc / TZ) % TY,
-                  gk = tz * TZ + lc % TZ;
-        if (gi >= g.Nx || gl >= g.Ny || gk >= g.Nz) continue;
-        const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
-        const int cnt = min(p.count[cell], to.cap), bc = max(0, min(p.bcount[cell], cnt));
-        const int slot = bc + PIC_INBOX + atomicAdd(&fill[lc], 1);
-        if (slot < cnt) place32(to, cell, slot, A, B, key);
+        int lo = 0, hi = C;   /* the cell whose range holds k: off[lo] <= k < off[lo + 1] */
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= k) lo = mid; else hi = mid;
+        }
+        if (cellof[lo] < 0) continue;
+        const float4 A = S[2 * k], B = S[2 * k + 1];
+        place32(to, cellof[lo], bcs[lo] + PIC_INBOX + (k - off[lo]), A, B, __float_as_uint(B.z),
+                coff);
     }
 }
 
@@ -812,7 +851,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_unpack_done32(long ncell, long nt
         p.bcount[e] = min(p.count[e], cap);
         L.arr[e] = 0;
     }
-    if (e < ntiles) L.spill_cnt[e] = 0;
+    if (e < ntiles) L.spill_u_cnt[e] = 0;
 }
 
 __global__ void k_clear_flags32(DevStatus *st, unsigned mask) { st->flags &= ~mask; }
@@ -963,7 +1002,7 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
      * starts inside a line, inbox lines; per tile: spill groups) */
     const size_t lds = (size_t)4 * (TXC * TYC * TZC * (2 * ((p.cap + GS32 - 1) / GS32) +
                                                      PIC_INBOX / 4 + 2) +
-                                    PIC_SPILL_CAP / GS32 + 1);
+                                    L.scap / SPG + 1);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), lds, s, a);
@@ -972,10 +1011,10 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                            dim3(PIC_BLOCK), lds, s, a);
 }
 
-void launch_arrivals_done32(long ncell, PartPtrs32 p, Layout32 L, DevStatus *st,
-                            hipStream_t s) {
-    hipLaunchKernelGGL(k_arrivals_done32, dim3(blocks_for(ncell)), dim3(PIC_BLOCK), 0, s, ncell,
-                       p, (const int *)L.arr, st);
+void launch_arrivals_done32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                            DevStatus *st, hipStream_t s) {
+    hipLaunchKernelGGL((k_arrivals_done32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
+                       dim3(PIC_BLOCK), 0, s, g, t, p, L, st);
 }
 
 void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
@@ -985,12 +1024,27 @@ void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
         hipLaunchKernelGGL((k_place_spills32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
                            dim3(PIC_BLOCK), 0, s, g, t, p, *from_queue, to);
     hipLaunchKernelGGL((k_unpack32<TXC, TYC, TZC>), dim3(blocks_for(ncell * to.cap)),
-                       dim3(PIC_BLOCK), 0, s, g, t, ncell, p, L, to);
+                       dim3(PIC_BLOCK), 0, s, g, t, ncell, p, L, to, (const long *)nullptr);
     if (!from_queue)
         hipLaunchKernelGGL((k_unspill32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), 0, s, g, t, p, L, to);
+                           dim3(PIC_BLOCK), 0, s, g, t, p, L, to, (const long *)nullptr);
     hipLaunchKernelGGL(k_unpack_done32, dim3(blocks_for(std::max(ncell, t.ntiles))),
                        dim3(PIC_BLOCK), 0, s, ncell, t.ntiles, p, L, to.cap);
+}
+
+void launch_gather_packed32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                            const long *offsets, float *dst[6], unsigned char *dsp, int *did,
+                            hipStream_t s) {
+    const long ncell = (long)g.Nx * g.Ny * g.Nz;
+    PartPtrs32 to{};
+    for (int k = 0; k < 6; k++) to.p[k] = dst[k];
+    to.sp = dsp;
+    to.id = did;
+    to.cap = p.cap;
+    hipLaunchKernelGGL((k_unpack32<TXC, TYC, TZC>), dim3(blocks_for(ncell * to.cap)),
+                       dim3(PIC_BLOCK), 0, s, g, t, ncell, p, L, to, offsets);
+    hipLaunchKernelGGL((k_unspill32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(PIC_BLOCK),
+                       0, s, g, t, p, L, to, offsets);
 }
 
 void launch_clear_arrivals32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,

@@ -72,6 +72,7 @@ struct AdjointStore {
     void *Eend[3] = {nullptr, nullptr, nullptr};   /* E at the end of the segment */
     AdjLam lam{};
     LamParts lp{};
+    double *lp_flat = nullptr;        /* one component of every id (host transfers) */
     long lam_ni = 0;                  /* particle adjoint entries allocated */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     double ms[3] = {0, 0, 0};
@@ -110,8 +111,9 @@ void free_lam(AdjointStore *s) {
         if (s->Eend[k]) (void)hipFree(s->Eend[k]);
         s->Eend[k] = nullptr;
     }
-    for (int k = 0; k < 6; k++)
-        if (s->lp.v[k]) (void)hipFree(s->lp.v[k]);
+    if (s->lp.v) (void)hipFree(s->lp.v);
+    if (s->lp_flat) (void)hipFree(s->lp_flat);
+    s->lp_flat = nullptr;
     s->lam = AdjLam{};
     s->lp = LamParts{};
 }
@@ -147,7 +149,9 @@ int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     const long NC = c->NC;
     const size_t es = elem(c);
     std::vector<int> cnt(NC);
-    int rc0 = settle(c);
+    /* FP32 form: the packed set is read where it lies (no unpack), once the
+     * last arrival's status is examined */
+    int rc0 = c->fp32 ? (c->arrival_pending ? finish_step(c) : PIC_SUCCESS) : settle(c);
     if (rc0) return rc0;
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(cnt.data(), live_count(c), NC * sizeof(int), hipMemcpyDeviceToHost));
@@ -162,7 +166,11 @@ int save_state(pic_ctx *c, Compact &dst, FieldSet &f) {
     if (c->fp32) {
         float *v[6];
         for (int j = 0; j < 6; j++) v[j] = (float *)dst.v[j];
-        launch_gather_raw32(NC, dst.off, c->p32, v, dst.sp, dst.id, c->stream);
+        if (c->lay32.packed)
+            launch_gather_packed32(c->g, c->t, c->p32, c->lay32, dst.off, v, dst.sp, dst.id,
+                                   c->stream);
+        else
+            launch_gather_raw32(NC, dst.off, c->p32, v, dst.sp, dst.id, c->stream);
     } else {
         double *v[6];
         for (int j = 0; j < 6; j++) v[j] = (double *)dst.v[j];
@@ -433,22 +441,24 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
             HIPCHK(hipMalloc(&s->lam.Q[k], G * sizeof(double)));
             HIPCHK(hipMalloc(&s->Eend[k], G * elem(c)));
         }
-        for (int k = 0; k < 6; k++) HIPCHK(hipMalloc(&s->lp.v[k], NI * sizeof(double)));
+        HIPCHK(hipMalloc(&s->lp.v, NI * PIC_LSTRIDE * sizeof(double)));
+        HIPCHK(hipMalloc(&s->lp_flat, NI * sizeof(double)));
         s->lam_ni = NI;
     }
     for (int k = 0; k < 3; k++) {
         if ((rc = zero_async(s->lam.E[k], G * sizeof(double), st))) return rc;
         if ((rc = zero_async(s->lam.H[k], G * sizeof(double), st))) return rc;
     }
-    for (int k = 0; k < 6; k++)
-        if ((rc = zero_async(s->lp.v[k], NI * sizeof(double), st))) return rc;
+    if ((rc = zero_async(s->lp.v, NI * PIC_LSTRIDE * sizeof(double), st))) return rc;
     s->lp.n = c->n_ids;
     if (seed) {
         const double *ps[6] = {seed->x, seed->y, seed->z, seed->px, seed->py, seed->pz};
         for (int k = 0; k < 6; k++)
-            if (ps[k] && c->n_ids)
-                HIPCHK(hipMemcpyAsync(s->lp.v[k], ps[k], c->n_ids * sizeof(double),
+            if (ps[k] && c->n_ids) {
+                HIPCHK(hipMemcpyAsync(s->lp_flat, ps[k], c->n_ids * sizeof(double),
                                       hipMemcpyHostToDevice, st));
+                launch_lam_copy(s->lp, s->lp_flat, k, true, st);
+            }
         for (int k = 0; k < 6; k++)
             if (seed->field[k])
                 HIPCHK(hipMemcpyAsync(k < 3 ? s->lam.E[k] : s->lam.H[k - 3], seed->field[k],
@@ -577,9 +587,11 @@ int pic_adjoint(pic_ctx *c, const double *weights, const pic_adjoint_io *seed,
     if (grad) {
         double *pg[6] = {grad->x, grad->y, grad->z, grad->px, grad->py, grad->pz};
         for (int k = 0; k < 6; k++)
-            if (pg[k] && c->n_ids)
-                HIPCHK(hipMemcpyAsync(pg[k], s->lp.v[k], c->n_ids * sizeof(double),
+            if (pg[k] && c->n_ids) {
+                launch_lam_copy(s->lp, s->lp_flat, k, false, st);
+                HIPCHK(hipMemcpyAsync(pg[k], s->lp_flat, c->n_ids * sizeof(double),
                                       hipMemcpyDeviceToHost, st));
+            }
         for (int k = 0; k < 6; k++)
             if (grad->field[k])
                 HIPCHK(hipMemcpyAsync(grad->field[k], k < 3 ? s->lam.E[k] : s->lam.H[k - 3],

@@ -68,8 +68,22 @@ long live_slots32(const pic_ctx *c) {
     return std::max(c->NC, c->t.ntiles * (long)(c->t.TX * c->t.TY * c->t.TZ));
 }
 
+/* FP32 form: the unsorted and sorted spill lists of every tile, scap
+ * records each (empty: no arrival pending) */
+int alloc_spill32(pic_ctx *c, int scap) {
+    Layout32 &L = c->lay32;
+    if (L.spill_u) (void)hipFree(L.spill_u);
+    if (L.spill_s) (void)hipFree(L.spill_s);
+    L.spill_u = L.spill_s = nullptr;
+    const size_t bytes = (size_t)c->t.ntiles * scap * 2 * sizeof(float4);
+    HIPCHK(hipMalloc(&L.spill_u, bytes));
+    HIPCHK(hipMalloc(&L.spill_s, bytes));
+    L.scap = scap;
+    return PIC_SUCCESS;
+}
+
 /* FP32 form, once per context: the per-cell counts, both inboxes and
- * arrival counters, the packed starts and both spill lists per tile */
+ * arrival counters, the packed starts and the spill lists per tile */
 int alloc_cells32(pic_ctx *c) {
     PartPtrs32 &p = c->p32;
     Layout32 &L = c->lay32;
@@ -85,16 +99,10 @@ int alloc_cells32(pic_ctx *c) {
     HIPCHK(hipMalloc(&L.arr_next, NC * sizeof(int)));
     HIPCHK(hipMemset(L.arr, 0, NC * sizeof(int)));
     HIPCHK(hipMemset(L.arr_next, 0, NC * sizeof(int)));
-    L.scap = PIC_SPILL_CAP;
-    const size_t spill = (size_t)nt * L.scap * 2 * sizeof(float4);
-    HIPCHK(hipMalloc(&L.spill, spill));
-    HIPCHK(hipMalloc(&L.spill_next, spill));
-    HIPCHK(hipMalloc(&L.spill_cnt, nt * sizeof(int)));
-    HIPCHK(hipMalloc(&L.spill_cnt_next, nt * sizeof(int)));
-    HIPCHK(hipMemset(L.spill_cnt, 0, nt * sizeof(int)));
-    HIPCHK(hipMemset(L.spill_cnt_next, 0, nt * sizeof(int)));
+    HIPCHK(hipMalloc(&L.spill_u_cnt, nt * sizeof(int)));
+    HIPCHK(hipMemset(L.spill_u_cnt, 0, nt * sizeof(int)));
     L.packed = 0;
-    return PIC_SUCCESS;
+    return alloc_spill32(c, PIC_SPILL_CAP);
 }
 
 /* FP32 form: the particle arrays of one set (p, sp, id), capacity cap */
@@ -133,8 +141,7 @@ int alloc_live32(pic_ctx *c, int cap) {
 
 void free_layout32(Layout32 &L) {
     for (void *q : {(void *)L.pstart, (void *)L.arr, (void *)L.arr_next, (void *)L.inbox_next,
-                    (void *)L.spill, (void *)L.spill_next, (void *)L.spill_cnt,
-                    (void *)L.spill_cnt_next})
+                    (void *)L.spill_u, (void *)L.spill_s, (void *)L.spill_u_cnt})
         if (q) (void)hipFree(q);
     L = Layout32{};
 }
@@ -144,8 +151,6 @@ void swap_arrivals32(pic_ctx *c) {
     Layout32 &L = c->lay32;
     std::swap(c->p32.inbox, L.inbox_next);
     std::swap(L.arr, L.arr_next);
-    std::swap(L.spill, L.spill_next);
-    std::swap(L.spill_cnt, L.spill_cnt_next);
 }
 
 /* the live and the other FP32 set trade places (per-cell arrays stay) */
@@ -512,7 +517,7 @@ void enq_push(pic_ctx *c) {
 void enq_arrive(pic_ctx *c, hipStream_t st) {
     if (c->fp32) {
         /* the push placed the arrivals; the counts take them in */
-        launch_arrivals_done32(c->NC, c->p32, c->lay32, c->d_st, st);
+        launch_arrivals_done32(c->g, c->t, c->p32, c->lay32, c->d_st, st);
         c->arrival_pending = true;
     }
     else launch_arrive_f64(c->g, c->t, c->p, c->q, c->d_st, st);
@@ -624,7 +629,15 @@ int finish_step(pic_ctx *c) {
         if (st.flags & PIC_F_BUCKET_OVERFLOW) rc = regrow32(c, over);
         else if (over) rc = despill(c);
         if (rc) return rc;
-        if (over) c->spill_repairs++;
+        if (over) {
+            /* the lists are empty now; larger ones for the steps to come
+             * (and for a recompute of these steps) */
+            c->spill_repairs++;
+            HIPCHK(hipStreamSynchronize(c->stream));
+            if (c->lay32.scap < PIC_SPILL_CAP_MAX &&
+                (rc = alloc_spill32(c, std::min(4 * c->lay32.scap, PIC_SPILL_CAP_MAX))))
+                return rc;
+        }
         c->last.spill_repairs = (int)c->spill_repairs;
     } else if (!c->fp32 && (st.flags & PIC_F_BUCKET_OVERFLOW)) {
         int rc = regrow_parts(c, c->p, c->q);

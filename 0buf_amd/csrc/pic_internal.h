@@ -113,25 +113,30 @@ struct PartPtrs32 {
  * The push also places its tile's leavers (the arrival): per destination
  * cell one atomic on arr_next claims a run of arrival indices r, and record
  * r goes to the cell's inbox (r < PIC_INBOX, inbox_next) or past it to the
- * spill list of the destination tile (spill_next: scap records of the
- * departure-record format, key = species | local cell << 8).  The arrays
- * the push reads (arr, the live inbox, spill) hold the previous step's
- * arrivals; the host swaps them with the *_next ones after each push.  A
- * cell holds bcount + arr particles: bcount in the particle arrays, then
- * its arrivals in order.  The host turns the packed set back into buckets
- * (unpack) for every other reader, and when a spill list or a bucket
- * overflows. */
+ * destination tile's unsorted spill list (spill_u: scap records of the
+ * departure-record format, key = species | local cell << 8 | (r -
+ * PIC_INBOX) << 16).  After the push, one workgroup per tile sums each
+ * cell's count (bcount + arrivals) and moves its spill records into the
+ * sorted list (spill_s): cell c's k-th record at the exclusive prefix over
+ * the tile's cells of max(0, arrivals - PIC_INBOX), plus k.  The arrays the
+ * push reads (arr, the live inbox, spill_s) hold the previous step's
+ * arrivals; the host swaps the per-cell ones with the *_next ones after
+ * each push.  A cell holds bcount + arr particles: bcount in the particle
+ * arrays, then its arrivals in order.  The host turns the packed set back
+ * into buckets (unpack) for every other reader, and when a spill list or a
+ * bucket overflows (a spill list then grows). */
 struct Layout32 {
     int packed;              /* the set being read is packed (else buckets) */
     int *pstart;             /* per cell: first slot in its tile's region */
     int *arr, *arr_next;     /* per cell: arrivals of the last step / being placed */
     float4 *inbox_next;      /* per cell: PIC_INBOX records being placed */
-    float4 *spill, *spill_next;  /* per tile: scap records (2 float4 each) */
-    int *spill_cnt, *spill_cnt_next;  /* per tile: records claimed (may exceed scap) */
+    float4 *spill_u, *spill_s;  /* per tile: scap records (2 float4 each) */
+    int *spill_u_cnt;        /* per tile: records claimed in spill_u (may exceed scap) */
     int scap;
     long tcap;               /* slots per tile region */
 };
-constexpr int PIC_SPILL_CAP = 256;  /* spill records per tile (indices kept in 8 bits) */
+constexpr int PIC_SPILL_CAP = 256;  /* initial spill records per tile */
+constexpr int PIC_SPILL_CAP_MAX = 1 << 16;
 
 /* FP32 departure queue: one 32-byte record per leaver, two float4
  * (ox, oy, oz, pu) and (pv, pw, key, id), with key = species | h << 8 |
@@ -436,9 +441,10 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
                              PartPtrs32 out, Layout32 L, QueuePtrs32 q, SpeciesTable sp,
                              int dep_cap, DevStatus *st, const PartPtrs32 *diff,
                              const long *snap_off, long snap_room, hipStream_t s);
-/* after the push: count = bcount + arrivals (PIC_F_BUCKET_OVERFLOW past cap) */
-void launch_arrivals_done32(long ncell, PartPtrs32 p, Layout32 L, DevStatus *st,
-                            hipStream_t s);
+/* after the push, per tile: count = bcount + arrivals (PIC_F_BUCKET_OVERFLOW
+ * past cap) and the spill records sorted by cell */
+void launch_arrivals_done32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                            DevStatus *st, hipStream_t s);
 /* the packed live set p, its inbox records and its arrivals past the inbox
  * (from the spill lists, or from the departure queues of the last push when
  * from_queue is set: the repair after a full spill list) into buckets of
@@ -450,6 +456,12 @@ void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
 void launch_clear_arrivals32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
                              hipStream_t s);
 void launch_clear_flags32(DevStatus *st, unsigned mask, hipStream_t s);
+/* the packed live set with its pending arrivals, compacted cell by cell at
+ * offsets[cell] (as launch_gather_raw32 does from buckets); the live set
+ * stays as it is */
+void launch_gather_packed32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
+                            const long *offsets, float *dst[6], unsigned char *dsp, int *did,
+                            hipStream_t s);
 void launch_scatter_load32(const long *offsets, long ncell, const float *src[6],
                            const unsigned char *srcsp, PartPtrs32 p, hipStream_t s);
 void launch_gather_particles32(const Grid &g, long ncell, const long *offsets, PartPtrs32 p,
@@ -501,10 +513,17 @@ struct AdjFwd {             /* forward fields the particle stage read */
     const double *E[3];     /* E^n */
     const double *H[3];     /* H^{n+1/2} = H^n + Q */
 };
+/* the particle adjoints by id, one 64-byte line per id: v[id * PIC_LSTRIDE
+ * + k] for k = d/dx, d/dy, d/dz, d/dpu, d/dpv, d/dpw (the reverse sweep
+ * reads and writes each particle's six together, at the id's place) */
+constexpr int PIC_LSTRIDE = 8;
 struct LamParts {
-    double *v[6];           /* d/dx, d/dy, d/dz, d/dpu, d/dpv, d/dpw */
+    double *v;
     long n;
 };
+/* component k of every id between the adjoint lines and a flat array
+ * (to_lines: flat -> lines) */
+void launch_lam_copy(LamParts lp, double *flat, int k, bool to_lines, hipStream_t s);
 void launch_adj_seed_energy(const Grid &g, const double *const E[3], double w, AdjLam l,
                             hipStream_t s);
 void launch_adj_e_periodic(const Grid &g, AdjLam l, hipStream_t s);
