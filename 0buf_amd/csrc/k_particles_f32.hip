@@ -693,8 +693,8 @@ __device__ __forceinline__ void cell_tile(const Grid &g, const TileGeom &t, long
  * cells of their spill counts, plus k (no atomics: the position follows
  * from the record) */
 template <int TX, int TY, int TZ>
-__global__ __launch_bounds__(PIC_BLOCK) void k_arrivals_done32(Grid g, TileGeom t, PartPtrs32 p,
-                                                              Layout32 L, DevStatus *st) {
+__global__ __launch_bounds__(64) void k_arrivals_done32(Grid g, TileGeom t, PartPtrs32 p,
+                                                       Layout32 L, DevStatus *st) {
     constexpr int C = TX * TY * TZ;
     static_assert(C <= 64, "one wavefront scans the tile's cells");
     __shared__ int off[C];
@@ -726,7 +726,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_arrivals_done32(Grid g, TileGeom 
     const int n = min(L.spill_u_cnt[tile], L.scap);
     const float4 *const U = L.spill_u + 2 * (tile * (long)L.scap);
     float4 *const S = L.spill_s + 2 * (tile * (long)L.scap);
-    for (int k = threadIdx.x; k < n; k += PIC_BLOCK) {
+    for (int k = threadIdx.x; k < n; k += 64) {
         const float4 A = U[2 * k], B = U[2 * k + 1];
         const unsigned key = __float_as_uint(B.z);
         const int lc = key_halo(key);
@@ -1013,8 +1013,8 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
 
 void launch_arrivals_done32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,
                             DevStatus *st, hipStream_t s) {
-    hipLaunchKernelGGL((k_arrivals_done32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles),
-                       dim3(PIC_BLOCK), 0, s, g, t, p, L, st);
+    hipLaunchKernelGGL((k_arrivals_done32<TXC, TYC, TZC>), dim3((unsigned)t.ntiles), dim3(64),
+                       0, s, g, t, p, L, st);
 }
 
 void launch_unpack32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L, PartPtrs32 to,
