@@ -197,7 +197,7 @@ def fd_check(cfg=3, deltas=(1e-5,), ghost_mode=1, theta=None, first=150, last=20
             fd[i] = (bp.loss_value(torch.tensor(base + e), first, last) -
                      bp.loss_value(torch.tensor(base - e), first, last)) / (2 * d)
         fds[d] = fd
-    return {"theta": base, "grad": grad, "fd": fds, "loss": float(L)}
+    return {"theta": base, "grad": grad, "fd": fds, "loss": float(L.detach())}
 
 
 def fd_check_frozen(cfg=3, deltas=(1e-5,), ghost_mode=1, theta=None, first=150, last=200):
@@ -232,7 +232,7 @@ def fd_check_frozen(cfg=3, deltas=(1e-5,), ghost_mode=1, theta=None, first=150, 
             fd[i] = (bp.loss_value(torch.tensor(base + e), first, last, tape=tape) -
                      bp.loss_value(torch.tensor(base - e), first, last, tape=tape)) / (2 * d)
         fds[d] = fd
-    return {"theta": base, "grad": grad, "fd": fds, "loss": float(L), "loss_replay": L_replay}
+    return {"theta": base, "grad": grad, "fd": fds, "loss": float(L.detach()), "loss_replay": L_replay}
 
 
 def default_theta(cfg):
@@ -261,7 +261,7 @@ def precision_compare(cfg=4, theta=None, first=150, last=200, ckpt_every=20, sca
         th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
         L = bp.loss(th, first, last)
         L.backward()
-        out[prec] = {"loss": float(L), "grad": th.grad.numpy().copy(),
+        out[prec] = {"loss": float(L.detach()), "grad": th.grad.numpy().copy(),
                      "ms": bp.run.last.adjoint_timing(), "n": int(bp.beam.numel())}
         bp.run.last.close()
         bp.run.last = None

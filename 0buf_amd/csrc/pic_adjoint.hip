@@ -35,6 +35,7 @@ void enq_fold(pic_ctx *c);
 void enq_h2(pic_ctx *c);
 void enq_e(pic_ctx *c);
 void enq_e_periodic(pic_ctx *c);
+void enq_fold_h2_e(pic_ctx *c);
 }  // namespace picx
 
 using namespace picx;
@@ -248,10 +249,7 @@ int recorded_step(pic_ctx *c, AdjointStore *s, int r) {
             }
         }
     }
-    enq_fold(c);
-    enq_h2(c);
-    enq_e(c);
-    enq_e_periodic(c);
+    enq_fold_h2_e(c);
     HIPCHK(hipGetLastError());
     return PIC_SUCCESS;
 }

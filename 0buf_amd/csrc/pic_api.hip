@@ -539,6 +539,19 @@ void enq_e_periodic(pic_ctx *c) {
     if (c->fp32) launch_e_periodic32(c->g, c->f32, c->stream);
     else launch_e_periodic(c->g, c->f, c->stream);
 }
+/* the step's field update after the push in one pass: current fold, H
+ * second half, E update and periodic E ghosts (equal to enq_fold, enq_h2,
+ * enq_e, enq_e_periodic bit for bit), H^{n+1} landing in the second H
+ * arrays, which then become the live ones */
+void enq_fold_h2_e(pic_ctx *c) {
+    if (c->fp32) {
+        launch_fold_h2_e32(c->g, c->f32, c->h_alt32, c->stream);
+        for (int k = 0; k < 3; k++) std::swap(c->f32.H[k], c->h_alt32[k]);
+    } else {
+        launch_fold_h2_e(c->g, c->f, c->h_alt, c->stream);
+        for (int k = 0; k < 3; k++) std::swap(c->f.f[PIC_HX + k], c->h_alt[k]);
+    }
+}
 void enq_energy(pic_ctx *c) {
     if (c->fp32) launch_energy32(c->g, c->f32, c->d_part, c->npart, c->d_red, c->stream);
     else launch_energy(c->g, c->f, c->d_part, c->npart, c->d_red, c->stream);
@@ -550,19 +563,30 @@ int check_ready(pic_ctx *c) {
     return PIC_SUCCESS;
 }
 
+/* stage timing: an event at the start of each stage that has kernels of
+ * its own; a stage whose work a fused pass does is not marked and counts 0 */
 inline void mark(pic_ctx *c, int k) {
-    if (c->prof) (void)hipEventRecord(c->ev[k], c->stream);
+    if (!c->prof) return;
+    (void)hipEventRecord(c->ev[k], c->stream);
+    c->marked |= 1u << k;
 }
 
 void accumulate_profile(pic_ctx *c) {
     if (!c->prof) return;
     for (int k = 0; k < PIC_NSTAGES; k++) {
+        if (!(c->marked & (1u << k))) {
+            c->prof_n[k] += 1;
+            continue;
+        }
+        int j = k + 1;
+        while (j < PIC_NSTAGES && !(c->marked & (1u << j))) j++;
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[k + 1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev[k], c->ev[j]) == hipSuccess) {
             c->prof_ms[k] += ms;
             c->prof_n[k] += 1;
         }
     }
+    c->marked = 0;
 }
 
 /* one whole step enqueued on the context stream (Plasma::Step,
@@ -586,19 +610,12 @@ int enq_step(pic_ctx *c) {
     } else {
         enq_arrive(c, c->stream);
     }
-    mark(c, PIC_STAGE_FOLD_J);
-    enq_fold(c);
-    mark(c, PIC_STAGE_H2);
-    enq_h2(c);
+    /* current fold, H second half, E update and the periodic E ghosts in
+     * one pass: its time is the E stage's (FOLD_J, H2, E_PERIODIC: 0) */
     mark(c, PIC_STAGE_E);
-    enq_e(c);
-    /* the periodic E ghosts fused with the energy pass: the same values as
-     * the stage entry points' separate launches (k_fields.hip) */
-    mark(c, PIC_STAGE_E_PERIODIC);
-    if (c->fp32) launch_ghosts_energy32(c->g, c->f32, c->d_part, c->npart, c->stream);
-    else launch_ghosts_energy(c->g, c->f, c->d_part, c->npart, c->stream);
+    enq_fold_h2_e(c);
     mark(c, PIC_STAGE_ENERGY);
-    launch_reduce_final(c->d_part, c->npart, c->d_red, c->stream);
+    enq_energy(c);
     mark(c, PIC_NSTAGES);
     if (overlap) HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
     HIPCHK(hipGetLastError());
@@ -718,6 +735,8 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
     if (!prm || !out) return PIC_ERR_INVALID;
     *out = nullptr;
     if (prm->nx < 3 || prm->ny < 3 || prm->nz < 3) return PIC_ERR_INVALID;
+    /* the field kernels index the ghosted nodes in 32 bits */
+    if ((long)(prm->nx + 2) * (prm->ny + 2) * (prm->nz + 2) >= (1L << 31)) return PIC_ERR_INVALID;
     if (!(prm->lx > 0 && prm->ly > 0 && prm->lz > 0 && prm->tau > 0)) return PIC_ERR_INVALID;
     if (prm->precision != PIC_FP64 && prm->precision != PIC_FP32) return PIC_ERR_INVALID;
     if (prm->ghost_mode != 0 && prm->ghost_mode != 1) return PIC_ERR_INVALID;
@@ -784,6 +803,14 @@ int pic_create(const pic_params *prm, pic_ctx **out) {
         e = hipMemset(*ptr, 0, g.G * esz);
         if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
     }
+    for (int k = 0; k < 3; k++) {
+        void **ptr = c->fp32 ? (void **)&c->h_alt32[k] : (void **)&c->h_alt[k];
+        const size_t esz = c->fp32 ? sizeof(float) : sizeof(double);
+        e = hipMalloc(ptr, g.G * esz);
+        if (e != hipSuccess) return fail(record_hip(e, "hipMalloc fields"));
+        e = hipMemset(*ptr, 0, g.G * esz);
+        if (e != hipSuccess) return fail(record_hip(e, "hipMemset fields"));
+    }
     c->diff = prm->differentiable != 0;
     for (int k = 0; c->diff && !c->fp32 && k < 3; k++) {
         if ((e = hipMalloc(&c->d_jfix[k], g.G * sizeof(long long))) != hipSuccess)
@@ -816,6 +843,10 @@ int pic_destroy(pic_ctx *c) {
         if (c->f32.H[k]) (void)hipFree(c->f32.H[k]);
         if (c->f32.Q[k]) (void)hipFree(c->f32.Q[k]);
         if (c->f32.J[k]) (void)hipFree(c->f32.J[k]);
+    }
+    for (int k = 0; k < 3; k++) {
+        if (c->h_alt[k]) (void)hipFree(c->h_alt[k]);
+        if (c->h_alt32[k]) (void)hipFree(c->h_alt32[k]);
     }
     free_adjoint(c);
     for (int k = 0; k < 3; k++)

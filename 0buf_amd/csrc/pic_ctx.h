@@ -43,6 +43,10 @@ struct pic_ctx {
     PartPtrs p{};
     QueuePtrs q{};
     FieldPtrs32 f32{};      /* FP32 precision form */
+    /* second H arrays: the step's fused H / E pass writes H^{n+1} there and
+     * swaps them with the live ones (f.f[PIC_HX..] or f32.H) */
+    double *h_alt[3] = {nullptr, nullptr, nullptr};
+    float *h_alt32[3] = {nullptr, nullptr, nullptr};
     PartPtrs32 p32{};       /* the live set (count, bcount, inbox: per cell) */
     PartPtrs32 alt32{};     /* the other set the push writes (p, sp, id) */
     Layout32 lay32{};       /* packed layout state; lay32.packed: p32 holds it */
@@ -92,6 +96,7 @@ struct pic_ctx {
     hipEvent_t ev[PIC_NSTAGES + 1] = {};
     double prof_ms[PIC_NSTAGES] = {0};
     long prof_n[PIC_NSTAGES] = {0};
+    unsigned marked = 0;    /* stages marked in the step being timed */
 };
 
 namespace picx {

@@ -363,12 +363,12 @@ struct SpeciesTable {
 };
 
 /* kernel launchers (k_fields.hip) */
-/* the step's fused pass: the periodic E ghosts with the energy partials
- * (as launch_e_periodic followed by launch_energy's first kernel) */
-void launch_ghosts_energy(const Grid &g, FieldPtrs f, double *partials, int nblocks,
-                          hipStream_t s);
-void launch_ghosts_energy32(const Grid &g, FieldPtrs32 f, double *partials, int nblocks,
-                            hipStream_t s);
+/* the step's fused pass after the push: the current fold, H^{n+1} into
+ * h_out (the second H arrays; the caller swaps them with the live ones), the
+ * E update and the periodic E ghosts, equal to launch_fold_j, launch_h2,
+ * launch_e, launch_e_periodic bit for bit */
+void launch_fold_h2_e(const Grid &g, FieldPtrs f, double *const h_out[3], hipStream_t s);
+void launch_fold_h2_e32(const Grid &g, FieldPtrs32 f, float *const h_out[3], hipStream_t s);
 /* sum of nblocks partials into *out (deterministic tree) */
 void launch_reduce_final(const double *partials, int nblocks, double *out, hipStream_t s);
 void launch_h_half(const Grid &g, FieldPtrs f, hipStream_t s);

@@ -35,9 +35,9 @@ def cfg0(picmod):
     return picmod.generate_workload(0)
 
 
-def fp32_pic(picmod, params):
+def fp32_pic(picmod, params, **kw):
     p = picmod.make_params((params.nx, params.ny, params.nz), (params.lx, params.ly, params.lz),
-                           params.tau, precision=picmod.FP32)
+                           params.tau, precision=picmod.FP32, **kw)
     return picmod.Pic(p)
 
 
@@ -236,17 +236,20 @@ def test_fp32_full_inbox_matches_fp64(picmod):
         assert np.abs(p32[k] - p64[k]).max() <= 1e-5, k
 
 
-def test_fp32_step_paths_bit_identical(picmod, cfg0):
+@pytest.mark.parametrize("ghost_mode", [0, 1])
+def test_fp32_step_paths_bit_identical(picmod, cfg0, ghost_mode):
     """The step runs its arrival on a second stream beside the field stages
-    unless stage timing is on; stage-by-stage calls run everything on one
-    stream.  With the order-free integer current all three give the same
-    energies, fields and particles, bit for bit."""
+    unless stage timing is on, and its H second half, E update and periodic
+    E ghosts as one fused pass; stage-by-stage calls run the separate
+    kernels on one stream.  With the order-free integer current all three
+    give the same energies, fields and particles, bit for bit, with either
+    ghost treatment."""
     from helpers import sorted_particles
     params, parts = cfg0
     names = ("yee_h_half", "push_deposit", "migrate", "fold_j", "yee_h2", "yee_e", "e_periodic")
     runs = []
     for mode in ("overlap", "timed", "stages"):
-        g = fp32_pic(picmod, params)
+        g = fp32_pic(picmod, params, ghost_mode=ghost_mode)
         load_workload(g, parts)
         if mode == "timed":
             g.profile(True)
@@ -258,7 +261,7 @@ def test_fp32_step_paths_bit_identical(picmod, cfg0):
                 e.append(g.energy())
             else:
                 e.append(g.step(nt))
-        runs.append((np.array(e), [g.get_field(i) for i in range(9)], g.cell_counts(),
+        runs.append((np.array(e), [g.get_field(i) for i in range(12)], g.cell_counts(),
                      sorted_particles(g.particles())))
         assert g.status()["flags"] == 0
         g.close()
