@@ -161,13 +161,19 @@ def yee_roofline(ncell, yee_ms, width):
     """Field stages of one step (H half step, H second half, E update, periodic
     ghosts, J fold) against the HBM roofline.  The fused minimum per cell reads
     E, H and J once and writes E and H once: 5 x 3 x 8 = 120 B in FP64; in the
-    FP32 form (FP32 E and H, int64 J) 4 x 3 x 4 + 3 x 8 = 72 B."""
+    FP32 form (FP32 E and H, int64 J) 4 x 3 x 4 + 3 x 8 = 72 B.  The step's
+    kernels also keep the reference's Q between the two H half steps and zero
+    J before the push: k_h_half moves 120 / 72 B per node (FP64 / FP32) and
+    k_h2_e 168 / 84 B, reported beside the minimum as the bytes moved."""
     per_cell = 120 if width == 8 else 72
+    moved = 288 if width == 8 else 156
     if yee_ms <= 0:
         return None
     achieved = ncell * per_cell / (yee_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes_per_cell": per_cell,
+            "bytes_moved_per_cell": moved,
+            "achieved_moved": ncell * moved / (yee_ms * 1e-3) / 1e9,
             "ms_per_step": yee_ms}
 
 
