@@ -45,7 +45,7 @@ struct AdjArgs32 {
 };
 
 /* adjoint of one vb_seg (k_particles_f32.hip) */
-template <int BY, int BZ, int NB>
+template <int BY, int BZ, int JY0, int JZ0>
 __device__ __forceinline__ void seg_adj(const float *sL, int ca, int cb, int cc, int rdx, int rdy,
                                         int rdz, float ax, float ay, float az, float bx, float by,
                                         float bz, const float *kj, float gA[3], float gB[3]) {
@@ -58,7 +58,7 @@ __device__ __forceinline__ void seg_adj(const float *sL, int ca, int cb, int cc,
                 sxy = dx * dy * (1.0f / 12.0f);
     const float qx = kj[0] * dx, qy = kj[1] * dy, qz = kj[2] * dz;
     const int n = ((ca + rdx) * BY + (cb + rdy)) * BZ + (cc + rdz);
-    const float *Lx = sL, *Ly = sL + NB, *Lz = sL + 2 * NB;
+    const float *Lx = sL, *Ly = sL + JY0, *Lz = sL + JZ0;
     const float a0 = Lx[n], a1 = Lx[n + 1], a2 = Lx[n + BZ], a3 = Lx[n + BZ + 1];
     const float b0 = Ly[n], b1 = Ly[n + 1], b2 = Ly[n + BY * BZ], b3 = Ly[n + BY * BZ + 1];
     const float c0 = Lz[n], c1 = Lz[n + BZ], c2 = Lz[n + BY * BZ], c3 = Lz[n + BY * BZ + BZ];
@@ -136,10 +136,17 @@ template <int TX, int TY, int TZ>
 __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
-    __shared__ float sF[6 * NB];
-    __shared__ float sL[3 * NB];
-    __shared__ double sA[ACOPIES * 6 * NB];
-    double *const myA = sA + (threadIdx.x % ACOPIES) * (6 * NB);
+    /* the push's box layout: Ex, Hy, Hz (node-type along x) take x planes
+     * 0..TX only (NBX nodes), the others all BX planes; the adjoint current
+     * box has the E boxes' layout (Jx takes x planes 0..TX) */
+    constexpr int NBX = (TX + 1) * BY * BZ;
+    constexpr int FB[6] = {0, NBX, NBX + NB, NBX + 2 * NB, NBX + 3 * NB, 2 * NBX + 3 * NB};
+    constexpr int NF = 3 * NBX + 3 * NB, NJ = NBX + 2 * NB;
+    constexpr int JY0 = FB[1], JZ0 = FB[2];
+    __shared__ float sF[NF];
+    __shared__ float sL[NJ];
+    __shared__ double sA[ACOPIES * NF];
+    double *const myA = sA + (threadIdx.x % ACOPIES) * NF;
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
 
@@ -167,8 +174,10 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         }
         sCnt[c] = cnt;
     }
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        int fi = e / NB, nn = e % NB;
+    for (int e = tid; e < NF; e += PIC_BLOCK) {
+        int fi = 0;
+        for (int q = 1; q < 6; q++) fi += e >= FB[q] ? 1 : 0;
+        const int nn = e - FB[fi];
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
         int gu = I0 + u, gv = L0 + v, gw = K0 + w;
         float val = 0.0f, lj = 0.0f;
@@ -180,7 +189,7 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         sF[e] = val;
         if (fi < 3) sL[e] = lj;
     }
-    for (int e = tid; e < ACOPIES * 6 * NB; e += PIC_BLOCK) sA[e] = 0.0;
+    for (int e = tid; e < ACOPIES * NF; e += PIC_BLOCK) sA[e] = 0.0;
     __syncthreads();
     if (tid == 0) {
         int s = 0;
@@ -216,12 +225,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
         const float ox = o[0], oy = o[1], oz = o[2];
         float Ef[3], Hf[3];
-        Ef[0] = tri<BY, BZ>(sF + 0 * NB, ca, cb + rdy, cc + rdz, nx0, ox, cy0, cy1, cz0, cz1);
-        Ef[1] = tri<BY, BZ>(sF + 1 * NB, ca + rdx, cb, cc + rdz, cx0, cx1, ny0, oy, cz0, cz1);
-        Ef[2] = tri<BY, BZ>(sF + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cx1, cy0, cy1, nz0, oz);
-        Hf[0] = tri<BY, BZ>(sF + 3 * NB, ca + rdx, cb, cc, cx0, cx1, ny0, oy, nz0, oz);
-        Hf[1] = tri<BY, BZ>(sF + 4 * NB, ca, cb + rdy, cc, nx0, ox, cy0, cy1, nz0, oz);
-        Hf[2] = tri<BY, BZ>(sF + 5 * NB, ca, cb, cc + rdz, nx0, ox, ny0, oy, cz0, cz1);
+        Ef[0] = tri<BY, BZ>(sF + FB[0], ca, cb + rdy, cc + rdz, ox, cy1, cz1);
+        Ef[1] = tri<BY, BZ>(sF + FB[1], ca + rdx, cb, cc + rdz, cx1, oy, cz1);
+        Ef[2] = tri<BY, BZ>(sF + FB[2], ca + rdx, cb + rdy, cc, cx1, cy1, oz);
+        Hf[0] = tri<BY, BZ>(sF + FB[3], ca + rdx, cb, cc, cx1, oy, oz);
+        Hf[1] = tri<BY, BZ>(sF + FB[4], ca, cb + rdy, cc, ox, cy1, oz);
+        Hf[2] = tri<BY, BZ>(sF + FB[5], ca, cb, cc + rdz, ox, oy, cz1);
         const Boris32 B = boris32(pu, pv, pw, Ef[0], Ef[1], Ef[2], Hf[0], Hf[1], Hf[2], tau1, a.tdh);
         const float o1[3] = {end_offset(ox, B.du[0]), end_offset(oy, B.du[1]),
                              end_offset(oz, B.du[2])};
@@ -238,14 +247,14 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         {
             float gA[3], gB[3];
             const float e1[3] = {m ? R.x2[0] : o1[0], m ? R.x2[1] : o1[1], m ? R.x2[2] : o1[2]};
-            seg_adj<BY, BZ, NB>(sL, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1[0], e1[1], e1[2],
+            seg_adj<BY, BZ, JY0, JZ0>(sL, ca, cb, cc, rdx, rdy, rdz, ox, oy, oz, e1[0], e1[1], e1[2],
                                 kj, gA, gB);
             for (int d = 0; d < 3; d++) lo[d] = gA[d];
             if (m == 0) {
                 for (int d = 0; d < 3; d++) lo1[d] += gB[d];
             } else {
                 float hA[3], hB[3];
-                seg_adj<BY, BZ, NB>(sL, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
+                seg_adj<BY, BZ, JY0, JZ0>(sL, ca, cb, cc, R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1],
                                     R.x2[2], o1[0], o1[1], o1[2], kj, hA, hB);
                 for (int d = 0; d < 3; d++) lo1[d] += hB[d];
                 const float lx2[3] = {gB[0] + hA[0], gB[1] + hA[1], gB[2] + hA[2]};
@@ -258,10 +267,14 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
                     lsr += lx2[d] * B.du[d];
                 }
                 const int nc = (int)cr[0] + (int)cr[1] + (int)cr[2];
+                /* (the crossed axes by an unrolled loop: a computed index
+                 * would put lo, ldu and du in LDS, a slice per thread) */
                 if (nc == 1) {
-                    const int d = cr[0] ? 0 : (cr[1] ? 1 : 2);
-                    lo[d] += lsr * (-1.0f / B.du[d]);
-                    ldu[d] += lsr * (-R.sr / B.du[d]);
+                    for (int d = 0; d < 3; d++) {
+                        if (!cr[d]) continue;
+                        lo[d] += lsr * (-1.0f / B.du[d]);
+                        ldu[d] += lsr * (-R.sr / B.du[d]);
+                    }
                 } else if (nc == 2) {
                     for (int d = 0; d < 3; d++) {
                         if (!cr[d]) continue;
@@ -323,12 +336,12 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         /* gather adjoint: every near weight has d/do = -1 */
         {
             float dx = 0.0f, dy = 0.0f, dz = 0.0f;
-            tri_adj<BY, BZ>(sF + 0 * NB, myA + 0 * NB, ca, cb + rdy, cc + rdz, nx0, cy0, cz0, lE[0], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 1 * NB, myA + 1 * NB, ca + rdx, cb, cc + rdz, cx0, ny0, cz0, lE[1], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 2 * NB, myA + 2 * NB, ca + rdx, cb + rdy, cc, cx0, cy0, nz0, lE[2], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 3 * NB, myA + 3 * NB, ca + rdx, cb, cc, cx0, ny0, nz0, lH[0], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 4 * NB, myA + 4 * NB, ca, cb + rdy, cc, nx0, cy0, nz0, lH[1], dx, dy, dz);
-            tri_adj<BY, BZ>(sF + 5 * NB, myA + 5 * NB, ca, cb, cc + rdz, nx0, ny0, cz0, lH[2], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[0], myA + FB[0], ca, cb + rdy, cc + rdz, nx0, cy0, cz0, lE[0], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[1], myA + FB[1], ca + rdx, cb, cc + rdz, cx0, ny0, cz0, lE[1], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[2], myA + FB[2], ca + rdx, cb + rdy, cc, cx0, cy0, nz0, lE[2], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[3], myA + FB[3], ca + rdx, cb, cc, cx0, ny0, nz0, lH[0], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[4], myA + FB[4], ca, cb + rdy, cc, nx0, cy0, nz0, lH[1], dx, dy, dz);
+            tri_adj<BY, BZ>(sF + FB[5], myA + FB[5], ca, cb, cc + rdz, nx0, ny0, cz0, lH[2], dx, dy, dz);
             lo[0] -= dx;
             lo[1] -= dy;
             lo[2] -= dz;
@@ -338,11 +351,13 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles32(AdjArgs32 a) {
         lq[2] = make_double2(lpm[1], lpm[2]);
     }
     __syncthreads();
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
+    for (int e = tid; e < NF; e += PIC_BLOCK) {
         double val = sA[e];
-        for (int r = 1; r < ACOPIES; r++) val += sA[r * 6 * NB + e];
+        for (int r = 1; r < ACOPIES; r++) val += sA[r * NF + e];
         if (val == 0.0) continue;
-        int fi = e / NB, nn = e % NB;
+        int fi = 0;
+        for (int q = 1; q < 6; q++) fi += e >= FB[q] ? 1 : 0;
+        const int nn = e - FB[fi];
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
         int gu = I0 + u, gv = L0 + v, gw = K0 + w;
         if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;

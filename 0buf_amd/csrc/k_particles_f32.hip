@@ -80,7 +80,7 @@ __device__ __forceinline__ void add64(long long *p, long long v) {
  * in the J box: the 12 VB node contributions of pqr (cell.h:1511-1693) in
  * cell units,
  *   Jx = (m hx/tau) du (Wy Wz +- dv dw/12)  on (x dual, y/z centres), etc. */
-template <int BY, int BZ, int NB>
+template <int BY, int BZ, int JY0, int JZ0>
 __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, int rdz, float ax,
                                        float ay, float az, float bx, float by, float bz,
                                        const float *cj) {
@@ -92,7 +92,7 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
     const float syz = dy * dz * (1.0f / 12.0f), sxz = dx * dz * (1.0f / 12.0f),
                 sxy = dx * dy * (1.0f / 12.0f);
     const double qx = (double)(cj[0] * dx), qy = (double)(cj[1] * dy), qz = (double)(cj[2] * dz);
-    long long *Jx = sJ, *Jy = sJ + NB, *Jz = sJ + 2 * NB;
+    long long *Jx = sJ, *Jy = sJ + JY0, *Jz = sJ + JZ0;
     add64(&Jx[n], fixmul(qx, ly * lz + syz));
     add64(&Jx[n + 1], fixmul(qx, ly * mz - syz));
     add64(&Jx[n + BZ], fixmul(qx, my * lz - syz));
@@ -112,16 +112,18 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
  * slots [bc + PIC_INBOX, cnt) by line (arrivals past a full inbox).
  * Packed: the particle-array part [0, bc), which starts at slot ps of the
  * tile's region, split where the region's lines begin; the inbox records by
- * 4; the spill records [bc + PIC_INBOX, cnt) by SPG (they lie in cell order
+ * IBG; the spill records [bc + PIC_INBOX, cnt) by SPG (they lie in cell order
  * in the tile's sorted spill list). */
 constexpr int SPG = 128;
+constexpr int IBG = 8;    /* inbox records per group: two 128-byte lines */
 __device__ __forceinline__ int cell_groups32(int cnt, int bc) {
     const int nin = min(PIC_INBOX, cnt - bc), t0 = bc + PIC_INBOX;
-    return (bc + GS32 - 1) / GS32 + (nin + 3) / 4 + (cnt > t0 ? (cnt - 1) / GS32 - t0 / GS32 + 1 : 0);
+    return (bc + GS32 - 1) / GS32 + (nin + IBG - 1) / IBG +
+           (cnt > t0 ? (cnt - 1) / GS32 - t0 / GS32 + 1 : 0);
 }
 __device__ __forceinline__ int cell_groups_packed(int ps, int bc, int nin, int nsp) {
     const int nb = bc > 0 ? (ps + bc - 1) / GS32 - ps / GS32 + 1 : 0;
-    return nb + (nin + 3) / 4 + (nsp + SPG - 1) / SPG;
+    return nb + (nin + IBG - 1) / IBG + (nsp + SPG - 1) / SPG;
 }
 
 template <int C>
@@ -131,7 +133,8 @@ __device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, 
     for (int w = tid; w < 4 * C; w += PIC_BLOCK) {
         const int c = w >> 2, part = w & 3;
         const int cnt = sCnt[c], bc = sBc[c];
-        const int nin = min(PIC_INBOX, cnt - bc), ninb = (nin + 3) / 4, t0 = bc + PIC_INBOX;
+        const int nin = min(PIC_INBOX, cnt - bc), ninb = (nin + IBG - 1) / IBG,
+                  t0 = bc + PIC_INBOX;
         const int ng = sG[c + 1] - sG[c];
         const int ps = sBase[c];
         const int nsoa = packed ? (bc > 0 ? (ps + bc - 1) / GS32 - ps / GS32 + 1 : 0)
@@ -149,8 +152,8 @@ __device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, 
                 }
             } else if (k < nsoa + ninb) {
                 kind = 1;
-                start = bc + 4 * (k - nsoa);
-                size = min(4, bc + nin - start);
+                start = bc + IBG * (k - nsoa);
+                size = min(IBG, bc + nin - start);
             } else if (packed) {
                 kind = 1;
                 start = t0 + SPG * (k - nsoa - ninb);
@@ -171,13 +174,23 @@ template <int TX, int TY, int TZ, bool DIFF>
 __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
+    /* Field boxes in LDS, one after another: a component interpolated as
+     * node-type along x (Ex, Hy, Hz) reads x planes 0..TX only, so its box
+     * stops one plane short (NBX nodes); the others take all BX planes.
+     * Likewise the current box: Jx (x-dual) takes x planes 0..TX, Jy, Jz
+     * all of them.  Fewer LDS bytes per workgroup: 6 per CU. */
+    constexpr int NBX = (TX + 1) * BY * BZ;
+    constexpr int FB[6] = {0, NBX, NBX + NB, NBX + 2 * NB, NBX + 3 * NB, 2 * NBX + 3 * NB};
+    constexpr int NF = 3 * NBX + 3 * NB;
+    constexpr int JY0 = NBX, JZ0 = NBX + NB, NJ = NBX + 2 * NB;
     constexpr int NW = PIC_BLOCK / 64;
-    __shared__ float sF[6 * NB];
-    /* J box copies JS elements apart: JS = 3 NB rounded to 8 mod 16, so
+    __shared__ float sF[NF];
+    __shared__ float sD[NF];   /* z differences sF[n+1] - sF[n] (tri_dz) */
+    /* J box copies JS elements apart: JS = NJ rounded up to 8 mod 16, so
      * that copy 1 sits 16 (of 32) or 48 (of 64) dword banks from copy 0 and
      * the banks one deposit instruction touches in the two copies (the 8
      * dual-cell offsets of a cell, 2 banks each) do not overlap */
-    constexpr int JS = (3 * NB + 7) / 16 * 16 + PIC_JPAD32;
+    constexpr int JS = (NJ + 15) / 16 * 16 + PIC_JPAD32;
     __shared__ long long sJ[JCOPIES * JS];
     __shared__ int sOff[C + 1];
     __shared__ int sCnt[C];
@@ -236,14 +249,20 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         sIB[c] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
     const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        int fi = e / NB, nn = e % NB;
+    for (int e = tid; e < NF; e += PIC_BLOCK) {
+        int fi = 0;
+        for (int q = 1; q < 6; q++) fi += e >= FB[q] ? 1 : 0;
+        const int nn = e - FB[fi];
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
         int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        float val = 0.0f;
-        if (gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1)
-            val = Fsrc[fi][(long)gu * g.sx + (long)gv * g.sy + gw];
+        float val = 0.0f, nxt = 0.0f;
+        if (gu <= g.Nx + 1 && gv <= g.Ny + 1) {
+            const float *src = Fsrc[fi] + ((long)gu * g.sx + (long)gv * g.sy + gw);
+            if (gw <= g.Nz + 1) val = src[0];
+            if (w + 1 < BZ && gw + 1 <= g.Nz + 1) nxt = src[1];
+        }
         sF[e] = val;
+        sD[e] = nxt - val;   /* the next node's box value (0 outside the grid) minus this */
     }
     for (int e = tid; e < JCOPIES * JS; e += PIC_BLOCK) sJ[e] = 0;
     for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
@@ -374,7 +393,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     auto deposit_pending = [&]() {
         const float4 sp4 = sSpec[pmeta >> 11];
         const float cjp[3] = {sp4.y, sp4.z, sp4.w};
-        vb_seg<BY, BZ, NB>(myJ, pmeta & 255, (pmeta >> 8) & 1, (pmeta >> 9) & 1,
+        vb_seg<BY, BZ, JY0, JZ0>(myJ, pmeta & 255, (pmeta >> 8) & 1, (pmeta >> 9) & 1,
                            (pmeta >> 10) & 1, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], cjp);
     };
     unsigned long long nstarts = 0, ncstarts = 0;
@@ -413,17 +432,19 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             const int rdx = W.rd[0], rdy = W.rd[1], rdz = W.rd[2];
             const float cx0 = W.c0[0], cx1 = 1.0f - cx0, cy0 = W.c0[1], cy1 = 1.0f - cy0,
                         cz0 = W.c0[2], cz1 = 1.0f - cz0;
-            const float nx0 = W.n0[0], ny0 = W.n0[1], nz0 = W.n0[2];
             /* node of the owner cell in the boxes (24-bit multiplies: full
              * rate, where a 32-bit one is a quarter) and the dual offsets */
             const int n0 = (int)__umul24(__umul24((unsigned)ca, BY) + (unsigned)cb, BZ) + cc;
             const int dX = rdx ? BY * BZ : 0, dY = rdy ? BZ : 0, dZ = rdz;
-            const float Ex = tri_at<BY, BZ>(sF + 0 * NB + n0 + dY + dZ, nx0, ox, cy0, cy1, cz0, cz1);
-            const float Ey = tri_at<BY, BZ>(sF + 1 * NB + n0 + dX + dZ, cx0, cx1, ny0, oy, cz0, cz1);
-            const float Ez = tri_at<BY, BZ>(sF + 2 * NB + n0 + dX + dY, cx0, cx1, cy0, cy1, nz0, oz);
-            const float Hx = tri_at<BY, BZ>(sF + 3 * NB + n0 + dX, cx0, cx1, ny0, oy, nz0, oz);
-            const float Hy = tri_at<BY, BZ>(sF + 4 * NB + n0 + dY, nx0, ox, cy0, cy1, nz0, oz);
-            const float Hz = tri_at<BY, BZ>(sF + 5 * NB + n0 + dZ, nx0, ox, ny0, oy, cz0, cz1);
+            const int e0 = FB[0] + n0 + dY + dZ, e1 = FB[1] + n0 + dX + dZ,
+                      e2 = FB[2] + n0 + dX + dY;
+            const int h0 = FB[3] + n0 + dX, h1 = FB[4] + n0 + dY, h2 = FB[5] + n0 + dZ;
+            const float Ex = tri_dz<BY, BZ>(sF + e0, sD + e0, ox, cy1, cz1);
+            const float Ey = tri_dz<BY, BZ>(sF + e1, sD + e1, cx1, oy, cz1);
+            const float Ez = tri_dz<BY, BZ>(sF + e2, sD + e2, cx1, cy1, oz);
+            const float Hx = tri_dz<BY, BZ>(sF + h0, sD + h0, cx1, oy, oz);
+            const float Hy = tri_dz<BY, BZ>(sF + h1, sD + h1, ox, cy1, oz);
+            const float Hz = tri_dz<BY, BZ>(sF + h2, sD + h2, ox, oy, cz1);
 
             const float4 spc = sSpec[s];
             const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, spc.x, a.tdh);
@@ -437,10 +458,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                 const int m = R.m;
                 const float cj[3] = {spc.y, spc.z, spc.w};
                 const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
-                vb_seg<BY, BZ, NB>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
+                vb_seg<BY, BZ, JY0, JZ0>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
                                    e1z, cj);
                 if (m && !DEFER_SEG) {
-                    vb_seg<BY, BZ, NB>(myJ,
+                    vb_seg<BY, BZ, JY0, JZ0>(myJ,
                                        n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2],
                                        R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1], R.x2[2], o1x,
                                        o1y, o1z, cj);
@@ -533,11 +554,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         }
     }
 
-    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
+    for (int e = tid; e < NJ; e += PIC_BLOCK) {
         long long val = sJ[e];
         for (int r = 1; r < JCOPIES; r++) val += sJ[r * JS + e];
         if (val == 0) continue;
-        int comp = e / NB, nn = e % NB;
+        const int comp = (e >= JY0 ? 1 : 0) + (e >= JZ0 ? 1 : 0);
+        const int nn = e - (comp == 0 ? 0 : comp == 1 ? JY0 : JZ0);
         int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
         int gu = I0 + u, gv = L0 + v, gw = K0 + w;
         if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
@@ -997,11 +1019,15 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     a.snap_on = diff && diff->p[0] != nullptr;
     a.snap_off = snap_off;
     a.snap_room = snap_room;
-    /* dynamic LDS: one 4-byte entry per group (per cell: particle-array
-     * lines before and after the inbox, one more for a packed part that
-     * starts inside a line, inbox lines; per tile: spill groups) */
-    const size_t lds = (size_t)4 * (TXC * TYC * TZC * (2 * ((p.cap + GS32 - 1) / GS32) +
-                                                     PIC_INBOX / 4 + 2) +
+    /* dynamic LDS: one 4-byte entry per group.  Per cell of count <= cap:
+     * particle-array lines at most ceil(cap/32) + 1 (buckets: the lines of
+     * [0, bc) and of [bc + PIC_INBOX, cnt), together cnt - PIC_INBOX slots
+     * when the inbox is full; packed: a run of bc slots that may start
+     * inside a line), inbox groups at most PIC_INBOX / IBG, spill groups at
+     * most nsp / SPG + 1 (at most scap / SPG + 1 for the tile's
+     * cells together beside the per-cell one) */
+    const size_t lds = (size_t)4 * (TXC * TYC * TZC * ((p.cap + GS32 - 1) / GS32 + 1 +
+                                                     PIC_INBOX / IBG + 1) +
                                     L.scap / SPG + 1);
     if (diff)
         hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),

@@ -15,26 +15,35 @@
 
 #include "pic_internal.h"
 
+/* Trilinear interpolation from the corner node's address p = F + (u BY + v)
+ * BZ + w, with the far weights x1, y1, z1 (node-type axes: the offset o;
+ * centre-type axes: 1 - the near weight):
+ *   each z pair  a = p0 + z1 (p1 - p0),  then  y pairs,  then x,
+ * every step one fmaf.  dp holds the z differences p[n+1] - p[n] of the same
+ * box (the push keeps them in LDS beside the values, so a z pair is one
+ * instruction); tri() forms them from F itself, with the same FP32
+ * subtraction, so both give the same value bit for bit. */
 template <int BY, int BZ>
-__device__ __forceinline__ float tri(const float *F, int u, int v, int w, float x0, float x1,
-                                     float y0, float y1, float z0, float z1) {
-    const float *p = F + (u * BY + v) * BZ + w;
-    float a = z0 * p[0] + z1 * p[1];
-    float b = z0 * p[BZ] + z1 * p[BZ + 1];
-    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
-    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
-    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
+__device__ __forceinline__ float tri_dz(const float *p, const float *dp, float x1, float y1,
+                                        float z1) {
+    const float a = fmaf(z1, dp[0], p[0]);
+    const float b = fmaf(z1, dp[BZ], p[BZ]);
+    const float c = fmaf(z1, dp[BY * BZ], p[BY * BZ]);
+    const float d = fmaf(z1, dp[BY * BZ + BZ], p[BY * BZ + BZ]);
+    const float ab = fmaf(y1, b - a, a), cd = fmaf(y1, d - c, c);
+    return fmaf(x1, cd - ab, ab);
 }
 
-/* the same from the corner node's address: p = F + (u BY + v) BZ + w */
 template <int BY, int BZ>
-__device__ __forceinline__ float tri_at(const float *p, float x0, float x1, float y0, float y1,
-                                        float z0, float z1) {
-    float a = z0 * p[0] + z1 * p[1];
-    float b = z0 * p[BZ] + z1 * p[BZ + 1];
-    float c = z0 * p[BY * BZ] + z1 * p[BY * BZ + 1];
-    float d = z0 * p[BY * BZ + BZ] + z1 * p[BY * BZ + BZ + 1];
-    return x0 * (y0 * a + y1 * b) + x1 * (y0 * c + y1 * d);
+__device__ __forceinline__ float tri(const float *F, int u, int v, int w, float x1, float y1,
+                                     float z1) {
+    const float *p = F + (u * BY + v) * BZ + w;
+    const float a = fmaf(z1, p[1] - p[0], p[0]);
+    const float b = fmaf(z1, p[BZ + 1] - p[BZ], p[BZ]);
+    const float c = fmaf(z1, p[BY * BZ + 1] - p[BY * BZ], p[BY * BZ]);
+    const float d = fmaf(z1, p[BY * BZ + BZ + 1] - p[BY * BZ + BZ], p[BY * BZ + BZ]);
+    const float ab = fmaf(y1, b - a, a), cd = fmaf(y1, d - c, c);
+    return fmaf(x1, cd - ab, ab);
 }
 
 struct Weights32 {
