@@ -81,36 +81,40 @@ struct Boris32 {
 __device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float Ex, float Ey,
                                            float Ez, float Hx, float Hy, float Hz, float tau1,
                                            const float tdh[3]) {
+    /* Every multiply-add is an explicit fmaf or a lone product: the particle
+     * adjoint recomputes this in another kernel, and its decisions (dual
+     * cells, crossing code, relay point) must see the same du bit for bit,
+     * whatever the compiler would contract in either kernel. */
     Boris32 B;
-    pu += tau1 * Ex;
-    pv += tau1 * Ey;
-    pw += tau1 * Ez;
+    pu = fmaf(tau1, Ex, pu);
+    pv = fmaf(tau1, Ey, pv);
+    pw = fmaf(tau1, Ez, pw);
     B.pm[0] = pu;
     B.pm[1] = pv;
     B.pm[2] = pw;
-    const float ps = tau1 * __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+    const float ps = tau1 * __frsqrt_rn(fmaf(pw, pw, fmaf(pv, pv, fmaf(pu, pu, 1.0f))));
     B.ps = ps;
     const float bx = ps * Hx, by = ps * Hy, bz = ps * Hz;
     B.b[0] = bx;
     B.b[1] = by;
     B.b[2] = bz;
-    const float su = pu + pv * bz - pw * by;
-    const float sv = pv + pw * bx - pu * bz;
-    const float sw = pw + pu * by - pv * bx;
+    const float su = fmaf(-pw, by, fmaf(pv, bz, pu));
+    const float sv = fmaf(-pu, bz, fmaf(pw, bx, pv));
+    const float sw = fmaf(-pv, bx, fmaf(pu, by, pw));
     B.s[0] = su;
     B.s[1] = sv;
     B.s[2] = sw;
     const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
     const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-    const float rden = __builtin_amdgcn_rcpf(q1 + 1.0f + q2 + q3);
+    const float rden = __builtin_amdgcn_rcpf(((q1 + 1.0f) + q2) + q3);
     B.rden = rden;
-    pu = ((q1 + 1.0f) * su + (q4 + bz) * sv + (q6 - by) * sw) * rden + tau1 * Ex;
-    pv = ((q4 - bz) * su + (q2 + 1.0f) * sv + (q5 + bx) * sw) * rden + tau1 * Ey;
-    pw = ((q6 + by) * su + (q5 - bx) * sv + (q3 + 1.0f) * sw) * rden + tau1 * Ez;
+    pu = fmaf(fmaf((q6 - by), sw, fmaf((q4 + bz), sv, (q1 + 1.0f) * su)), rden, tau1 * Ex);
+    pv = fmaf(fmaf((q5 + bx), sw, fmaf((q2 + 1.0f), sv, (q4 - bz) * su)), rden, tau1 * Ey);
+    pw = fmaf(fmaf((q3 + 1.0f), sw, fmaf((q5 - bx), sv, (q6 + by) * su)), rden, tau1 * Ez);
     B.p1[0] = pu;
     B.p1[1] = pv;
     B.p1[2] = pw;
-    const float gi2 = __frsqrt_rn(pu * pu + pv * pv + pw * pw + 1.0f);
+    const float gi2 = __frsqrt_rn(fmaf(pw, pw, fmaf(pv, pv, fmaf(pu, pu, 1.0f))));
     B.gi2 = gi2;
     B.du[0] = step_du(tdh[0], gi2, pu);
     B.du[1] = step_du(tdh[1], gi2, pv);
@@ -157,17 +161,18 @@ __device__ __forceinline__ Relay32 relay32(float ox, float oy, float oz, float o
      * sr unused. */
     const float wx = cx ? hh[0] : 0.0f, wy = cy ? hh[1] : 0.0f, wz = cz ? hh[2] : 0.0f;
     const bool used = m != 0 && m != 7;
-    const float num = used ? (0.5f - oz) * duz * wz + (0.5f - oy) * duy * wy +
-                                 (0.5f - ox) * dux * wx
+    /* (explicit fmaf, as in boris32: the adjoint recomputes these) */
+    const float num = used ? fmaf((0.5f - ox) * dux, wx,
+                                  fmaf((0.5f - oy) * duy, wy, ((0.5f - oz) * duz) * wz))
                            : 0.0f;
-    const float den = used ? duz * duz * wz + duy * duy * wy + dux * dux * wx : 1.0f;
+    const float den = used ? fmaf(dux * dux, wx, fmaf(duy * duy, wy, (duz * duz) * wz)) : 1.0f;
     R.num = num;
     R.den = den;
     const float sr = num * __builtin_amdgcn_rcpf(den);
     R.sr = sr;
-    R.x2[0] = (m & 4) ? 0.5f : ox + dux * sr;
-    R.x2[1] = (m & 2) ? 0.5f : oy + duy * sr;
-    R.x2[2] = (m & 1) ? 0.5f : oz + duz * sr;
+    R.x2[0] = (m & 4) ? 0.5f : fmaf(dux, sr, ox);
+    R.x2[1] = (m & 2) ? 0.5f : fmaf(duy, sr, oy);
+    R.x2[2] = (m & 1) ? 0.5f : fmaf(duz, sr, oz);
     return R;
 }
 
