@@ -56,6 +56,7 @@ struct PushArgs32 {
     float hh[3];                   /* h_d^2, weights of the two-face relay projection */
     float tau;
     int dep_cap;
+    int ngrp;                      /* entries of the dynamic group table */
     DevStatus *st;
     PartPtrs32 snap;               /* differentiable form: record of the pre-step state */
     const long *snap_off;          /* record offset of each tile */
@@ -65,9 +66,16 @@ struct PushArgs32 {
 
 /* round-to-nearest int64 of a*b (|a*b| < 2^51) by the 1.5*2^52 shifter: the
  * product of two FP32 values is exact in FP64, so the fma rounds once */
+template <bool RAW>
 __device__ __forceinline__ long long fixmul(double a, float b) {
+    /* d = 1.5 2^52 + round(a b) lies in (2^52, 2^53) for |a b| < 2^51, so
+     * its bits are 0x4338000000000000 + round(a b).  RAW keeps the offset:
+     * the offset's low 48 bits are 0, so a sum of such words carries the
+     * exact sum of the values in its low 48 bits while that sum stays
+     * below 2^47 in magnitude (the launcher checks the bound) */
     const double d = fma(a, (double)b, 6755399441055744.0);
-    return (long long)(__double_as_longlong(d) - 0x4338000000000000LL);
+    return RAW ? __double_as_longlong(d)
+               : (long long)(__double_as_longlong(d) - 0x4338000000000000LL);
 }
 
 __device__ __forceinline__ void add64(long long *p, long long v) {
@@ -80,7 +88,7 @@ __device__ __forceinline__ void add64(long long *p, long long v) {
  * in the J box: the 12 VB node contributions of pqr (cell.h:1511-1693) in
  * cell units,
  *   Jx = (m hx/tau) du (Wy Wz +- dv dw/12)  on (x dual, y/z centres), etc. */
-template <int BY, int BZ, int JY0, int JZ0>
+template <int BY, int BZ, int JY0, int JZ0, bool RAW>
 __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, int rdz, float ax,
                                        float ay, float az, float bx, float by, float bz,
                                        const float *cj) {
@@ -93,18 +101,18 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
                 sxy = dx * dy * (1.0f / 12.0f);
     const double qx = (double)(cj[0] * dx), qy = (double)(cj[1] * dy), qz = (double)(cj[2] * dz);
     long long *Jx = sJ, *Jy = sJ + JY0, *Jz = sJ + JZ0;
-    add64(&Jx[n], fixmul(qx, ly * lz + syz));
-    add64(&Jx[n + 1], fixmul(qx, ly * mz - syz));
-    add64(&Jx[n + BZ], fixmul(qx, my * lz - syz));
-    add64(&Jx[n + BZ + 1], fixmul(qx, my * mz + syz));
-    add64(&Jy[n], fixmul(qy, lx * lz + sxz));
-    add64(&Jy[n + 1], fixmul(qy, lx * mz - sxz));
-    add64(&Jy[n + BY * BZ], fixmul(qy, mx * lz - sxz));
-    add64(&Jy[n + BY * BZ + 1], fixmul(qy, mx * mz + sxz));
-    add64(&Jz[n], fixmul(qz, lx * ly + sxy));
-    add64(&Jz[n + BZ], fixmul(qz, lx * my - sxy));
-    add64(&Jz[n + BY * BZ], fixmul(qz, mx * ly - sxy));
-    add64(&Jz[n + BY * BZ + BZ], fixmul(qz, mx * my + sxy));
+    add64(&Jx[n], fixmul<RAW>(qx, ly * lz + syz));
+    add64(&Jx[n + 1], fixmul<RAW>(qx, ly * mz - syz));
+    add64(&Jx[n + BZ], fixmul<RAW>(qx, my * lz - syz));
+    add64(&Jx[n + BZ + 1], fixmul<RAW>(qx, my * mz + syz));
+    add64(&Jy[n], fixmul<RAW>(qy, lx * lz + sxz));
+    add64(&Jy[n + 1], fixmul<RAW>(qy, lx * mz - sxz));
+    add64(&Jy[n + BY * BZ], fixmul<RAW>(qy, mx * lz - sxz));
+    add64(&Jy[n + BY * BZ + 1], fixmul<RAW>(qy, mx * mz + sxz));
+    add64(&Jz[n], fixmul<RAW>(qz, lx * ly + sxy));
+    add64(&Jz[n + BZ], fixmul<RAW>(qz, lx * my - sxy));
+    add64(&Jz[n + BY * BZ], fixmul<RAW>(qz, mx * ly - sxy));
+    add64(&Jz[n + BY * BZ + BZ], fixmul<RAW>(qz, mx * my + sxy));
 }
 
 /* Line groups of one cell, in slot order.  Buckets: lines of [0, bc),
@@ -170,7 +178,7 @@ __device__ __forceinline__ void build_groups32(const int *sCnt, const int *sBc, 
 
 /* 7 waves per SIMD (at most 72 VGPRs): about 1% faster than 8 at 64 VGPRs,
  * which spill scalar registers */
-template <int TX, int TY, int TZ, bool DIFF>
+template <int TX, int TY, int TZ, bool DIFF, bool RAW>
 __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     constexpr int C = TX * TY * TZ;
     constexpr int BX = TX + 2, BY = TY + 2, BZ = TZ + 2, NB = BX * BY * BZ;
@@ -292,7 +300,13 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             }
         }
         int tot_s, tot_g, tot_p;
-        const int so = wave_excl_scan(cnt, tot_s), sg = wave_excl_scan(ngc, tot_g);
+        const int so = wave_excl_scan(cnt, tot_s);
+        int sg = wave_excl_scan(ngc, tot_g);
+        if (tot_g > a.ngrp) {   /* past the table's bound (which holds): no group, fail */
+            if (tid == 0) atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
+            sg = 0;
+            tot_g = 0;
+        }
         const int spo = wave_excl_scan(nspc, tot_p);
         if (tid == 0 && tot_p > a.L.scap)   /* a full list is repaired before a push */
             atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
@@ -393,7 +407,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     auto deposit_pending = [&]() {
         const float4 sp4 = sSpec[pmeta >> 11];
         const float cjp[3] = {sp4.y, sp4.z, sp4.w};
-        vb_seg<BY, BZ, JY0, JZ0>(myJ, pmeta & 255, (pmeta >> 8) & 1, (pmeta >> 9) & 1,
+        vb_seg<BY, BZ, JY0, JZ0, RAW>(myJ, pmeta & 255, (pmeta >> 8) & 1, (pmeta >> 9) & 1,
                            (pmeta >> 10) & 1, pa[0], pa[1], pa[2], pb[0], pb[1], pb[2], cjp);
     };
     unsigned long long nstarts = 0, ncstarts = 0;
@@ -458,10 +472,10 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                 const int m = R.m;
                 const float cj[3] = {spc.y, spc.z, spc.w};
                 const float e1x = m ? R.x2[0] : o1x, e1y = m ? R.x2[1] : o1y, e1z = m ? R.x2[2] : o1z;
-                vb_seg<BY, BZ, JY0, JZ0>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
+                vb_seg<BY, BZ, JY0, JZ0, RAW>(myJ, n0 + dX + dY + dZ, rdx, rdy, rdz, ox, oy, oz, e1x, e1y,
                                    e1z, cj);
                 if (m && !DEFER_SEG) {
-                    vb_seg<BY, BZ, JY0, JZ0>(myJ,
+                    vb_seg<BY, BZ, JY0, JZ0, RAW>(myJ,
                                        n0 + (R.r1[0] ? BY * BZ : 0) + (R.r1[1] ? BZ : 0) + R.r1[2],
                                        R.r1[0], R.r1[1], R.r1[2], R.x2[0], R.x2[1], R.x2[2], o1x,
                                        o1y, o1z, cj);
@@ -557,6 +571,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     for (int e = tid; e < NJ; e += PIC_BLOCK) {
         long long val = sJ[e];
         for (int r = 1; r < JCOPIES; r++) val += sJ[r * JS + e];
+        if (RAW) val = (long long)((unsigned long long)val << 16) >> 16;   /* the low 48 bits */
         if (val == 0) continue;
         const int comp = (e >= JY0 ? 1 : 0) + (e >= JZ0 ? 1 : 0);
         const int nn = e - (comp == 0 ? 0 : comp == 1 ? JY0 : JZ0);
@@ -991,6 +1006,8 @@ inline unsigned blocks_for(long n) { return (unsigned)((n + PIC_BLOCK - 1) / PIC
 
 }  // namespace
 
+static bool raw_current_sums_exact(const PushArgs32 &a, int cap);
+
 void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, PartPtrs32 p,
                              PartPtrs32 out, Layout32 L, QueuePtrs32 q, SpeciesTable sp,
                              int dep_cap, DevStatus *st, const PartPtrs32 *diff,
@@ -1029,12 +1046,32 @@ void launch_push_deposit_f32(const Grid &g, const TileGeom &t, FieldPtrs32 f, Pa
     const size_t lds = (size_t)4 * (TXC * TYC * TZC * ((p.cap + GS32 - 1) / GS32 + 1 +
                                                      PIC_INBOX / IBG + 1) +
                                     L.scap / SPG + 1);
+    a.ngrp = (int)(lds / 4);
     if (diff)
-        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), lds, s, a);
+        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, true, false>),
+                           dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), lds, s, a);
+    else if (raw_current_sums_exact(a, p.cap))
+        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false, true>),
+                           dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), lds, s, a);
     else
-        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false>), dim3((unsigned)t.ntiles),
-                           dim3(PIC_BLOCK), lds, s, a);
+        hipLaunchKernelGGL((k_push_deposit32<TXC, TYC, TZC, false, false>),
+                           dim3((unsigned)t.ntiles), dim3(PIC_BLOCK), lds, s, a);
+}
+
+/* Whether the push may sum its current contributions as raw offset words
+ * (fixmul<true>): every sum it forms in the LDS J box must stay below 2^47
+ * units of 2^-40 in magnitude.  A node of the box takes contributions from
+ * the particles of at most 18 cells of the tile (2 x 3 x 3: the cells whose
+ * dual cell at the start or the end covers it), at most cap per cell; one
+ * particle's contributions to one node, over its two segments, come to at
+ * most |cj| |du| (1 + 1/48) < |cj| / 2 x 13/12 (|du| < 1/2 per axis, a
+ * weight at most 1 + 1/48), plus 1/2 unit of rounding for each of the two. */
+static bool raw_current_sums_exact(const PushArgs32 &a, int cap) {
+    double cjmax = 0.0;
+    for (int k = 0; k < PIC_MAX_SLOTS; k++)
+        for (int d = 0; d < 3; d++) cjmax = std::max(cjmax, std::fabs((double)a.cj[k][d]));
+    const double bound = 18.0 * (double)cap * (0.5 * (13.0 / 12.0) * cjmax + 1.0);
+    return bound < 0.99 * 140737488355328.0;   /* 2^47 */
 }
 
 void launch_arrivals_done32(const Grid &g, const TileGeom &t, PartPtrs32 p, Layout32 L,

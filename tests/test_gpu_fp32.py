@@ -189,6 +189,47 @@ def test_fp32_bucket_regrow_bit_identical(picmod, cfg0):
         assert np.array_equal(runs[0][3][k], runs[48][3][k]), k
 
 
+def raw_sum_cap_limit(params, parts):
+    """The largest bucket capacity at which the FP32 push sums its current
+    contributions as raw offset words (the launcher's bound: 18 cells of at
+    most cap particles, each adding at most |cj| / 2 x 13/12 + 1 units to a
+    node, below 0.99 x 2^47 units of 2^-40)"""
+    h = (params.lx / params.nx, params.ly / params.ny, params.lz / params.nz)
+    cjmax = max(float(np.float32(abs(sp[7]) * hd / params.tau * 2.0 ** 40))
+                for sp in parts for hd in h)
+    return int(0.99 * 2.0 ** 47 / (18.0 * (0.5 * (13.0 / 12.0) * cjmax + 1.0)))
+
+
+def test_fp32_raw_current_sums_bit_identical(picmod, cfg0):
+    """With buckets small enough for the bound the push sums its fixed-point
+    contributions as raw offset words and recovers each node's sum from the
+    low 48 bits; with larger buckets it subtracts the offset from every
+    contribution.  Both give the same integer sums: energies, fields and
+    counts bit for bit over 20 steps."""
+    params, parts = cfg0
+    lim = raw_sum_cap_limit(params, parts)
+    runs = {}
+    for cap in (lim, 2 * lim):
+        p = picmod.make_params((params.nx, params.ny, params.nz),
+                               (params.lx, params.ly, params.lz), params.tau,
+                               precision=picmod.FP32, bucket_capacity=cap)
+        g = picmod.Pic(p)
+        load_workload(g, parts)
+        e = np.array([g.step(nt) for nt in range(1, 21)])
+        st = g.status()
+        assert st["flags"] == 0
+        runs[cap] = (e, [g.get_field(i) for i in range(12)], g.cell_counts(), st)
+        g.close()
+    print(f"raw-sum capacity limit {lim}: end capacities {runs[lim][3]['bucket_capacity']}, "
+          f"{runs[2 * lim][3]['bucket_capacity']}")
+    assert runs[lim][3]["bucket_capacity"] == lim      # no regrow: raw sums throughout
+    assert runs[2 * lim][3]["bucket_capacity"] > lim
+    assert np.array_equal(runs[lim][0], runs[2 * lim][0])
+    for a, b in zip(runs[lim][1], runs[2 * lim][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(runs[lim][2], runs[2 * lim][2])
+
+
 def test_fp32_full_inbox_matches_fp64(picmod):
     """A cold beam crossing cell faces in bulk: every few steps the 200
     particles of a cell all move into the next one, more than the 16-record
