@@ -10,13 +10,16 @@
  *     into inbox records and spill lists that the next push reads;
  *   * positions held as FP32 offsets in [0,1) inside the owning cell, so the
  *     resolution is h * 2^-24 everywhere in the box;
- *   * FP32 gather and Boris push (the reference's Cayley form);
+ *   * FP32 gather, from field boxes staged in LDS beside their z
+ *     differences (a z pair of the trilinear interpolation is one fmaf), and
+ *     Boris push (the reference's Cayley form);
  *   * the reference's charge-conserving deposit (Villasenor-Buneman segments,
  *     relay on the dual-cell faces as in CurrentToMesh, cell.h:1306-1422)
  *     written in cell units, each contribution rounded once to int64 fixed
  *     point (2^-40) and summed with native 64-bit integer atomics (ds_add_u64
  *     in LDS, then global): the current is exact given the per-particle
- *     contributions and independent of summation order;
+ *     contributions and independent of summation order (in LDS as raw
+ *     offset words when the launcher's bound allows, see fixmul);
  *   * migration by the cell-local offset: a particle whose offset leaves
  *     [0,1) moves to the neighbouring cell (periodic), exactly one receiver.
  */
