@@ -626,6 +626,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         float4 B = QE[2 * j + 1];
         const unsigned key = __float_as_uint(B.z);
         const int h = key_halo(key);
+        if (h >= NB) {   /* (the loop above writes h < NB only) */
+            atomicOr(&sFlags, (unsigned)PIC_F_QUEUE_CORRUPT);
+            a.q.slot[tile * (long)a.q.qcap + j] = -1;
+            continue;
+        }
         const int r = sHalo[h] + key_rank(key);
         a.q.slot[tile * (long)a.q.qcap + j] = r;
         long cell, dt;
