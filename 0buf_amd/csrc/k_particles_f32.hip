@@ -195,8 +195,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     constexpr int NF = 3 * NBX + 3 * NB;
     constexpr int JY0 = NBX, JZ0 = NBX + NB, NJ = NBX + 2 * NB;
     constexpr int NW = PIC_BLOCK / 64;
-    __shared__ float sF[NF];
-    __shared__ float sD[NF];   /* z differences sF[n+1] - sF[n] (tri_dz) */
+    __shared__ float2 sF[NF];   /* (value, z difference to the next node) per box node */
     /* J box copies JS elements apart: JS = NJ rounded up to 8 mod 16, so
      * that copy 1 sits 16 (of 32) or 48 (of 64) dword banks from copy 0 and
      * the banks one deposit instruction touches in the two copies (the 8
@@ -272,8 +271,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             if (gw <= g.Nz + 1) val = src[0];
             if (w + 1 < BZ && gw + 1 <= g.Nz + 1) nxt = src[1];
         }
-        sF[e] = val;
-        sD[e] = nxt - val;   /* the next node's box value (0 outside the grid) minus this */
+        sF[e] = make_float2(val, nxt - val);   /* the next node's value (0 outside the grid) */
     }
     for (int e = tid; e < JCOPIES * JS; e += PIC_BLOCK) sJ[e] = 0;
     for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
@@ -456,12 +454,12 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             const int e0 = FB[0] + n0 + dY + dZ, e1 = FB[1] + n0 + dX + dZ,
                       e2 = FB[2] + n0 + dX + dY;
             const int h0 = FB[3] + n0 + dX, h1 = FB[4] + n0 + dY, h2 = FB[5] + n0 + dZ;
-            const float Ex = tri_dz<BY, BZ>(sF + e0, sD + e0, ox, cy1, cz1);
-            const float Ey = tri_dz<BY, BZ>(sF + e1, sD + e1, cx1, oy, cz1);
-            const float Ez = tri_dz<BY, BZ>(sF + e2, sD + e2, cx1, cy1, oz);
-            const float Hx = tri_dz<BY, BZ>(sF + h0, sD + h0, cx1, oy, oz);
-            const float Hy = tri_dz<BY, BZ>(sF + h1, sD + h1, ox, cy1, oz);
-            const float Hz = tri_dz<BY, BZ>(sF + h2, sD + h2, ox, oy, cz1);
+            const float Ex = tri_dz<BY, BZ>(sF + e0, ox, cy1, cz1);
+            const float Ey = tri_dz<BY, BZ>(sF + e1, cx1, oy, cz1);
+            const float Ez = tri_dz<BY, BZ>(sF + e2, cx1, cy1, oz);
+            const float Hx = tri_dz<BY, BZ>(sF + h0, cx1, oy, oz);
+            const float Hy = tri_dz<BY, BZ>(sF + h1, ox, cy1, oz);
+            const float Hz = tri_dz<BY, BZ>(sF + h2, ox, oy, cz1);
 
             const float4 spc = sSpec[s];
             const Boris32 B = boris32(pu, pv, pw, Ex, Ey, Ez, Hx, Hy, Hz, spc.x, a.tdh);

@@ -19,17 +19,17 @@
  * BZ + w, with the far weights x1, y1, z1 (node-type axes: the offset o;
  * centre-type axes: 1 - the near weight):
  *   each z pair  a = p0 + z1 (p1 - p0),  then  y pairs,  then x,
- * every step one fmaf.  dp holds the z differences p[n+1] - p[n] of the same
- * box (the push keeps them in LDS beside the values, so a z pair is one
- * instruction); tri() forms them from F itself, with the same FP32
- * subtraction, so both give the same value bit for bit. */
+ * every step one fmaf.  tri_dz reads a box of (value, z difference
+ * p[n+1] - p[n]) pairs (the push keeps it in LDS, so a z pair is one fmaf
+ * and one 8-byte read); tri() forms the differences from F itself, with the
+ * same FP32 subtraction, so both give the same value bit for bit. */
 template <int BY, int BZ>
-__device__ __forceinline__ float tri_dz(const float *p, const float *dp, float x1, float y1,
-                                        float z1) {
-    const float a = fmaf(z1, dp[0], p[0]);
-    const float b = fmaf(z1, dp[BZ], p[BZ]);
-    const float c = fmaf(z1, dp[BY * BZ], p[BY * BZ]);
-    const float d = fmaf(z1, dp[BY * BZ + BZ], p[BY * BZ + BZ]);
+__device__ __forceinline__ float tri_dz(const float2 *p, float x1, float y1, float z1) {
+    const float2 q0 = p[0], q1 = p[BZ], q2 = p[BY * BZ], q3 = p[BY * BZ + BZ];
+    const float a = fmaf(z1, q0.y, q0.x);
+    const float b = fmaf(z1, q1.y, q1.x);
+    const float c = fmaf(z1, q2.y, q2.x);
+    const float d = fmaf(z1, q3.y, q3.x);
     const float ab = fmaf(y1, b - a, a), cd = fmaf(y1, d - c, c);
     return fmaf(x1, cd - ab, ab);
 }
