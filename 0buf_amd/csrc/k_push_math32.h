@@ -66,10 +66,15 @@ __device__ __forceinline__ Weights32 weights32(float ox, float oy, float oz) {
     return W;
 }
 
-/* displacement in cell units, du = (tau/h) gi2 p, each product rounded */
+/* displacement in cell units, du = (tau/h) gi2 p, each product rounded.
+ * Built with -ffp-contract=fast the backend fuses a multiply into a
+ * following add whatever the source says, so du leaves through an empty asm
+ * statement (no instruction), which it cannot see through: the end offset
+ * o + du (end_offset) is then a plain add in every kernel that forms it. */
 __device__ __forceinline__ float step_du(float tdh, float gi2, float p) {
-#pragma clang fp contract(off)
-    return (tdh * gi2) * p;
+    float du = (tdh * gi2) * p;
+    asm volatile("" : "+v"(du));
+    return du;
 }
 
 /* Boris push, Cayley form (particle.h:160-202), FP32, with the
@@ -104,13 +109,16 @@ __device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float E
     B.s[0] = su;
     B.s[1] = sv;
     B.s[2] = sw;
-    const float q1 = bx * bx, q2 = by * by, q3 = bz * bz;
-    const float q4 = bx * by, q5 = by * bz, q6 = bz * bx;
-    const float rden = __builtin_amdgcn_rcpf(((q1 + 1.0f) + q2) + q3);
+    /* the rotation matrix 1 + b b^T +- [b]x, each entry one fmaf */
+    const float m11 = fmaf(bx, bx, 1.0f), m22 = fmaf(by, by, 1.0f), m33 = fmaf(bz, bz, 1.0f);
+    const float m12 = fmaf(bx, by, bz), m21 = fmaf(bx, by, -bz);
+    const float m23 = fmaf(by, bz, bx), m32 = fmaf(by, bz, -bx);
+    const float m31 = fmaf(bz, bx, by), m13 = fmaf(bz, bx, -by);
+    const float rden = __builtin_amdgcn_rcpf(fmaf(bz, bz, fmaf(by, by, m11)));
     B.rden = rden;
-    pu = fmaf(fmaf((q6 - by), sw, fmaf((q4 + bz), sv, (q1 + 1.0f) * su)), rden, tau1 * Ex);
-    pv = fmaf(fmaf((q5 + bx), sw, fmaf((q2 + 1.0f), sv, (q4 - bz) * su)), rden, tau1 * Ey);
-    pw = fmaf(fmaf((q3 + 1.0f), sw, fmaf((q5 - bx), sv, (q6 + by) * su)), rden, tau1 * Ez);
+    pu = fmaf(fmaf(m13, sw, fmaf(m12, sv, m11 * su)), rden, tau1 * Ex);
+    pv = fmaf(fmaf(m23, sw, fmaf(m22, sv, m21 * su)), rden, tau1 * Ey);
+    pw = fmaf(fmaf(m33, sw, fmaf(m32, sv, m31 * su)), rden, tau1 * Ez);
     B.p1[0] = pu;
     B.p1[1] = pv;
     B.p1[2] = pw;
@@ -124,14 +132,9 @@ __device__ __forceinline__ Boris32 boris32(float pu, float pv, float pw, float E
 
 /* the end offset o + du.  Every decision of the deposit and the migration
  * (end dual cell, crossing code, receiving cell) is taken from it, and the
- * particle adjoint recomputes them in another kernel: both products of du
- * and this sum are rounded one by one (no contraction into an FMA), so the
- * two kernels derive them from identical values whatever the compiler fuses
- * elsewhere. */
-__device__ __forceinline__ float end_offset(float o, float du) {
-#pragma clang fp contract(off)
-    return o + du;
-}
+ * particle adjoint recomputes them in another kernel: the products of du
+ * (step_du) and this sum are rounded one by one in both. */
+__device__ __forceinline__ float end_offset(float o, float du) { return o + du; }
 
 /* deposit geometry (CurrentToMesh, cell.h:1306-1422, in cell units): dual
  * cell of each end (floor(o + 1/2)), crossing code m and the relay point on
