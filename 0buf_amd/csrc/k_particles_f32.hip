@@ -119,7 +119,7 @@ __device__ __forceinline__ void vb_seg(long long *sJ, int n, int rdx, int rdy, i
 }
 
 /* Line groups of one cell, in slot order.  Buckets: lines of [0, bc),
- * inbox records [bc, bc + nin) by 4 (one 128-byte line of records), bucket
+ * inbox records [bc, bc + nin) by IBG (two 128-byte lines of records), bucket
  * slots [bc + PIC_INBOX, cnt) by line (arrivals past a full inbox).
  * Packed: the particle-array part [0, bc), which starts at slot ps of the
  * tile's region, split where the region's lines begin; the inbox records by
