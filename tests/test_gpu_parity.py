@@ -146,6 +146,45 @@ def test_fused_field_pass_equals_stages(picmod, cfg0, ghost_mode):
         assert np.array_equal(a, b), NAMES[i]
 
 
+@pytest.mark.parametrize("n", [(3, 3, 3), (3, 5, 4), (5, 4, 3)])
+@pytest.mark.parametrize("form", ["fp32", "fp64-diff"])
+def test_fused_field_pass_small_grids(picmod, n, form):
+    """The step's fused field passes against the stage calls on grids of 3 to
+    5 cells per axis (every node on a fold or ghost layer, face classes with
+    Nx - 2 = 1), random particles, both ghost modes, 8 steps: energies and all
+    12 fields bit for bit (the FP32 form's and the differentiable form's
+    currents are order-free)."""
+    L = tuple(0.1 * k for k in n)
+    rng = np.random.default_rng(11)
+    N = 600
+    parts = []
+    for sort, qm, m in ((1, -1.0, -0.02), (0, 1 / 1836.0, 0.02)):
+        x = [rng.uniform(0, L[d], N) for d in range(3)]
+        p = rng.normal(0, 0.5, (3, N)) if sort == 1 else np.zeros((3, N))
+        parts.append((sort, x[0], x[1], x[2], p[0], p[1], p[2], m, qm))
+    names = ("yee_h_half", "push_deposit", "migrate", "fold_j", "yee_h2", "yee_e", "e_periodic")
+    kw = dict(precision=picmod.FP32) if form == "fp32" else dict(differentiable=1)
+    for gm in (0, 1):
+        runs = []
+        for mode in ("step", "stages"):
+            g = picmod.Pic(picmod.make_params(n, L, 0.05, departure_cap=0, ghost_mode=gm, **kw))
+            load_workload(g, parts)
+            e = []
+            for nt in range(1, 9):
+                if mode == "stages":
+                    for nm in names:
+                        g.stage(nm)
+                    g.sync()
+                    e.append(g.energy())
+                else:
+                    e.append(g.step(nt))
+            runs.append((np.array(e), [g.get_field(i) for i in range(12)]))
+            g.close()
+        assert np.array_equal(runs[0][0], runs[1][0]), gm
+        for i, (a, b) in enumerate(zip(runs[0][1], runs[1][1])):
+            assert np.array_equal(a, b), (gm, NAMES[i])
+
+
 def test_full_run_config0(picmod, orc, cfg0):
     """Config 0: 32x8x8 cells, 3 x 16 ppc, 100 steps."""
     params, parts = cfg0
