@@ -240,38 +240,56 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     const int cap = a.p.cap;
     const float4 *const SPB = a.L.spill_s + 2 * (tile * (long)a.L.scap);
 
-    for (int c = tid; c < C; c += PIC_BLOCK) {
-        int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
-        int cnt = 0, bc = 0, ps = 0;
-        if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+    /* The field boxes and the cells' counts.  Thread n takes box node n in
+     * all six component boxes (one decode, one 32-bit offset: the grid has
+     * fewer than 2^31 nodes) and cell n of the tile; every global load is
+     * issued before the first LDS write, from uniform base addresses, so
+     * the workgroup waits for one memory round trip here. */
+    static_assert(NB <= PIC_BLOCK && C <= PIC_BLOCK, "one box node and one cell per thread");
+    constexpr int FN[6] = {NBX, NB, NB, NB, NBX, NBX};   /* nodes in each box */
+    const float *const Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
+    float fv[6], fd[6];   /* node value, next node's value (0 outside the grid) */
+#pragma unroll
+    for (int q = 0; q < 6; q++) fv[q] = fd[q] = 0.0f;
+    {
+        const int u = tid / (BY * BZ), v = (tid / BZ) % BY, w = tid % BZ;
+        const int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        const bool in_uv = tid < NB && gu <= g.Nx + 1 && gv <= g.Ny + 1;
+        const unsigned off = (unsigned)gu * g.sx + (unsigned)gv * g.sy + gw;
+        if (in_uv && gw <= g.Nz + 1) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) fv[q] = Fsrc[q][off];
+        }
+        if (in_uv && w + 1 < BZ && gw + 1 <= g.Nz + 1) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) fd[q] = Fsrc[q][off + 1];
+        }
+    }
+    const int ca = tid / (TY * TZ), cb = (tid / TZ) % TY, cc = tid % TZ;
+    int cnt = 0, bc = 0, ps = 0;
+    {
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+        if (tid < C && gi < g.Nx && gl < g.Ny && gk < g.Nz) {
             const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
             cnt = a.p.count[cell];
-            cnt = cnt < cap ? cnt : cap;
-            bc = max(0, min(a.p.bcount[cell], cnt));
+            bc = a.p.bcount[cell];
             if (packed) ps = a.L.pstart[cell];
+            cnt = cnt < cap ? cnt : cap;
+            bc = max(0, min(bc, cnt));
             if (snap) a.snap.count[cell] = cnt;
         }
-        sCnt[c] = cnt;
-        sBc[c] = bc;
-        sLeave[c] = 0;
-        sBase[c] = packed ? ps : ((ca * g.Ny + cb) * g.Nz + cc) * cap;
-        sIB[c] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
-    const float *Fsrc[6] = {a.f.E[0], a.f.E[1], a.f.E[2], a.f.H[0], a.f.H[1], a.f.H[2]};
-    for (int e = tid; e < NF; e += PIC_BLOCK) {
-        int fi = 0;
-        for (int q = 1; q < 6; q++) fi += e >= FB[q] ? 1 : 0;
-        const int nn = e - FB[fi];
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        float val = 0.0f, nxt = 0.0f;
-        if (gu <= g.Nx + 1 && gv <= g.Ny + 1) {
-            const float *src = Fsrc[fi] + ((long)gu * g.sx + (long)gv * g.sy + gw);
-            if (gw <= g.Nz + 1) val = src[0];
-            if (w + 1 < BZ && gw + 1 <= g.Nz + 1) nxt = src[1];
-        }
-        sF[e] = make_float2(val, nxt - val);   /* the next node's value (0 outside the grid) */
+    if (tid < NB) {
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+            if (tid < FN[q]) sF[FB[q] + tid] = make_float2(fv[q], fd[q] - fv[q]);
+    }
+    if (tid < C) {
+        sCnt[tid] = cnt;
+        sBc[tid] = bc;
+        sLeave[tid] = 0;
+        sBase[tid] = packed ? ps : ((ca * g.Ny + cb) * g.Nz + cc) * cap;
+        sIB[tid] = ((ca * g.Ny + cb) * g.Nz + cc) * PIC_INBOX;
     }
     for (int e = tid; e < JCOPIES * JS; e += PIC_BLOCK) sJ[e] = 0;
     for (int h = tid; h < NB; h += PIC_BLOCK) sHalo[h] = 0;
@@ -569,17 +587,23 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
         }
     }
 
-    for (int e = tid; e < NJ; e += PIC_BLOCK) {
-        long long val = sJ[e];
-        for (int r = 1; r < JCOPIES; r++) val += sJ[r * JS + e];
-        if (RAW) val = (long long)((unsigned long long)val << 16) >> 16;   /* the low 48 bits */
-        if (val == 0) continue;
-        const int comp = (e >= JY0 ? 1 : 0) + (e >= JZ0 ? 1 : 0);
-        const int nn = e - (comp == 0 ? 0 : comp == 1 ? JY0 : JZ0);
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
-        add64(&a.f.J[comp][(long)gu * g.sx + (long)gv * g.sy + gw], val);
+    /* the current boxes: thread n adds box node n of the three components
+     * (Jx's box stops one x plane short) */
+    if (tid < NB) {
+        const int u = tid / (BY * BZ), v = (tid / BZ) % BY, w = tid % BZ;
+        const int gu = I0 + u, gv = L0 + v, gw = K0 + w;
+        const bool in = gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1;
+        const unsigned off = (unsigned)gu * g.sx + (unsigned)gv * g.sy + gw;
+        constexpr int J0[3] = {0, JY0, JZ0};
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            if (q == 0 && tid >= NBX) continue;
+            const int e = J0[q] + tid;
+            long long val = sJ[e];
+            for (int r = 1; r < JCOPIES; r++) val += sJ[r * JS + e];
+            if (RAW) val = (long long)((unsigned long long)val << 16) >> 16;   /* the low 48 bits */
+            if (val != 0 && in) add64(&a.f.J[q][off], val);
+        }
     }
 
     if (maxout > 0) {
