@@ -361,28 +361,41 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
     const int I0 = tx * TX, L0 = ty * TY, K0 = tz * TZ;
     const int cap = a.p.cap;
 
-    for (int c = tid; c < C; c += PIC_BLOCK) {
-        int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+    /* thread n takes box node n of every box (one decode, one 32-bit
+     * offset: the grid has fewer than 2^31 nodes) and cell n of the tile;
+     * every global load is issued before the first LDS write */
+    static_assert(NB <= PIC_BLOCK && C <= PIC_BLOCK, "one box node and one cell per thread");
+    const double *const Fsrc[6] = {a.fwd.E[0], a.fwd.E[1], a.fwd.E[2],
+                                   a.fwd.H[0], a.fwd.H[1], a.fwd.H[2]};
+    const int nu = tid / (BY * BZ), nv = (tid / BZ) % BY, nw = tid % BZ;
+    const bool node_in = tid < NB && I0 + nu <= g.Nx + 1 && L0 + nv <= g.Ny + 1 &&
+                         K0 + nw <= g.Nz + 1;
+    const unsigned noff = (unsigned)(I0 + nu) * g.sx + (unsigned)(L0 + nv) * g.sy + (K0 + nw);
+    {
+        double fv[6], lj[3];
+#pragma unroll
+        for (int q = 0; q < 6; q++) fv[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < 3; q++) lj[q] = 0.0;
+        if (node_in) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) fv[q] = Fsrc[q][noff];
+#pragma unroll
+            for (int q = 0; q < 3; q++) lj[q] = a.lam.J[q][noff];
+        }
+        const int gi = I0 + tid / (TY * TZ), gl = L0 + (tid / TZ) % TY, gk = K0 + tid % TZ;
         int cnt = 0;
-        if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+        if (tid < C && gi < g.Nx && gl < g.Ny && gk < g.Nz) {
             cnt = a.p.count[((long)gi * g.Ny + gl) * g.Nz + gk];
             cnt = cnt < cap ? cnt : cap;
         }
-        sCnt[c] = cnt;
-    }
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        int fi = e / NB, nn = e % NB;
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        double val = 0.0, lj = 0.0;
-        if (gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1) {
-            const long gidx = (long)gu * g.sx + (long)gv * g.sy + gw;
-            val = fi < 3 ? a.fwd.E[fi][gidx] : a.fwd.H[fi - 3][gidx];
-            if (fi < 3) lj = a.lam.J[fi][gidx];
+        if (tid < NB) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) sF[q * NB + tid] = fv[q];
+#pragma unroll
+            for (int q = 0; q < 3; q++) sL[q * NB + tid] = lj[q];
         }
-        sF[e] = val;
-        if (fi < 3) sL[e] = lj;
+        if (tid < C) sCnt[tid] = cnt;
     }
     for (int e = tid; e < ACOPIES * 6 * NB; e += PIC_BLOCK) sA[e] = 0.0;
     if (tid == 0) sFlags = 0;
@@ -584,16 +597,15 @@ __global__ __launch_bounds__(PIC_BLOCK) void k_adj_particles(AdjArgs a) {
     }
     if (bad) atomicOr(&sFlags, (unsigned)PIC_F_RANGE);
     __syncthreads();
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        double val = sA[e];
-        for (int r = 1; r < ACOPIES; r++) val += sA[r * 6 * NB + e];
-        if (val == 0.0) continue;
-        int fi = e / NB, nn = e % NB;
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
-        const long gidx = (long)gu * g.sx + (long)gv * g.sy + gw;
-        atomicAdd(fi < 3 ? &a.lam.E[fi][gidx] : &a.lam.H[fi - 3][gidx], val);
+    if (tid < NB) {   /* thread n adds box node n of the six adjoint boxes */
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int e = q * NB + tid;
+            double val = sA[e];
+            for (int r = 1; r < ACOPIES; r++) val += sA[r * 6 * NB + e];
+            if (val != 0.0 && node_in)
+                atomicAdd(q < 3 ? &a.lam.E[q][noff] : &a.lam.H[q - 3][noff], val);
+        }
     }
     if (tid == 0 && sFlags) atomicOr(&a.st->flags, sFlags);
 }

@@ -149,29 +149,41 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     const int K0 = __builtin_amdgcn_readfirstlane(tz * TZ);
     const int cap = a.p.cap;
 
-    for (int c = tid; c < C; c += PIC_BLOCK) {
-        int ca = c / (TY * TZ), cb = (c / TZ) % TY, cc = c % TZ;
-        int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
+    /* Thread n takes box node n of the six field boxes (one decode, one
+     * 32-bit offset: the grid has fewer than 2^31 nodes) and cell n of the
+     * tile; every global load is issued before the first LDS write. */
+    static_assert(NB <= PIC_BLOCK && C <= PIC_BLOCK, "one box node and one cell per thread");
+    const int nu = tid / (BY * BZ), nv = (tid / BZ) % BY, nw = tid % BZ;
+    const bool node_in = tid < NB && I0 + nu <= g.Nx + 1 && L0 + nv <= g.Ny + 1 &&
+                         K0 + nw <= g.Nz + 1;
+    const unsigned noff = (unsigned)(I0 + nu) * g.sx + (unsigned)(L0 + nv) * g.sy + (K0 + nw);
+    {
+        double fv[6];
+#pragma unroll
+        for (int q = 0; q < 6; q++) fv[q] = 0.0;
+        if (node_in) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) fv[q] = a.f.f[PIC_EX + q][noff];
+        }
+        const int ca = tid / (TY * TZ), cb = (tid / TZ) % TY, cc = tid % TZ;
+        const int gi = I0 + ca, gl = L0 + cb, gk = K0 + cc;
         int cnt = 0;
-        if (gi < g.Nx && gl < g.Ny && gk < g.Nz) {
+        if (tid < C && gi < g.Nx && gl < g.Ny && gk < g.Nz) {
             const long cell = ((long)gi * g.Ny + gl) * g.Nz + gk;
             cnt = a.p.count[cell];
             cnt = cnt < cap ? cnt : cap;
             if (snap) a.d.snap.count[cell] = cnt;
         }
-        sCnt[c] = cnt;
-        sLeave[c] = 0;
-        sOut[c] = 0;
-        sBase[c] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
-    }
-    for (int e = tid; e < 6 * NB; e += PIC_BLOCK) {
-        int fi = e / NB, nn = e % NB;
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        double val = 0.0;
-        if (gu <= g.Nx + 1 && gv <= g.Ny + 1 && gw <= g.Nz + 1)
-            val = a.f.f[PIC_EX + fi][(long)gu * g.sx + (long)gv * g.sy + gw];
-        sF[e] = val;
+        if (tid < NB) {
+#pragma unroll
+            for (int q = 0; q < 6; q++) sF[q * NB + tid] = fv[q];
+        }
+        if (tid < C) {
+            sCnt[tid] = cnt;
+            sLeave[tid] = 0;
+            sOut[tid] = 0;
+            sBase[tid] = ((ca * g.Ny + cb) * g.Nz + cc) * cap;
+        }
     }
     for (int e = tid; e < JCOPIES * 3 * NB; e += PIC_BLOCK) sJ[e] = 0.0;
     /* weight and q/m per species slot: a kernel-argument table indexed by a
@@ -521,23 +533,21 @@ __global__ __launch_bounds__(PIC_BLOCK, 4) void k_push_deposit(PushArgs a) {
     }
 
     /* tile J box -> global J (GPU_WriteAllCurrents, kernels.cu:83-112) */
-    for (int e = tid; e < 3 * NB; e += PIC_BLOCK) {
-        int comp = e / NB, nn = e % NB;
-        int u = nn / (BY * BZ), v = (nn / BZ) % BY, w = nn % BZ;
-        int gu = I0 + u, gv = L0 + v, gw = K0 + w;
-        if (gu > g.Nx + 1 || gv > g.Ny + 1 || gw > g.Nz + 1) continue;
-        const long gidx = (long)gu * g.sx + (long)gv * g.sy + gw;
-        if constexpr (DIFF) {
-            const unsigned long long *u = reinterpret_cast<const unsigned long long *>(sJ);
-            unsigned long long val = u[e];
-            for (int r = 1; r < JCOPIES; r++) val += u[r * 3 * NB + e];
-            if (val == 0ull) continue;
-            atomicAdd(reinterpret_cast<unsigned long long *>(a.d.jfix[comp]) + gidx, val);
-        } else {
-            double val = sJ[e];
-            for (int r = 1; r < JCOPIES; r++) val += sJ[r * 3 * NB + e];
-            if (val == 0.0) continue;
-            atomicAdd(&a.f.f[PIC_JX + comp][gidx], val);
+    if (tid < NB) {   /* thread n adds box node n of the three components */
+#pragma unroll
+        for (int comp = 0; comp < 3; comp++) {
+            const int e = comp * NB + tid;
+            if constexpr (DIFF) {
+                const unsigned long long *u = reinterpret_cast<const unsigned long long *>(sJ);
+                unsigned long long val = u[e];
+                for (int r = 1; r < JCOPIES; r++) val += u[r * 3 * NB + e];
+                if (val != 0ull && node_in)
+                    atomicAdd(reinterpret_cast<unsigned long long *>(a.d.jfix[comp]) + noff, val);
+            } else {
+                double val = sJ[e];
+                for (int r = 1; r < JCOPIES; r++) val += sJ[r * 3 * NB + e];
+                if (val != 0.0 && node_in) atomicAdd(&a.f.f[PIC_JX + comp][noff], val);
+            }
         }
     }
 
