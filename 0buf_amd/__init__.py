@@ -133,6 +133,15 @@ def load_library(path=LIB_PATH):
         raise RuntimeError(
             f"{path} is missing: build it with `python 0buf_amd/build.py` "
             "(there is no CPU fallback for the PIC step)")
+    # One HIP runtime per process.  PyTorch ships its own libamdhip64 and
+    # libhsa-runtime64; loaded first, they satisfy the library's dependencies
+    # (same sonames), so torch tensors and the library share one runtime.
+    # Loaded after the library, torch maps its copies beside the system
+    # ones and finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     dp, ip, lp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_long)
     vp = C.c_void_p
