@@ -398,12 +398,14 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
             qpu = slot(P3, nrel); qpv = slot(P4, nrel); qpw = slot(P5, nrel);
             qs = slot(SP, nrel);
             if (DIFF) qid = slot(ID, nrel);
+            asm volatile("" ::: "memory");
         } else {
             const float4 A = slot(RB, ir);
             const float2 B = *(const float2 *)&slot(RB, ir + 1u);
             qox = A.x; qoy = A.y; qoz = A.z; qpu = A.w; qpv = B.x; qpw = B.y;
             qs = *((const unsigned char *)&slot(RB, ir + 1u) + 8);   /* key & 255 */
             if (DIFF) qid = *((const int *)&slot(RB, ir + 1u) + 3);
+            asm volatile("" ::: "memory");
         }
     };
     const int ghi = __builtin_amdgcn_readfirstlane(sG[sFirst[wv + 1]]);
