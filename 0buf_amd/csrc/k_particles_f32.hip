@@ -370,7 +370,11 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
      * issued (that would wait for those loads): the addresses are formed
      * before the branch (the empty asm statements keep the compiler from
      * sinking them into it), and the record path loads the species byte of
-     * the record's key word by itself rather than masking the loaded word. */
+     * the record's key word by itself rather than masking the loaded word.
+     * Each path ends in an empty asm statement with a memory clobber, so the
+     * compiler cannot sink the two paths' loads into one shared set: that
+     * set took a 64-bit address per lane and load, and split the records'
+     * 16- and 8-byte loads into 4-byte ones. */
     float qox = 0.f, qoy = 0.f, qoz = 0.f, qpu = 0.f, qpv = 0.f, qpw = 0.f;
     unsigned char qs = 0;
     int qid = 0;
