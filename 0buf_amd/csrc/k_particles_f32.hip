@@ -359,6 +359,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                  *const O4 = a.o.p[4] + obase, *const O5 = a.o.p[5] + obase;
     unsigned char *const OSP = a.o.sp + obase;
     int *const OID = DIFF ? a.o.id + obase : nullptr;
+    float4 *const QT = a.q.e + 2 * (tile * (long)a.q.qcap);   /* the tile's departure queue */
 
     /* the wavefront's next chunk is composed and its particles loaded one
      * chunk ahead, so the memory latency overlaps this chunk's arithmetic */
@@ -543,9 +544,9 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
                 const int qs_ = atomicAdd(&sQ, 1);
                 if (qs_ < a.q.qcap) {
                     const unsigned key = queue_key(s, (unsigned)h, (unsigned)hr);
-                    float4 *q = a.q.e + 2 * (tile * (long)a.q.qcap + qs_);
-                    q[0] = make_float4(nxo, nyo, nzo, pu);
-                    q[1] = make_float4(pv, pw, __uint_as_float(key), __int_as_float(pid));
+                    slot(QT, 2u * (unsigned)qs_) = make_float4(nxo, nyo, nzo, pu);
+                    slot(QT, 2u * (unsigned)qs_ + 1u) =
+                        make_float4(pv, pw, __uint_as_float(key), __int_as_float(pid));
                 }
             }
         }
@@ -648,7 +649,7 @@ __global__ __launch_bounds__(PIC_BLOCK, 7) void k_push_deposit32(PushArgs32 a) {
     }
     __syncthreads();
     const int nq = min(sQ, a.q.qcap);
-    const float4 *const QE = a.q.e + 2 * (tile * (long)a.q.qcap);
+    const float4 *const QE = QT;
     for (int j = tid; j < nq; j += PIC_BLOCK) {
         const float4 A = QE[2 * j];
         float4 B = QE[2 * j + 1];
