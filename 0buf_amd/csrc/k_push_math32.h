@@ -165,10 +165,13 @@ __device__ __forceinline__ Relay32 relay32(float ox, float oy, float oz, float o
     const float wx = cx ? hh[0] : 0.0f, wy = cy ? hh[1] : 0.0f, wz = cz ? hh[2] : 0.0f;
     const bool used = m != 0 && m != 7;
     /* (explicit fmaf, as in boris32: the adjoint recomputes these) */
-    const float num = used ? fmaf((0.5f - ox) * dux, wx,
-                                  fmaf((0.5f - oy) * duy, wy, ((0.5f - oz) * duz) * wz))
-                           : 0.0f;
-    const float den = used ? fmaf(dux * dux, wx, fmaf(duy * duy, wy, (duz * duz) * wz)) : 1.0f;
+    /* both formed unconditionally and then selected: as conditional
+     * expressions the compiler branched around them, twice */
+    const float num_all = fmaf((0.5f - ox) * dux, wx,
+                               fmaf((0.5f - oy) * duy, wy, ((0.5f - oz) * duz) * wz));
+    const float den_all = fmaf(dux * dux, wx, fmaf(duy * duy, wy, (duz * duz) * wz));
+    const float num = used ? num_all : 0.0f;
+    const float den = used ? den_all : 1.0f;
     R.num = num;
     R.den = den;
     const float sr = num * __builtin_amdgcn_rcpf(den);
